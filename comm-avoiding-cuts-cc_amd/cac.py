@@ -1,0 +1,214 @@
+"""ctypes binding of libcac.so (the C ABI declared in include/cac.h).
+
+This is the host-side mirror used by the tests and bench.py; the C++ command
+line tools link the same library directly.  Loading fails loudly when the
+library has not been built: there is no CPU fallback anywhere on this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libcac.so"
+
+CAC_OK = 0
+CAC_EINVAL = -1
+CAC_ENOMEM = -2
+CAC_EDEVICE = -3
+CAC_ENOTCONN = -4
+CAC_ERANGE = -5
+CAC_EIO = -6
+LABEL_MIN_ID = 0
+LABEL_RANK = 1
+
+# Edge records, identical to cac_uedge / cac_edge.
+UEDGE = np.dtype([("u", "<u4"), ("v", "<u4")])
+EDGE = np.dtype([("u", "<u4"), ("v", "<u4"), ("w", "<u8")])
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cac_ctx_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "cac_ctx_destroy": (_i32, [_vp]),
+    "cac_strerror": (ctypes.c_char_p, [_i32]),
+    "cac_last_error": (ctypes.c_char_p, [_vp]),
+    "cac_ctx_stream": (_vp, [_vp]),
+    "cac_ctx_synchronize": (_i32, [_vp]),
+    "cac_dev_alloc": (_i32, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "cac_dev_free": (_i32, [_vp, _vp]),
+    "cac_memcpy_h2d": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "cac_memcpy_d2h": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "cac_prof_enable": (_i32, [_vp, _i32]),
+    "cac_prof_reset": (_i32, [_vp]),
+    "cac_prof_count": (_i32, [_vp]),
+    "cac_prof_get": (_i32, [_vp, _i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float)]),
+    "cac_cc": (_i32, [_vp, _vp, _u64, _u32, _i32, _pu32, _pu32]),
+    "cac_cc_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
+    "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
+}
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load libcac.so and declare every signature; raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"libcac.so not built at {p}: run __graft_entry__.build() (make -C comm-avoiding-cuts-cc_amd)")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class CacError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (for the _dev entry points)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        ctx._check(ctx.lib.cac_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        assert host.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.cac_memcpy_h2d(self.ctx.handle, self.ptr, _ptr(host), host.nbytes))
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.cac_memcpy_d2h(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(self.ctx.lib.cac_dev_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One cac_ctx on one device (not thread-safe, like the C object)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.cac_ctx_create(device, ctypes.byref(h))
+        if rc != CAC_OK:
+            raise CacError(rc, f"cac_ctx_create: {self.lib.cac_strerror(rc).decode()}")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self.lib.cac_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != CAC_OK:
+            msg = self.lib.cac_last_error(self.handle).decode() if self.handle else ""
+            raise CacError(rc, f"{self.lib.cac_strerror(rc).decode()}: {msg}")
+
+    # ---- device memory / timing ------------------------------------------
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    @property
+    def stream(self):
+        return self.lib.cac_ctx_stream(self.handle)
+
+    def synchronize(self):
+        self._check(self.lib.cac_ctx_synchronize(self.handle))
+
+    def prof_enable(self, on: bool = True):
+        self._check(self.lib.cac_prof_enable(self.handle, 1 if on else 0))
+
+    def prof_reset(self):
+        self._check(self.lib.cac_prof_reset(self.handle))
+
+    def prof_records(self):
+        out = []
+        for i in range(self.lib.cac_prof_count(self.handle)):
+            name = ctypes.c_char_p()
+            ms = ctypes.c_float()
+            self._check(self.lib.cac_prof_get(self.handle, i, ctypes.byref(name), ctypes.byref(ms)))
+            out.append((name.value.decode(), float(ms.value)))
+        return out
+
+    # ---- connected components --------------------------------------------
+    def cc(self, edges: np.ndarray, n: int, mode: int = LABEL_MIN_ID):
+        """edges: (m, 2) uint32 or UEDGE records.  Returns (labels, ncomp)."""
+        e = np.ascontiguousarray(edges).view(np.uint32).reshape(-1)
+        m = e.size // 2
+        labels = np.empty(n, dtype=np.uint32)
+        nc = ctypes.c_uint32()
+        self._check(self.lib.cac_cc(self.handle, _ptr(e), m, n, mode,
+                                    labels.ctypes.data_as(_pu32), ctypes.byref(nc)))
+        return labels, int(nc.value)
+
+    def cc_dev(self, d_edges: int, m: int, n: int, d_labels: int, mode: int = LABEL_MIN_ID, stream=None) -> int:
+        nc = ctypes.c_uint32()
+        self._check(self.lib.cac_cc_dev(self.handle, d_edges, m, n, mode, d_labels, ctypes.byref(nc), stream))
+        return int(nc.value)
+
+    # ---- generators -------------------------------------------------------
+    def gen_rmat_dev(self, scale: int, m: int, seed: int, thresholds, d_edges: int) -> int:
+        ta, tab, tabc = thresholds
+        used = ctypes.c_uint64()
+        self._check(self.lib.cac_gen_rmat_dev(self.handle, scale, m, seed, ta, tab, tabc, d_edges,
+                                              ctypes.byref(used)))
+        return int(used.value)
+
+
+def rmat_thresholds(a: float, b: float, c: float):
+    """Cumulative u32 thresholds floor(2^32 * x) for the R-MAT quadrant choice,
+    computed from decimal fractions exactly (a=0.45 -> 45/100)."""
+    from fractions import Fraction
+
+    fa, fb, fc = (Fraction(str(x)) for x in (a, b, c))
+    t = lambda f: int(f * (1 << 32))
+    return t(fa), t(fa + fb), t(fa + fb + fc)
+
+
+def declared_symbols(header: str | os.PathLike | None = None):
+    """Function names declared in include/cac.h (for the export check)."""
+    import re
+
+    h = Path(header) if header else PKG_DIR.parent / "include" / "cac.h"
+    text = h.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cac_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,205 @@
+// cac_ctx.hip -- context lifecycle, errors, device buffers and kernel timing.
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "cac_internal.h"
+
+namespace cac {
+
+int fail(cac_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+int hip_fail(cac_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return fail(ctx, CAC_ENOMEM, m);
+  return fail(ctx, CAC_EDEVICE, m);
+}
+
+int bind(cac_ctx* ctx) {
+  if (!ctx) return CAC_EINVAL;
+  CAC_HIP(ctx, hipSetDevice(ctx->device));
+  return CAC_OK;
+}
+
+int scratch_reserve(cac_ctx* ctx, size_t bytes, void** base) {
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) {
+      CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      CAC_HIP(ctx, hipDeviceSynchronize());
+      CAC_HIP(ctx, hipFree(ctx->scratch));
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&ctx->scratch, want);
+    if (e != hipSuccess) {
+      ctx->scratch = nullptr;
+      return hip_fail(ctx, e, "hipMalloc(scratch)");
+    }
+    ctx->scratch_bytes = want;
+  }
+  *base = ctx->scratch;
+  return CAC_OK;
+}
+
+static hipEvent_t take_event(cac_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void prof_begin(cac_ctx* ctx, const char* name, hipStream_t s) {
+  if (!ctx->profiling) return;
+  cac_prof_rec r{name, take_event(ctx), take_event(ctx)};
+  if (r.start) (void)hipEventRecord(r.start, s);
+  ctx->prof.push_back(r);
+}
+
+void prof_end(cac_ctx* ctx, hipStream_t s) {
+  if (!ctx->profiling || ctx->prof.empty()) return;
+  cac_prof_rec& r = ctx->prof.back();
+  if (r.stop) (void)hipEventRecord(r.stop, s);
+}
+
+}  // namespace cac
+
+extern "C" {
+
+int cac_ctx_create(int device, cac_ctx** out) {
+  if (!out) return CAC_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return CAC_EDEVICE;
+  if (device < 0 || device >= count) return CAC_EINVAL;
+  cac_ctx* ctx = new (std::nothrow) cac_ctx();
+  if (!ctx) return CAC_ENOMEM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return CAC_EDEVICE;
+  }
+  *out = ctx;
+  return CAC_OK;
+}
+
+int cac_ctx_destroy(cac_ctx* ctx) {
+  if (!ctx) return CAC_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->prof) {
+    if (r.start) (void)hipEventDestroy(r.start);
+    if (r.stop) (void)hipEventDestroy(r.stop);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CAC_OK;
+}
+
+const char* cac_strerror(int code) {
+  switch (code) {
+    case CAC_OK: return "ok";
+    case CAC_EINVAL: return "invalid argument";
+    case CAC_ENOMEM: return "out of memory";
+    case CAC_EDEVICE: return "device error";
+    case CAC_ENOTCONN: return "graph is not connected";
+    case CAC_ERANGE: return "weight sum overflows int64";
+    case CAC_EIO: return "input file cannot be read";
+    default: return "unknown error";
+  }
+}
+
+const char* cac_last_error(const cac_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+void* cac_ctx_stream(cac_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int cac_ctx_synchronize(cac_ctx* ctx) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int cac_dev_alloc(cac_ctx* ctx, size_t bytes, void** dptr) {
+  if (!dptr) return CAC_EINVAL;
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+  return CAC_OK;
+}
+
+int cac_dev_free(cac_ctx* ctx, void* dptr) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CAC_HIP(ctx, hipFree(dptr));
+  return CAC_OK;
+}
+
+int cac_memcpy_h2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!bytes) return CAC_OK;
+  if (!dst || !src) return CAC_EINVAL;
+  CAC_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int cac_memcpy_d2h(cac_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!bytes) return CAC_OK;
+  if (!dst || !src) return CAC_EINVAL;
+  CAC_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int cac_prof_enable(cac_ctx* ctx, int on) {
+  if (!ctx) return CAC_EINVAL;
+  ctx->profiling = on != 0;
+  return CAC_OK;
+}
+
+int cac_prof_reset(cac_ctx* ctx) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipDeviceSynchronize());
+  for (auto& r : ctx->prof) {
+    if (r.start) ctx->event_pool.push_back(r.start);
+    if (r.stop) ctx->event_pool.push_back(r.stop);
+  }
+  ctx->prof.clear();
+  return CAC_OK;
+}
+
+int cac_prof_count(cac_ctx* ctx) { return ctx ? (int)ctx->prof.size() : CAC_EINVAL; }
+
+int cac_prof_get(cac_ctx* ctx, int i, const char** name, float* ms) {
+  if (!ctx || i < 0 || i >= (int)ctx->prof.size()) return CAC_EINVAL;
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  cac_prof_rec& r = ctx->prof[(size_t)i];
+  if (!r.start || !r.stop) return cac::fail(ctx, CAC_EDEVICE, "event creation failed");
+  CAC_HIP(ctx, hipEventSynchronize(r.stop));
+  float t = 0.f;
+  CAC_HIP(ctx, hipEventElapsedTime(&t, r.start, r.stop));
+  if (name) *name = r.name;
+  if (ms) *ms = t;
+  return CAC_OK;
+}
+
+}  // extern "C"

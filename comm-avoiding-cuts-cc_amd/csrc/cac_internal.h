@@ -1,0 +1,78 @@
+// cac_internal.h -- context object, error plumbing and scratch arena shared by
+// the entry points of libcac.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "cac.h"
+
+struct cac_prof_rec {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct cac_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // Grow-only device scratch arena; entry points carve sub-buffers from it.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Small pinned host buffer for scalar results.
+  void* pinned = nullptr;
+  std::string last_error;
+  bool profiling = false;
+  std::vector<cac_prof_rec> prof;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace cac {
+
+// Records a failure message on the context and returns the code.
+int fail(cac_ctx* ctx, int code, const std::string& msg);
+int hip_fail(cac_ctx* ctx, hipError_t e, const char* what);
+
+// Makes the context's device current for the calling thread.
+int bind(cac_ctx* ctx);
+
+// Ensures the scratch arena holds at least 'bytes'; returns its base.
+int scratch_reserve(cac_ctx* ctx, size_t bytes, void** base);
+
+inline hipStream_t pick_stream(cac_ctx* ctx, void* s) {
+  return s ? (hipStream_t)s : ctx->stream;
+}
+
+// Carves aligned sub-buffers from a base pointer.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base((char*)b) {}
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = (T*)(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+inline size_t carve_size(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// Kernel timing: brackets one launch with events when profiling is enabled.
+void prof_begin(cac_ctx* ctx, const char* name, hipStream_t s);
+void prof_end(cac_ctx* ctx, hipStream_t s);
+
+}  // namespace cac
+
+#define CAC_HIP(ctx, call)                                  \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
+    if (_e != hipSuccess) return cac::hip_fail(ctx, _e, #call); \
+  } while (0)
+
+#define CAC_LAUNCH_CHECK(ctx, what)                          \
+  do {                                                       \
+    hipError_t _e = hipGetLastError();                       \
+    if (_e != hipSuccess) return cac::hip_fail(ctx, _e, what); \
+  } while (0)

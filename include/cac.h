@@ -1,0 +1,111 @@
+/*
+ * cac.h -- C ABI of the MI355X (gfx950) contraction core: connected components,
+ * weighted sampling contraction, dense Karger-Stein and the min-cut drivers.
+ *
+ * Conventions
+ *   - Every entry point returns CAC_OK (0) or a negative CAC_E* code; no C++
+ *     exception crosses this boundary.  cac_strerror() names a code and
+ *     cac_last_error() gives the message of the last failure on a context.
+ *   - Functions without the _dev suffix take and return HOST memory; the caller
+ *     owns every buffer, the context owns device scratch.  _dev variants take
+ *     device pointers and an optional hipStream_t passed as void* (NULL = the
+ *     context's own stream) and return once the results are on the host side
+ *     of the call (scalars) / enqueued (device arrays) as documented per call.
+ *   - A context is bound to one device and is not thread-safe: use one per
+ *     host thread.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef CAC_H
+#define CAC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define CAC_OK 0
+#define CAC_EINVAL (-1)   /* bad argument: null pointer, vertex id >= n, size out of range */
+#define CAC_ENOMEM (-2)   /* device or host allocation failed */
+#define CAC_EDEVICE (-3)  /* HIP runtime error */
+#define CAC_ENOTCONN (-4) /* graph is not connected (min-cut precondition) */
+#define CAC_ERANGE (-5)   /* a weight sum would overflow int64 */
+#define CAC_EIO (-6)      /* edge-list file cannot be opened or parsed */
+
+/* ---- edge records --------------------------------------------------------
+ * cac_uedge mirrors UnweightedGraph::Edge {unsigned from, to} (src/UnweightedGraph.hpp);
+ * cac_edge mirrors AdjacencyListGraph::Edge {unsigned from, to; unsigned long weight}
+ * (src/AdjacencyListGraph.hpp:13-50), 16 bytes. */
+typedef struct cac_uedge {
+  uint32_t u, v;
+} cac_uedge;
+
+typedef struct cac_edge {
+  uint32_t u, v;
+  uint64_t w;
+} cac_edge;
+
+typedef struct cac_ctx cac_ctx;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces the per-rank MPI setup of the reference CLIs (src/executables/,
+ * MPI_Init / communicator); one context per GPU and host thread. */
+int cac_ctx_create(int device, cac_ctx** out);
+int cac_ctx_destroy(cac_ctx* ctx);
+const char* cac_strerror(int code);
+const char* cac_last_error(const cac_ctx* ctx);
+/* The context's HIP stream (hipStream_t as void*). */
+void* cac_ctx_stream(cac_ctx* ctx);
+int cac_ctx_synchronize(cac_ctx* ctx);
+
+/* Device buffers for the _dev entry points (plain hipMalloc/hipMemcpy on the
+ * context's device; no counterpart in the reference, which has no device). */
+int cac_dev_alloc(cac_ctx* ctx, size_t bytes, void** dptr);
+int cac_dev_free(cac_ctx* ctx, void* dptr);
+int cac_memcpy_h2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes);
+int cac_memcpy_d2h(cac_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* Per-kernel timing with HIP events on the stream each kernel is launched on.
+ * No counterpart in the reference beyond TimeUtils::measure (src/utils.hpp:21-40). */
+int cac_prof_enable(cac_ctx* ctx, int on);
+int cac_prof_reset(cac_ctx* ctx);
+int cac_prof_count(cac_ctx* ctx);
+/* Synchronizes and returns name and duration (ms) of recorded launch i. */
+int cac_prof_get(cac_ctx* ctx, int i, const char** name, float* ms);
+
+/* ---- connected components ------------------------------------------------
+ * Replaces UnweightedIteratedSparseSampling::connectedComponents(std::vector<unsigned>&)
+ * (src/UnweightedIteratedSparseSampling.hpp:19, .cpp:40-72).  Undirected edges;
+ * loops and duplicates allowed; every id must be < n (CAC_EINVAL otherwise).
+ * label_mode CAC_LABEL_MIN_ID: labels[v] = smallest vertex id of v's component.
+ * label_mode CAC_LABEL_RANK:   labels[v] = rank of that smallest id among all
+ *                              components (the reference's [0, count) labels).
+ * *ncomp_out = number of components (the value parallel_cc prints). */
+#define CAC_LABEL_MIN_ID 0
+#define CAC_LABEL_RANK 1
+int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int label_mode,
+           uint32_t* labels_out, uint32_t* ncomp_out);
+/* Device variant: d_labels (n entries) is also the working array.  Ids are not
+ * validated (the caller guarantees < n).  Blocks until *ncomp_out is known. */
+int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
+               uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
+
+/* ---- synthetic inputs ----------------------------------------------------
+ * R-MAT edge list on the device (the reference generates R-MAT inputs with
+ * PaRMAT, input_generators/rmat_driver.sh:13-16).  Draw d makes 'scale' quadrant
+ * choices from Philox-4x32-10(ctr = {d_lo, level/4, d_hi, 0}, key = {seed, 'RMAT'}),
+ * word level%4, against the cumulative u32 thresholds ta < tab < tabc.  Loops
+ * are dropped, pairs canonicalised to (min, max), the first m distinct pairs in
+ * draw order kept, and d_edges (m entries) is written sorted by (u, v). */
+int cac_gen_rmat_dev(cac_ctx* ctx, uint32_t scale, uint64_t m, uint32_t seed, uint32_t ta,
+                     uint32_t tab, uint32_t tabc, cac_uedge* d_edges, uint64_t* draws_used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAC_H */

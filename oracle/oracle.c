@@ -1,0 +1,262 @@
+/*
+ * oracle.c -- CPU restatement of the contraction core (TEST INFRASTRUCTURE).
+ * See oracle.h: only tests/, smoke() and bench.py's cpu_baseline use this.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* Philox (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as    */
+/* 1, 2, 3", SC'11).  Round: multiply the even words by the round multipliers, */
+/* cross the high halves with the odd words and the key; Weyl key schedule.    */
+/* ------------------------------------------------------------------------- */
+#define PHILOX_M4x32_0 0xD2511F53u
+#define PHILOX_M4x32_1 0xCD9E8D57u
+#define PHILOX_W32_0 0x9E3779B9u
+#define PHILOX_W32_1 0xBB67AE85u
+#define PHILOX_M4x64_0 0xD2E7470EE14C6C93ull
+#define PHILOX_M4x64_1 0xCA5A826395121157ull
+#define PHILOX_W64_0 0x9E3779B97F4A7C15ull
+#define PHILOX_W64_1 0xBB67AE8584CAA73Bull
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += PHILOX_W32_0;
+            k1 += PHILOX_W32_1;
+        }
+        uint64_t p0 = (uint64_t)PHILOX_M4x32_0 * x0;
+        uint64_t p1 = (uint64_t)PHILOX_M4x32_1 * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+void orc_philox4x64_10(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+    uint64_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint64_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += PHILOX_W64_0;
+            k1 += PHILOX_W64_1;
+        }
+        __uint128_t p0 = (__uint128_t)PHILOX_M4x64_0 * x0;
+        __uint128_t p1 = (__uint128_t)PHILOX_M4x64_1 * x2;
+        uint64_t y0 = (uint64_t)(p1 >> 64) ^ x1 ^ k0;
+        uint64_t y1 = (uint64_t)p1;
+        uint64_t y2 = (uint64_t)(p0 >> 64) ^ x3 ^ k1;
+        uint64_t y3 = (uint64_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+uint64_t orc_uniform1(uint64_t x, uint64_t W) {
+    return 1u + (uint64_t)(((__uint128_t)x * W) >> 64);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Connected components: union by size + full path compression, then          */
+/* canonicalise each component to its minimum vertex id.                       */
+/* ------------------------------------------------------------------------- */
+static uint32_t uf_find(uint32_t* p, uint32_t x) {
+    uint32_t r = x;
+    while (p[r] != r) r = p[r];
+    while (p[x] != r) {
+        uint32_t nx = p[x];
+        p[x] = r;
+        x = nx;
+    }
+    return r;
+}
+
+uint32_t orc_cc(uint32_t n, uint64_t m, const uint32_t* edges, int mode, uint32_t* labels) {
+    uint32_t* p = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    uint32_t* sz = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    for (uint32_t i = 0; i < n; ++i) { p[i] = i; sz[i] = 1; }
+    for (uint64_t e = 0; e < m; ++e) {
+        uint32_t a = uf_find(p, edges[2 * e]), b = uf_find(p, edges[2 * e + 1]);
+        if (a == b) continue;
+        if (sz[a] < sz[b]) { uint32_t t = a; a = b; b = t; }
+        p[b] = a;
+        sz[a] += sz[b];
+    }
+    /* minimum id per root: vertices visited in increasing order, so the first
+     * vertex seen for a root is its minimum */
+    uint32_t* minid = sz; /* reuse */
+    for (uint32_t i = 0; i < n; ++i) minid[i] = UINT32_MAX;
+    uint32_t ncomp = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        uint32_t r = uf_find(p, v);
+        if (minid[r] == UINT32_MAX) {
+            minid[r] = (mode == 1) ? ncomp : v;
+            ++ncomp;
+        }
+        labels[v] = minid[r];
+    }
+    free(p);
+    free(sz);
+    return ncomp;
+}
+
+/* ------------------------------------------------------------------------- */
+/* R-MAT generator                                                            */
+/* ------------------------------------------------------------------------- */
+#define DOM_RMAT 0x524D4154u /* 'RMAT' */
+
+static void rmat_draw(uint32_t scale, uint64_t d, uint32_t seed, uint32_t ta, uint32_t tab,
+                      uint32_t tabc, uint32_t* u_out, uint32_t* v_out) {
+    uint32_t u = 0, v = 0;
+    uint32_t key[2] = {seed, DOM_RMAT};
+    uint32_t word[4] = {0, 0, 0, 0};
+    for (uint32_t lvl = 0; lvl < scale; ++lvl) {
+        if ((lvl & 3u) == 0) {
+            uint32_t ctr[4] = {(uint32_t)d, lvl >> 2, (uint32_t)(d >> 32), 0};
+            orc_philox4x32_10(ctr, key, word);
+        }
+        uint32_t r = word[lvl & 3u];
+        uint32_t bu, bv;
+        if (r < ta) { bu = 0; bv = 0; }
+        else if (r < tab) { bu = 0; bv = 1; }
+        else if (r < tabc) { bu = 1; bv = 0; }
+        else { bu = 1; bv = 1; }
+        u = (u << 1) | bu;
+        v = (v << 1) | bv;
+    }
+    *u_out = u;
+    *v_out = v;
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return (x > y) - (x < y);
+}
+
+uint64_t orc_rmat(uint32_t scale, uint64_t m, uint32_t seed, uint32_t ta, uint32_t tab,
+                  uint32_t tabc, uint32_t* edges_out) {
+    /* open-addressing hash set of canonical keys */
+    uint64_t cap = 16;
+    while (cap < 4 * m) cap <<= 1;
+    uint64_t* table = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+    for (uint64_t i = 0; i < cap; ++i) table[i] = UINT64_MAX;
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (m ? m : 1));
+    uint64_t have = 0, d = 0;
+    while (have < m) {
+        uint32_t u, v;
+        rmat_draw(scale, d, seed, ta, tab, tabc, &u, &v);
+        ++d;
+        if (u == v) continue;
+        uint64_t k = u < v ? (((uint64_t)u << 32) | v) : (((uint64_t)v << 32) | u);
+        uint64_t h = (k * 0x9E3779B97F4A7C15ull) >> 17;
+        for (;;) {
+            uint64_t slot = h & (cap - 1);
+            if (table[slot] == UINT64_MAX) {
+                table[slot] = k;
+                keys[have++] = k;
+                break;
+            }
+            if (table[slot] == k) break;
+            ++h;
+        }
+    }
+    qsort(keys, m, sizeof(uint64_t), cmp_u64);
+    for (uint64_t i = 0; i < m; ++i) {
+        edges_out[2 * i] = (uint32_t)(keys[i] >> 32);
+        edges_out[2 * i + 1] = (uint32_t)keys[i];
+    }
+    free(keys);
+    free(table);
+    return d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Exact minimum cut                                                          */
+/* ------------------------------------------------------------------------- */
+int64_t orc_stoer_wagner(uint32_t n, const int64_t* adj) {
+    if (n < 2) return 0;
+    int64_t* w = (int64_t*)malloc(sizeof(int64_t) * (size_t)n * n);
+    memcpy(w, adj, sizeof(int64_t) * (size_t)n * n);
+    uint32_t* alive = (uint32_t*)malloc(sizeof(uint32_t) * n); /* list of live vertices */
+    int64_t* key = (int64_t*)malloc(sizeof(int64_t) * n);
+    unsigned char* added = (unsigned char*)malloc(n);
+    uint32_t nl = n;
+    for (uint32_t i = 0; i < n; ++i) alive[i] = i;
+    int64_t best = INT64_MAX;
+    while (nl > 1) {
+        /* maximum adjacency ordering from alive[0] */
+        for (uint32_t i = 0; i < nl; ++i) { key[alive[i]] = 0; added[alive[i]] = 0; }
+        uint32_t prev = alive[0], last = alive[0];
+        added[last] = 1;
+        for (uint32_t i = 0; i < nl; ++i) key[alive[i]] += w[(size_t)last * n + alive[i]];
+        for (uint32_t step = 1; step < nl; ++step) {
+            uint32_t sel = UINT32_MAX;
+            for (uint32_t i = 0; i < nl; ++i) {
+                uint32_t x = alive[i];
+                if (!added[x] && (sel == UINT32_MAX || key[x] > key[sel])) sel = x;
+            }
+            prev = last;
+            last = sel;
+            added[sel] = 1;
+            if (step == nl - 1) break;
+            for (uint32_t i = 0; i < nl; ++i) {
+                uint32_t x = alive[i];
+                if (!added[x]) key[x] += w[(size_t)sel * n + x];
+            }
+        }
+        /* cut of the phase: weight of 'last' to everything else */
+        int64_t cut = 0;
+        for (uint32_t i = 0; i < nl; ++i) cut += w[(size_t)last * n + alive[i]];
+        if (cut < best) best = cut;
+        /* merge last into prev */
+        for (uint32_t i = 0; i < nl; ++i) {
+            uint32_t x = alive[i];
+            if (x == prev || x == last) continue;
+            w[(size_t)prev * n + x] += w[(size_t)last * n + x];
+            w[(size_t)x * n + prev] = w[(size_t)prev * n + x];
+        }
+        w[(size_t)prev * n + last] = w[(size_t)last * n + prev] = 0;
+        uint32_t j = 0;
+        for (uint32_t i = 0; i < nl; ++i)
+            if (alive[i] != last) alive[j++] = alive[i];
+        nl = j;
+    }
+    free(w);
+    free(alive);
+    free(key);
+    free(added);
+    return best;
+}
+
+int64_t orc_brute_mincut(uint32_t n, const int64_t* adj) {
+    if (n < 2 || n > 24) return -1;
+    /* S ranges over nonempty subsets of {0..n-2}; vertex n-1 is always outside.
+     * Gray-code walk: flipping vertex x changes the cut by
+     * (weight from x to the other side) - (weight from x to its own side). */
+    uint32_t k = n - 1;
+    unsigned char* in = (unsigned char*)calloc(n, 1);
+    int64_t cut = 0, best = INT64_MAX;
+    for (uint64_t g = 1; g < (1ull << k); ++g) {
+        uint32_t x = (uint32_t)__builtin_ctzll(g);
+        int64_t same = 0, other = 0;
+        for (uint32_t y = 0; y < n; ++y) {
+            if (y == x) continue;
+            if (in[y] == in[x]) same += adj[(size_t)x * n + y];
+            else other += adj[(size_t)x * n + y];
+        }
+        /* moving x to the other side: edges to its current side become cut,
+         * edges to the other side stop being cut */
+        cut += same - other;
+        in[x] ^= 1;
+        if (cut < best) best = cut;
+    }
+    free(in);
+    return best;
+}
