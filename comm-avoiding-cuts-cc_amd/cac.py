@@ -56,6 +56,8 @@ SIGNATURES = {
     "cac_cc": (_i32, [_vp, _vp, _u64, _u32, _i32, _pu32, _pu32]),
     "cac_cc_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
     "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
+    "cac_ks_dense": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64]),
+    "cac_ks_dense_dev": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64, _vp]),
 }
 
 _lib = None
@@ -184,6 +186,24 @@ class Context:
         nc = ctypes.c_uint32()
         self._check(self.lib.cac_cc_dev(self.handle, d_edges, m, n, mode, d_labels, ctypes.byref(nc), stream))
         return int(nc.value)
+
+    # ---- dense Karger-Stein ------------------------------------------------
+    def ks_dense(self, adj: np.ndarray, seed: int, first_trial: int, ntrials: int):
+        """Returns (raw cut per trial, min cut)."""
+        a = np.ascontiguousarray(adj, dtype=np.int64)
+        n = a.shape[0]
+        raw = np.empty(ntrials, dtype=np.int64)
+        mn = ctypes.c_int64()
+        self._check(self.lib.cac_ks_dense(self.handle, _ptr(a), n, seed, first_trial, ntrials,
+                                          raw.ctypes.data_as(_pi64), ctypes.byref(mn)))
+        return raw, int(mn.value)
+
+    def ks_dense_dev(self, d_adj: int, n: int, seed: int, first_trial: int, ntrials: int, stream=None):
+        raw = np.empty(ntrials, dtype=np.int64)
+        mn = ctypes.c_int64()
+        self._check(self.lib.cac_ks_dense_dev(self.handle, d_adj, n, seed, first_trial, ntrials,
+                                              raw.ctypes.data_as(_pi64), ctypes.byref(mn), stream))
+        return raw, int(mn.value)
 
     # ---- generators -------------------------------------------------------
     def gen_rmat_dev(self, scale: int, m: int, seed: int, thresholds, d_edges: int) -> int:

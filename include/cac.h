@@ -94,6 +94,22 @@ int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int lab
 int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
                uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
 
+/* ---- dense Karger-Stein --------------------------------------------------
+ * Replaces comincut::minimum_cut(adjacency_matrix<T>*, double, int) and
+ * minimum_cut_try (src/karger-stein/co_mincut.h:466-490; trials co_mincut.h:315-342).
+ * adj: n x n row-major int64, symmetric, zero diagonal, non-negative, n >= 2,
+ * n <= 8192.  Runs trials first_trial .. first_trial+ntrials-1; trial t's draws are
+ * Philox-4x32-10(ctr = {t, node, pick, 0}, key = {seed, 'KARG'}).  Every trial's
+ * upper bound is the minimum weighted degree (co_mincut.h:381).
+ * raw_cuts[i] (optional, ntrials entries) = min(bound, every leaf of trial i);
+ * *min_cut = min over the trials.  The number of trials for success probability
+ * p is cac_ks_trials(n, p) (comincut::number_of_trials, co_mincut.cpp:43-74). */
+int cac_ks_dense(cac_ctx* ctx, const int64_t* adj, uint32_t n, uint32_t seed, uint32_t first_trial,
+                 uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut);
+/* Device variant: d_adj in device memory (not validated); results on the host. */
+int cac_ks_dense_dev(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, uint32_t first_trial,
+                     uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut, void* stream);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * R-MAT edge list on the device (the reference generates R-MAT inputs with
  * PaRMAT, input_generators/rmat_driver.sh:13-16).  Draw d makes 'scale' quadrant

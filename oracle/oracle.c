@@ -260,3 +260,302 @@ int64_t orc_brute_mincut(uint32_t n, const int64_t* adj) {
     free(in);
     return best;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Trial-count and size rules                                                */
+/* ------------------------------------------------------------------------- */
+#include <math.h>
+
+#define KS_BASE 128u
+#define DOM_KS 0x4B415247u     /* 'KARG' */
+#define DOM_SAMPLE 0x53414D50u /* 'SAMP' */
+
+uint32_t orc_ks_target(uint32_t s) {
+    int t = (int)ceil((double)s / sqrt((double)2)) + 1;
+    return t > (int)KS_BASE ? (uint32_t)t : KS_BASE;
+}
+
+double orc_ks_min_success(uint32_t s) {
+    if (s <= KS_BASE) return 1.0;
+    double r = orc_ks_min_success(orc_ks_target(s));
+    return 1.0 - pow((1 - 0.5 * r), 2);
+}
+
+uint32_t orc_ks_trials(uint32_t n, double p) {
+    if (n <= KS_BASE) return 1;
+    if (p == 0) return 1;
+    double s = orc_ks_min_success(n), cur = 1.0, target = 1.0 - p;
+    uint32_t k = 0;
+    while (cur > target) {
+        cur = cur * (1.0 - s);
+        ++k;
+    }
+    return k;
+}
+
+uint32_t orc_intermediate_size(uint32_t n, uint64_t m) {
+    uint32_t c = (uint32_t)ceil(2.0 * sqrt((double)(uint32_t)m) + 1);
+    return c > n ? n : c;
+}
+
+uint32_t orc_sqrt_trials(uint32_t n, uint64_t m, double p) {
+    double pk = orc_ks_min_success(orc_intermediate_size(n, m));
+    double cp = 1.0 / (1 - p);
+    return (uint32_t)ceil((n * (n * log(cp))) / (2.0 * 2.0 * ((uint32_t)m * pk)));
+}
+
+uint32_t orc_approx_trials(double p) {
+    double alpha = log(1 - exp(-(double)0.75));
+    return (uint32_t)ceil(log((double)1.0 - p) / alpha);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Dense Karger-Stein                                                        */
+/* ------------------------------------------------------------------------- */
+int64_t orc_min_degree(uint32_t n, const int64_t* adj) {
+    int64_t best = INT64_MAX;
+    for (uint32_t i = 0; i < n; ++i) {
+        int64_t d = 0;
+        for (uint32_t j = 0; j < n; ++j) d += adj[(size_t)i * n + j];
+        if (d < best) best = d;
+    }
+    return n ? best : 0;
+}
+
+typedef struct {
+    uint32_t s;
+    int64_t* A;     /* s x s working copy */
+    uint32_t* rep;  /* representative of every original row */
+    int64_t* cap;   /* weighted degree of every live row, 0 for dead rows */
+    int64_t W;      /* sum of cap */
+    uint32_t nv;    /* live rows */
+} ks_state;
+
+/* lazy_adjacency_matrix::contract_edge + indexed capacity update: row a dies into row b */
+static int64_t ks_merge(ks_state* g, uint32_t a, uint32_t b) {
+    uint32_t s = g->s;
+    for (uint32_t x = 0; x < s; ++x)
+        if (g->rep[x] == a) g->rep[x] = b;
+    int64_t* ra = g->A + (size_t)a * s;
+    int64_t* rb = g->A + (size_t)b * s;
+    int64_t sum = 0;
+    for (uint32_t x = 0; x < s; ++x) {
+        int64_t v = rb[x] + ra[x];
+        if (g->rep[x] == b) v = 0;
+        rb[x] = v;
+        sum += v;
+    }
+    int64_t w = (g->cap[a] + g->cap[b] - sum) / 2;
+    g->W -= 2 * w;
+    g->cap[b] = sum;
+    g->cap[a] = 0;
+    g->nv -= 1;
+    return w;
+}
+
+static uint32_t first_prefix_ge(const int64_t* v, uint32_t len, uint64_t r) {
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < len; ++i) {
+        acc += (uint64_t)v[i];
+        if (acc >= r) return i;
+    }
+    return len; /* unreachable for 1 <= r <= sum */
+}
+
+/* contract a copy of A (s x s) to tgt super-vertices; returns the compacted
+ * k x k matrix (malloc'd) or NULL when the graph ran out of edges (disconnected). */
+static int64_t* ks_contract(uint32_t s, const int64_t* A, uint32_t tgt, uint32_t seed, uint32_t trial,
+                            uint32_t nid, int64_t bound, uint32_t* k_out) {
+    ks_state g;
+    g.s = s;
+    g.A = (int64_t*)malloc(sizeof(int64_t) * (size_t)s * s);
+    memcpy(g.A, A, sizeof(int64_t) * (size_t)s * s);
+    g.rep = (uint32_t*)malloc(sizeof(uint32_t) * s);
+    g.cap = (int64_t*)malloc(sizeof(int64_t) * s);
+    g.W = 0;
+    g.nv = s;
+    for (uint32_t x = 0; x < s; ++x) {
+        g.rep[x] = x;
+        int64_t d = 0;
+        for (uint32_t y = 0; y < s; ++y) d += A[(size_t)x * s + y];
+        g.cap[x] = d;
+        g.W += d;
+    }
+    uint32_t key[2] = {seed, DOM_KS};
+    int pr1 = 0, disconnected = 0;
+    for (uint32_t p = 0; p + tgt < s; ++p) {
+        if (g.W == 0) {
+            disconnected = 1;
+            break;
+        }
+        uint32_t ctr[4] = {trial, nid, p, 0}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        uint64_t xr = ((uint64_t)o[1] << 32) | o[0];
+        uint64_t xc = ((uint64_t)o[3] << 32) | o[2];
+        uint32_t i = first_prefix_ge(g.cap, s, orc_uniform1(xr, (uint64_t)g.W));
+        uint32_t k = first_prefix_ge(g.A + (size_t)i * s, s, orc_uniform1(xc, (uint64_t)g.cap[i]));
+        uint32_t j = g.rep[k];
+        uint32_t a = i > j ? i : j, b = i > j ? j : i;
+        int64_t w = ks_merge(&g, a, b);
+        if (w >= bound) ++pr1;
+    }
+    int64_t* out = NULL;
+    if (!disconnected) {
+        /* indexed_adjacency_matrix::pr_pass: contract any edge of weight >= bound */
+        if (pr1 > 0 && bound > 0) {
+            for (uint32_t i = 0; i < s && g.nv > 2; ++i) {
+                if (g.rep[i] != i || g.cap[i] < bound) continue;
+                const int64_t* row = g.A + (size_t)i * s;
+                for (uint32_t y = 0; y < s; ++y) {
+                    if (row[y] >= bound) {
+                        ks_merge(&g, i, g.rep[y]);
+                        break;
+                    }
+                }
+            }
+        }
+        /* compaction: rank of live rows; sum columns by class */
+        uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * s);
+        uint32_t k = 0;
+        for (uint32_t x = 0; x < s; ++x)
+            if (g.rep[x] == x) rank[x] = k++;
+        out = (int64_t*)calloc((size_t)k * k, sizeof(int64_t));
+        for (uint32_t a = 0; a < s; ++a) {
+            if (g.rep[a] != a) continue;
+            int64_t* dst = out + (size_t)rank[a] * k;
+            const int64_t* row = g.A + (size_t)a * s;
+            for (uint32_t y = 0; y < s; ++y) dst[rank[g.rep[y]]] += row[y];
+        }
+        *k_out = k;
+        free(rank);
+    }
+    free(g.A);
+    free(g.rep);
+    free(g.cap);
+    return out;
+}
+
+static int64_t ks_node(uint32_t s, const int64_t* A, uint32_t seed, uint32_t trial, uint32_t nid, int64_t bound) {
+    if (s <= KS_BASE) return orc_stoer_wagner(s, A);
+    uint32_t tgt = orc_ks_target(s);
+    int64_t best = INT64_MAX;
+    for (uint32_t c = 0; c < 2; ++c) {
+        uint32_t cid = 2 * nid + c, k = 0;
+        int64_t* C = ks_contract(s, A, tgt, seed, trial, cid, bound, &k);
+        int64_t v = C ? ks_node(k, C, seed, trial, cid, bound) : 0;
+        free(C);
+        if (v < best) best = v;
+    }
+    return best;
+}
+
+int64_t orc_ks_trial(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t trial) {
+    if (n < 2) return 0;
+    int64_t bound = orc_min_degree(n, adj);
+    int64_t v = ks_node(n, adj, seed, trial, 1, bound);
+    return v < bound ? v : bound;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Weighted iterated sampling                                                */
+/* ------------------------------------------------------------------------- */
+static uint32_t root_min(uint32_t* p, uint32_t x) {
+    uint32_t r = x;
+    while (p[r] != r) r = p[r];
+    while (p[x] != r) {
+        uint32_t nx = p[x];
+        p[x] = r;
+        x = nx;
+    }
+    return r;
+}
+
+uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
+                             uint32_t trial, uint32_t* labels) {
+    uint64_t* P = (uint64_t*)malloc(sizeof(uint64_t) * (m ? m : 1));
+    uint32_t* par = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t key[2] = {seed, DOM_SAMPLE};
+    for (uint32_t v = 0; v < n; ++v) labels[v] = v;
+    uint32_t nv = n;
+    for (uint32_t round = 0; nv > t; ++round) {
+        uint64_t W = 0;
+        for (uint64_t e = 0; e < m; ++e) {
+            uint32_t u = (uint32_t)edges[3 * e], v = (uint32_t)edges[3 * e + 1];
+            if (labels[u] != labels[v]) W += edges[3 * e + 2];
+            P[e] = W;
+        }
+        if (W == 0) break;
+        for (uint32_t v = 0; v < n; ++v) par[v] = v;
+        uint32_t draws = nv;
+        for (uint32_t d = 0; d < draws; ++d) {
+            uint32_t ctr[4] = {trial, round, d, 0}, o[4];
+            orc_philox4x32_10(ctr, key, o);
+            uint64_t r = orc_uniform1(((uint64_t)o[1] << 32) | o[0], W);
+            uint64_t lo = 0, hi = m - 1; /* first e with P[e] >= r */
+            while (lo < hi) {
+                uint64_t mid = (lo + hi) / 2;
+                if (P[mid] >= r) hi = mid;
+                else lo = mid + 1;
+            }
+            uint32_t a = root_min(par, labels[edges[3 * lo]]);
+            uint32_t b = root_min(par, labels[edges[3 * lo + 1]]);
+            if (a == b) continue;
+            if (a < b) par[b] = a;
+            else par[a] = b;
+            if (--nv == t) break;
+        }
+        for (uint32_t v = 0; v < n; ++v) labels[v] = root_min(par, labels[v]);
+    }
+    free(P);
+    free(par);
+    return nv;
+}
+
+uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32_t* labels, int64_t* out) {
+    uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t k = 0;
+    for (uint32_t x = 0; x < n; ++x)
+        if (labels[x] == x) rank[x] = k++;
+    for (uint64_t e = 0; e < m; ++e) {
+        uint32_t a = rank[labels[edges[3 * e]]], b = rank[labels[edges[3 * e + 1]]];
+        if (a == b) continue;
+        out[(size_t)a * k + b] += (int64_t)edges[3 * e + 2];
+        out[(size_t)b * k + a] += (int64_t)edges[3 * e + 2];
+    }
+    free(rank);
+    return k;
+}
+
+int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial) {
+    uint32_t* lab = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    orc_sample_contract(n, m, edges, t, seed, trial, lab);
+    uint32_t k = 0;
+    for (uint32_t x = 0; x < n; ++x) k += lab[x] == x;
+    int64_t* M = (int64_t*)calloc((size_t)k * k, sizeof(int64_t));
+    orc_densify(n, m, edges, lab, M);
+    int64_t cut = orc_ks_trial(k, M, seed, trial);
+    free(M);
+    free(lab);
+    return cut;
+}
+
+/* First pick of node nid on matrix A (the selection rule alone), for the
+ * distribution test: pair {a, b} must be drawn with probability A[a][b] / W. */
+void orc_ks_first_pick(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t trial, uint32_t nid, uint32_t* a,
+                       uint32_t* b) {
+    int64_t* cap = (int64_t*)malloc(sizeof(int64_t) * n);
+    uint64_t W = 0;
+    for (uint32_t x = 0; x < n; ++x) {
+        int64_t d = 0;
+        for (uint32_t y = 0; y < n; ++y) d += adj[(size_t)x * n + y];
+        cap[x] = d;
+        W += (uint64_t)d;
+    }
+    uint32_t key[2] = {seed, DOM_KS}, ctr[4] = {trial, nid, 0, 0}, o[4];
+    orc_philox4x32_10(ctr, key, o);
+    uint32_t i = first_prefix_ge(cap, n, orc_uniform1(((uint64_t)o[1] << 32) | o[0], W));
+    uint32_t k = first_prefix_ge(adj + (size_t)i * n, n, orc_uniform1(((uint64_t)o[3] << 32) | o[2], (uint64_t)cap[i]));
+    *a = i > k ? i : k;
+    *b = i > k ? k : i;
+    free(cap);
+}

@@ -59,7 +59,61 @@ int64_t orc_stoer_wagner(uint32_t n, const int64_t* adj);
 /* 2^(n-1) subset enumeration, n <= 24. */
 int64_t orc_brute_mincut(uint32_t n, const int64_t* adj);
 
+
+
+/* ---- trial-count and size rules (floating point exactly as the reference) ---
+ * comincut::target_number_of_vertices_for_contraction (src/karger-stein/co_mincut.cpp:18-20),
+ * min_success_in_one_trial (:32-40), number_of_trials (:43-74),
+ * SquareRootCut::intermediate_size (src/SquareRootCut.cpp:63-70), numberOfTrials (:76-83),
+ * ApproximateCut::numberOfTrials (src/ApproximateCut.cpp:82-87). */
+uint32_t orc_ks_target(uint32_t s);
+double orc_ks_min_success(uint32_t s);
+uint32_t orc_ks_trials(uint32_t n, double p);
+uint32_t orc_intermediate_size(uint32_t n, uint64_t m);
+uint32_t orc_sqrt_trials(uint32_t n, uint64_t m, double p);
+uint32_t orc_approx_trials(double p);
+
+/* ---- dense Karger-Stein (one trial) -----------------------------------------
+ * Restates comincut::minimum_cut_trials_start(dense) + recursive_contraction_inner
+ * (co_mincut.h:370-390, 176-223), contract_graph (co_mincut.h:38-66),
+ * indexed_adjacency_matrix::{contract_random_edge, contract_edge, pr_pass, compact}
+ * (indexed_adjacency_matrix.hpp:80-98, 137-191) and lazy_adjacency_matrix::
+ * {contract_edge, compact} (lazy_adjacency_matrix.hpp:120-216), with:
+ *   - picks keyed by Philox(ctr={trial, node, pick, 0}, key={seed,'KARG'}): row i
+ *     = first index with prefix(capacity) >= 1 + floor(lo64*W/2^64), column k =
+ *     first index with prefix(row i) >= 1 + floor(hi64*cap_i/2^64), j = rep[k];
+ *   - natural-order prefix instead of the sum_tree order;
+ *   - every node of the trial uses the trial's initial bound (min weighted degree);
+ *   - leaves (<= 128 vertices) return min(bound, exact min cut).
+ * Returns the raw cut of the trial: min(bound, every leaf).  adj is n x n,
+ * symmetric, zero diagonal, non-negative. */
+int64_t orc_ks_trial(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t trial);
+int64_t orc_min_degree(uint32_t n, const int64_t* adj);
+/* The first pick of node nid (max, min order), for the selection-distribution test. */
+void orc_ks_first_pick(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t trial, uint32_t nid, uint32_t* a,
+                       uint32_t* b);
+
+/* ---- weighted iterated sampling (one sqrt-cut trial's shrink phase) ----------
+ * Restates SequentialSquareRootCut::iteratedSampling (src/SequentialSquareRootCut.cpp:21-67)
+ * with AdjacencyListGraph::{weaklyContractEdge, finalizeContractionPhase}
+ * (src/AdjacencyListGraph.cpp:54-69, 87-112): rounds of |V| draws (with
+ * replacement) proportional to weight over the edges whose endpoints are still
+ * in different super-vertices, draw d of round r keyed Philox(ctr={trial, r, d, 0},
+ * key={seed,'SAMP'}) lo64; contraction in draw order, stopping as soon as |V| == t.
+ * labels[v] = smallest original id of v's super-vertex.  Returns |V| at the end
+ * (== t unless the graph has more than t components). edges: m x (u, v, w). */
+uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
+                             uint32_t trial, uint32_t* labels);
+/* Contracted multigraph -> dense matrix over the ranks of the canonical labels
+ * (AdjacencyListGraph::compact, src/AdjacencyListGraph.cpp:34-52, renamed by rank
+ * of the minimum id instead of first appearance).  out: k x k, k = distinct labels. */
+uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32_t* labels, int64_t* out);
+/* One low-concurrency sqrt-cut trial: sample_contract to t, densify, one dense KS
+ * trial with the same seed and trial id (SequentialSquareRootCut::compute,
+ * src/SequentialSquareRootCut.cpp:4-19). */
+int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial);
+
 #ifdef __cplusplus
 }
 #endif
-#endif
+#endif /* CAC_ORACLE_H */

@@ -28,6 +28,18 @@ SIGS = {
     "orc_rmat": (_u64, [_u32, _u64, _u32, _u32, _u32, _u32, _vp]),
     "orc_stoer_wagner": (_i64, [_u32, _vp]),
     "orc_brute_mincut": (_i64, [_u32, _vp]),
+    "orc_ks_target": (_u32, [_u32]),
+    "orc_ks_min_success": (ctypes.c_double, [_u32]),
+    "orc_ks_trials": (_u32, [_u32, ctypes.c_double]),
+    "orc_intermediate_size": (_u32, [_u32, _u64]),
+    "orc_sqrt_trials": (_u32, [_u32, _u64, ctypes.c_double]),
+    "orc_approx_trials": (_u32, [ctypes.c_double]),
+    "orc_ks_trial": (_i64, [_u32, _vp, _u32, _u32]),
+    "orc_min_degree": (_i64, [_u32, _vp]),
+    "orc_sample_contract": (_u32, [_u32, _u64, _vp, _u32, _u32, _u32, _vp]),
+    "orc_densify": (_u32, [_u32, _u64, _vp, _vp, _vp]),
+    "orc_sqrt_trial": (_i64, [_u32, _u64, _vp, _u32, _u32, _u32]),
+    "orc_ks_first_pick": (None, [_u32, _vp, _u32, _u32, _u32, _vp, _vp]),
 }
 
 _lib = None
@@ -99,6 +111,74 @@ def brute_mincut(adj: np.ndarray) -> int:
     return int(lib().orc_brute_mincut(a.shape[0], _p(a)))
 
 
+def ks_target(s: int) -> int:
+    return int(lib().orc_ks_target(s))
+
+
+def ks_min_success(s: int) -> float:
+    return float(lib().orc_ks_min_success(s))
+
+
+def ks_trials(n: int, p: float) -> int:
+    return int(lib().orc_ks_trials(n, p))
+
+
+def intermediate_size(n: int, m: int) -> int:
+    return int(lib().orc_intermediate_size(n, m))
+
+
+def sqrt_trials(n: int, m: int, p: float) -> int:
+    return int(lib().orc_sqrt_trials(n, m, p))
+
+
+def approx_trials(p: float) -> int:
+    return int(lib().orc_approx_trials(p))
+
+
+def min_degree(adj) -> int:
+    a = np.ascontiguousarray(adj, dtype=np.int64)
+    return int(lib().orc_min_degree(a.shape[0], _p(a)))
+
+
+def ks_trial(adj, seed: int, trial: int) -> int:
+    """Raw cut of one dense Karger-Stein trial (min(bound, leaves))."""
+    a = np.ascontiguousarray(adj, dtype=np.int64)
+    return int(lib().orc_ks_trial(a.shape[0], _p(a), seed, trial))
+
+
+def ks_first_pick(adj, seed: int, trial: int, nid: int = 1):
+    a = np.ascontiguousarray(adj, dtype=np.int64)
+    x = np.zeros(1, np.uint32)
+    y = np.zeros(1, np.uint32)
+    lib().orc_ks_first_pick(a.shape[0], _p(a), seed, trial, nid, _p(x), _p(y))
+    return int(x[0]), int(y[0])
+
+
+def _edges3(edges):
+    return np.ascontiguousarray(np.asarray(edges, dtype=np.uint64).reshape(-1, 3))
+
+
+def sample_contract(n: int, edges, t: int, seed: int, trial: int):
+    e = _edges3(edges)
+    lab = np.empty(n, dtype=np.uint32)
+    nv = lib().orc_sample_contract(n, e.shape[0], _p(e), t, seed, trial, _p(lab))
+    return lab, int(nv)
+
+
+def densify(n: int, edges, labels):
+    e = _edges3(edges)
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    k = int(np.count_nonzero(lab == np.arange(n, dtype=np.uint32)))
+    out = np.zeros((k, k), dtype=np.int64)
+    lib().orc_densify(n, e.shape[0], _p(e), _p(lab), _p(out))
+    return out
+
+
+def sqrt_trial(n: int, edges, t: int, seed: int, trial: int) -> int:
+    e = _edges3(edges)
+    return int(lib().orc_sqrt_trial(n, e.shape[0], _p(e), t, seed, trial))
+
+
 def edges_to_dense(n: int, edges: np.ndarray) -> np.ndarray:
     """(m, 3) [u, v, w] -> symmetric int64 matrix; parallel edges summed, loops dropped."""
     A = np.zeros((n, n), dtype=np.int64)
@@ -109,3 +189,41 @@ def edges_to_dense(n: int, edges: np.ndarray) -> np.ndarray:
         np.add.at(A, (u[keep], v[keep]), w[keep])
         np.add.at(A, (v[keep], u[keep]), w[keep])
     return A
+
+
+# ---- the reference's own Karger-Stein library (oracle/_ref, built from the
+# reference sources by build_ref.sh when the reference tree is available) ----
+REF_LIB = HERE / "_ref" / "libref.so"
+_ref = None
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(str(REF_LIB))
+        L.ref_ks_target.restype = _i32
+        L.ref_ks_target.argtypes = [_i32]
+        L.ref_ks_min_success.restype = ctypes.c_double
+        L.ref_ks_min_success.argtypes = [_i32]
+        L.ref_ks_trials.restype = _i32
+        L.ref_ks_trials.argtypes = [_i32, ctypes.c_double]
+        L.ref_deterministic_mincut.restype = _u64
+        L.ref_deterministic_mincut.argtypes = [_i32, ctypes.c_long, _vp]
+        L.ref_ks_mincut.restype = _u64
+        L.ref_ks_mincut.argtypes = [_i32, ctypes.c_long, _vp, ctypes.c_double, _i32]
+        _ref = L
+    return _ref
+
+
+def ref_deterministic_mincut(n: int, edges) -> int:
+    e = np.ascontiguousarray(edges, dtype=np.uint64).reshape(-1)
+    return int(ref().ref_deterministic_mincut(n, e.size // 3, _p(e)))
+
+
+def ref_ks_mincut(n: int, edges, p: float, seed: int) -> int:
+    e = np.ascontiguousarray(edges, dtype=np.uint64).reshape(-1)
+    return int(ref().ref_ks_mincut(n, e.size // 3, _p(e), p, seed))

@@ -1,0 +1,80 @@
+"""Dense Karger-Stein parity on the device: every trial's raw cut from
+libcac.so equals the oracle's trial fed the same PRNG stream, bit for bit; final
+cuts equal exact Stoer-Wagner on planted-cut graphs."""
+import sys
+
+import numpy as np
+import pytest
+from graphgen import dense, erdos_renyi, planted
+
+import cac
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,seed", [(2, 0), (5, 1), (64, 2), (128, 3), (129, 4), (150, 5), (260, 6), (520, 7)])
+def test_ks_trials_bit_exact(ctx, orc, n, seed):
+    rng = np.random.default_rng(seed)
+    if n < 10:
+        A = dense(n, erdos_renyi(rng, n, 1.0, wmax=9))
+    else:
+        A = dense(n, planted(rng, n, deg=10, wmax=30, c=int(rng.integers(1, 6))))
+    T = 6
+    _say(f"ks parity n {n} seed {seed} trials {T}")
+    raw, mn = ctx.ks_dense(A, 17, 3, T)
+    want = [orc.ks_trial(A, 17, 3 + t) for t in range(T)]
+    assert raw.tolist() == want
+    assert mn == min(want)
+    assert mn >= orc.stoer_wagner(A)
+
+
+@pytest.mark.parametrize("n", [200, 700])
+def test_ks_pr_pass(ctx, orc, n):
+    # a vertex of degree 2 makes the bound 2, so almost every contracted weight
+    # reaches it and the PR pass of each contraction runs
+    rng = np.random.default_rng(21 + n)
+    e = erdos_renyi(rng, n - 1, 8.0 / n, wmax=5)
+    e = np.concatenate([e, [[n - 1, 0, 1], [n - 1, 1, 1]]])
+    A = dense(n, e)
+    _say(f"ks pr pass n {n}")
+    raw, mn = ctx.ks_dense(A, 5, 0, 8)
+    want = [orc.ks_trial(A, 5, t) for t in range(8)]
+    assert raw.tolist() == want
+    assert mn == orc.stoer_wagner(A)
+
+
+def test_ks_disconnected_is_zero(ctx, orc):
+    rng = np.random.default_rng(3)
+    h = 150
+    e1 = erdos_renyi(rng, h, 0.1, wmax=5)
+    e2 = erdos_renyi(rng, h, 0.1, wmax=5)
+    e2[:, :2] += h
+    A = dense(2 * h, np.concatenate([e1, e2]))
+    _say("ks disconnected n 300")
+    raw, mn = ctx.ks_dense(A, 1, 0, 3)
+    assert mn == 0 and raw.tolist() == [orc.ks_trial(A, 1, t) for t in range(3)]
+
+
+def test_ks_final_cut_planted(ctx, orc):
+    rng = np.random.default_rng(99)
+    n = 1000
+    c = 4
+    A = dense(n, planted(rng, n, deg=16, wmax=100, c=c))
+    T = orc.ks_trials(n, 0.95)
+    _say(f"ks planted n {n} trials {T}")
+    raw, mn = ctx.ks_dense(A, 2, 0, T)
+    assert mn == c == orc.stoer_wagner(A)
+    assert raw.tolist() == [orc.ks_trial(A, 2, t) for t in range(T)]
+
+
+def test_ks_rejects_bad_matrix(ctx):
+    A = np.zeros((4, 4), dtype=np.int64)
+    A[0, 1] = 3
+    with pytest.raises(cac.CacError) as ei:
+        ctx.ks_dense(A, 0, 0, 1)
+    assert ei.value.code == cac.CAC_EINVAL
