@@ -58,6 +58,13 @@ SIGNATURES = {
     "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
     "cac_ks_dense": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64]),
     "cac_ks_dense_dev": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64, _vp]),
+    "cac_ks_trials": (_u32, [_u32, ctypes.c_double]),
+    "cac_intermediate_size": (_u32, [_u32, _u64]),
+    "cac_sqrt_trials": (_u32, [_u32, _u64, ctypes.c_double]),
+    "cac_sample_contract": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _pu32, _pu32, _vp, _u64, _pu64]),
+    "cac_sqrt_cut_trials": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _u32, _pi64]),
+    "cac_mincut": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
+    "cac_mincut_dev": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32, _vp]),
 }
 
 _lib = None
@@ -205,6 +212,37 @@ class Context:
                                               raw.ctypes.data_as(_pi64), ctypes.byref(mn), stream))
         return raw, int(mn.value)
 
+    # ---- sampling contraction and sqrt-cut min cut ------------------------
+    def sample_contract(self, edges, n: int, t: int, seed: int, trial: int):
+        """Returns (vmap, n_out, merged (k, 3) uint64 [u, v, w])."""
+        e = to_edges(edges)
+        vmap = np.empty(n, dtype=np.uint32)
+        nout = ctypes.c_uint32()
+        mout = ctypes.c_uint64()
+        cap = len(e)
+        merged = np.empty(max(cap, 1), dtype=EDGE)
+        self._check(self.lib.cac_sample_contract(self.handle, _ptr(e), len(e), n, t, seed, trial,
+                                                 vmap.ctypes.data_as(_pu32), ctypes.byref(nout), _ptr(merged), cap,
+                                                 ctypes.byref(mout)))
+        mg = merged[: mout.value]
+        out = np.stack([mg["u"].astype(np.uint64), mg["v"].astype(np.uint64), mg["w"]], axis=1)
+        return vmap, int(nout.value), out
+
+    def sqrt_cut_trials(self, edges, n: int, t: int, seed: int, first_trial: int, ntrials: int) -> np.ndarray:
+        e = to_edges(edges)
+        raw = np.empty(ntrials, dtype=np.int64)
+        self._check(self.lib.cac_sqrt_cut_trials(self.handle, _ptr(e), len(e), n, t, seed, first_trial, ntrials,
+                                                 raw.ctypes.data_as(_pi64)))
+        return raw
+
+    def mincut(self, edges, n: int, p: float, seed: int):
+        """Returns (cut, trials) with SquareRootCut low-concurrency semantics."""
+        e = to_edges(edges)
+        cut = ctypes.c_int64()
+        T = ctypes.c_uint32()
+        self._check(self.lib.cac_mincut(self.handle, _ptr(e), len(e), n, p, seed, ctypes.byref(cut), ctypes.byref(T)))
+        return int(cut.value), int(T.value)
+
     # ---- generators -------------------------------------------------------
     def gen_rmat_dev(self, scale: int, m: int, seed: int, thresholds, d_edges: int) -> int:
         ta, tab, tabc = thresholds
@@ -212,6 +250,31 @@ class Context:
         self._check(self.lib.cac_gen_rmat_dev(self.handle, scale, m, seed, ta, tab, tabc, d_edges,
                                               ctypes.byref(used)))
         return int(used.value)
+
+
+def to_edges(edges) -> np.ndarray:
+    """(m, 3) [u, v, w] integers (or EDGE records) -> contiguous EDGE records."""
+    a = np.asarray(edges)
+    if a.dtype == EDGE:
+        return np.ascontiguousarray(a)
+    a = a.reshape(-1, 3)
+    out = np.empty(len(a), dtype=EDGE)
+    out["u"] = a[:, 0]
+    out["v"] = a[:, 1]
+    out["w"] = a[:, 2]
+    return out
+
+
+def ks_trials(n: int, p: float) -> int:
+    return int(load_library().cac_ks_trials(n, p))
+
+
+def intermediate_size(n: int, m: int) -> int:
+    return int(load_library().cac_intermediate_size(n, m))
+
+
+def sqrt_trials(n: int, m: int, p: float) -> int:
+    return int(load_library().cac_sqrt_trials(n, m, p))
 
 
 def rmat_thresholds(a: float, b: float, c: float):

@@ -57,6 +57,17 @@ __host__ __device__ inline uint64_t uniform1(uint64_t x, uint64_t W) {
 #endif
 }
 
+// Barrier after which global-memory writes of this workgroup are visible to all
+// of its waves.  __syncthreads() alone only orders LDS on gfx950: the compiler
+// does not wait for outstanding vector stores before s_barrier, and the CU's
+// vector L1 may still hold lines loaded before another wave's store.  The
+// agent-scope acquire fence waits for this wave's memory operations
+// (vmcnt(0)) and invalidates the L1, so loads after the barrier read L2.
+__device__ inline void block_sync_global() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+}
+
 inline unsigned grid_for(uint64_t work, unsigned block, unsigned cap) {
   uint64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;

@@ -14,14 +14,35 @@ namespace cac {
 struct KsRoot {
   const int64_t* mat;
   uint32_t n, trial, slot;
+  int64_t bound;  // the trial's upper bound (minimum weighted degree)
 };
 
 // Minimum weighted degree of a dense device matrix into *d_bound (device).
 int ks_min_degree(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_bound, hipStream_t s);
-// Runs one trial per root; d_bounds[slot] holds each trial's bound and
-// d_results[slot] (initialised to the bound) receives min(bound, leaves).
-int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, const int64_t* d_bounds,
-           int64_t* d_results, hipStream_t s);
+// Runs one trial per root; d_results[slot] (initialised to the root's bound)
+// receives min(bound, leaves).
+int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s);
+
+// Host forms of the reference's size and trial-count rules (same floating point).
+uint32_t ks_target_host(uint32_t s);
+uint32_t ks_trials_host(uint32_t n, double p);
+uint32_t sqrt_intermediate_size(uint32_t n, uint64_t m);
+uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p);
+// Sqrt-cut trials on a device edge list: raw cut of trial_ids[i] to d_results[i].
+int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                 const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s);
 int ks_check_matrix(cac_ctx* ctx, const int64_t* adj, uint32_t n);
+
+// One sampling contraction (cac_sample_contract): canonical labels and the merged
+// k x k matrix stay on the device until sample_out_free.
+struct SampleOut {
+  uint32_t* lab = nullptr;
+  int64_t* mat = nullptr;
+  uint32_t k = 0;
+  void* arena = nullptr;
+};
+int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                        uint32_t trial, SampleOut* out, hipStream_t s);
+void sample_out_free(SampleOut* o);
 
 }  // namespace cac

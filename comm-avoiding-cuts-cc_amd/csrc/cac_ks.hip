@@ -37,6 +37,8 @@ __device__ inline const int64_t* row_of(const KsChild& c, const KsLds& L, uint32
 // (cap[a] + cap[b] - cap_new) / 2.  Called by the whole block.
 __device__ int64_t ks_merge(const KsChild& c, const KsLds& L, uint32_t a, uint32_t b) {
   const uint32_t s = c.s;
+  // every wave has read the representatives that chose (a, b) before any is relabelled
+  __syncthreads();
   for (uint32_t x = threadIdx.x; x < s; x += kKsBlock)
     if (L.rep[x] == a) L.rep[x] = b;
   __syncthreads();
@@ -50,6 +52,8 @@ __device__ int64_t ks_merge(const KsChild& c, const KsLds& L, uint32_t a, uint32
     dst[x] = v;
     sum += (uint64_t)v;
   }
+  // the new row b is read by other waves in later picks and the compaction
+  block_sync_global();
   const int64_t newcap = (int64_t)block_sum_u64(sum, L.red);
   const int64_t w = (L.cap[a] + L.cap[b] - newcap) / 2;
   __syncthreads();
@@ -150,6 +154,7 @@ __global__ void __launch_bounds__(kKsBlock) k_ks_contract(const KsChild* __restr
   }
 
   // ---- compaction: rank live rows, sum columns by class --------------------
+  __syncthreads();  // the PR loop's reads of cap/rep precede the overwrites below
   uint32_t* rank = (uint32_t*)L.cap;           // [s] in the first half of cap
   uint32_t* colmap_tmp = rank + s;             // [s] in the second half of cap
   {
@@ -399,14 +404,17 @@ __global__ void k_copy_i64(int64_t* dst, const int64_t* src, uint32_t count) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+}  // namespace
+
 // Host mirror of co_mincut.cpp:18-20 (same double arithmetic).
-uint32_t ks_target(uint32_t s) {
+uint32_t ks_target_host(uint32_t s) {
   int t = (int)std::ceil((double)s / std::sqrt((double)2)) + 1;
   return t > (int)kKsBase ? (uint32_t)t : kKsBase;
 }
 
+namespace {
 size_t contract_lds_bytes(uint32_t s) { return ((13 * (size_t)s + 15) & ~(size_t)15) + 16 * kKsWaves + 64; }
-
+inline uint32_t ks_target(uint32_t s) { return ks_target_host(s); }
 }  // namespace
 
 int ks_min_degree(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_bound, hipStream_t s) {
@@ -417,21 +425,9 @@ int ks_min_degree(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_bou
 }
 
 // Runs one Karger-Stein trial per root.  roots[i]: device matrix (n_i x n_i),
-// trial id, result slot and bound (device pointer d_bounds[slot] must hold it).
-// results[slot] must be initialised by the caller (to the bound).
-int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, const int64_t* d_bounds, int64_t* d_results,
-           hipStream_t s) {
+// trial id, result slot and bound; d_results[slot] must hold the bound already.
+int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s) {
   if (roots.empty()) return CAC_OK;
-  std::vector<int64_t> bounds(roots.size());
-  {
-    // bounds live on the device (computed there); the contraction kernels take them by value
-    uint32_t maxslot = 0;
-    for (auto& r : roots) maxslot = std::max(maxslot, r.slot);
-    std::vector<int64_t> hb(maxslot + 1);
-    CAC_HIP(ctx, hipMemcpyAsync(hb.data(), d_bounds, hb.size() * 8, hipMemcpyDeviceToHost, s));
-    CAC_HIP(ctx, hipStreamSynchronize(s));
-    for (size_t i = 0; i < roots.size(); ++i) bounds[i] = hb[roots[i].slot];
-  }
   struct Node {
     const int64_t* mat;
     uint32_t s, trial, nid, slot;
@@ -439,7 +435,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, const 
   };
   std::vector<Node> level;
   for (size_t i = 0; i < roots.size(); ++i)
-    level.push_back({roots[i].mat, roots[i].n, roots[i].trial, 1u, roots[i].slot, bounds[i]});
+    level.push_back({roots[i].mat, roots[i].n, roots[i].trial, 1u, roots[i].slot, roots[i].bound});
 
   void* prev_arena = nullptr;  // buffers holding 'level' matrices (except the roots)
   int rc = CAC_OK;
@@ -575,8 +571,8 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
   for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
     std::vector<KsRoot> roots;
     for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
-      roots.push_back({d_adj, n, first_trial + t, t});
-    rc = ks_run(ctx, roots, seed, d_bound, d_res, s);
+      roots.push_back({d_adj, n, first_trial + t, t, bound});
+    rc = ks_run(ctx, roots, seed, d_res, s);
   }
   if (rc == CAC_OK) {
     std::vector<int64_t> res(ntrials);

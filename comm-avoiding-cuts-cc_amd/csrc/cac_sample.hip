@@ -1,0 +1,412 @@
+// cac_sample.hip -- weighted iterated sampling (the sqrt-cut trial's shrink
+// phase) and the low-concurrency sqrt-cut driver on gfx950.
+//
+// Replaces SequentialSquareRootCut::iteratedSampling / compute
+// (src/SequentialSquareRootCut.cpp:4-67) with AdjacencyListGraph::
+// {weaklyContractEdge, finalizeContractionPhase, compact}
+// (src/AdjacencyListGraph.cpp:34-112) and the trial loop of
+// SquareRootCut::runLowConcurrencyMaster (src/SquareRootCut.cpp:86-148).
+//
+// One workgroup runs one trial's rounds (trials run side by side):
+//   1. prefix of the edge weights with loops masked to 0 (u64, block scan);
+//   2. |V| draws: Philox(ctr = {trial, round, draw, 0}, key = {seed, 'SAMP'}),
+//      r = 1 + floor(x W / 2^64), edge = first prefix >= r (binary search);
+//   3. "contract in draw order until |V| == t" (a prefix condition): binary
+//      search over the prefix length with lock-free union-find probes, each
+//      counting its successful hooks;
+//   4. canonical relabel: every vertex takes the smallest original id of its
+//      super-vertex (hooks always attach the larger root under the smaller).
+// Densification then ranks the canonical labels and accumulates the weights of
+// the surviving edges into a t x t int64 matrix (the merge of parallel edges),
+// which feeds one dense Karger-Stein trial per sqrt-cut trial.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cac_common.h"
+#include "cac_internal.h"
+#include "cac_ks.h"
+#include "cac_ks_kernels.h"
+
+namespace cac {
+namespace {
+
+constexpr int kSmpBlock = 1024;
+constexpr int kSmpWaves = kSmpBlock / 64;
+
+struct TrialScratch {
+  uint32_t* lab;    // [n]
+  uint32_t* par;    // [n]
+  uint64_t* pre;    // [m]
+  uint2* draws;     // [n]
+  int64_t* mat;     // [t x t]
+  uint32_t* nv_out; // [1]
+};
+
+__device__ inline uint64_t wsum64(uint64_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+__device__ inline uint64_t bsum64(uint64_t x, uint64_t* lds) {
+  x = wsum64(x);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint64_t t = 0;
+  for (int w = 0; w < kSmpWaves; ++w) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+__device__ inline uint32_t uf_root(uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    uint32_t g = par[p];
+    if (g != p) par[x] = g;
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
+// Hook the sets of a and b; true when this call merged two roots.
+__device__ inline bool uf_union(uint32_t* par, uint32_t a, uint32_t b) {
+  uint32_t ra = uf_root(par, a), rb = uf_root(par, b);
+  while (ra != rb) {
+    uint32_t lo = min(ra, rb), hi = max(ra, rb);
+    uint32_t old = atomicCAS(par + hi, hi, lo);
+    if (old == hi) return true;
+    ra = uf_root(par, old);
+    rb = uf_root(par, lo);
+  }
+  return false;
+}
+
+// Union-find over the first L draws from a clean forest; returns the hooks made.
+__device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t* par, uint32_t n, uint64_t* lds) {
+  for (uint32_t x = threadIdx.x; x < n; x += kSmpBlock) par[x] = x;
+  block_sync_global();  // no wave may see the previous probe's forest
+  uint64_t mine = 0;
+  for (uint32_t d = threadIdx.x; d < L; d += kSmpBlock) mine += uf_union(par, draws[d].x, draws[d].y) ? 1 : 0;
+  block_sync_global();  // the relabel reads roots after the last hook
+  return (uint32_t)bsum64(mine, lds);
+}
+
+__global__ void __launch_bounds__(kSmpBlock) k_sample_trials(const cac_edge* __restrict__ edges, uint64_t m, uint32_t n,
+                                                             uint32_t t, uint32_t seed, const uint32_t* __restrict__ trials,
+                                                             const TrialScratch* __restrict__ scratch) {
+  __shared__ uint64_t lds[kSmpWaves + 2];
+  const TrialScratch S = scratch[blockIdx.x];
+  const uint32_t trial = trials[blockIdx.x];
+  for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = v;
+  block_sync_global();
+  uint32_t nv = n;
+  for (uint32_t round = 0; nv > t; ++round) {
+    // 1. masked inclusive prefix of the weights
+    uint64_t carry = 0;
+    constexpr int kItems = 4;
+    for (uint64_t base = 0; base < m; base += (uint64_t)kSmpBlock * kItems) {
+      const uint64_t e0 = base + (uint64_t)threadIdx.x * kItems;
+      uint64_t w[kItems], s = 0;
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) {
+        const uint64_t e = e0 + k;
+        w[k] = 0;
+        if (e < m) {
+          const cac_edge x = edges[e];
+          if (S.lab[x.u] != S.lab[x.v]) w[k] = x.w;
+        }
+        s += w[k];
+      }
+      // block exclusive scan of s
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      uint64_t inc = s;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        uint64_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
+      }
+      __shared__ uint64_t wtot[kSmpWaves];
+      if (lane == 63) wtot[wave] = inc;
+      __syncthreads();
+      uint64_t off = carry, tot = 0;
+      for (int v = 0; v < kSmpWaves; ++v) {
+        const uint64_t q = wtot[v];
+        if (v < wave) off += q;
+        tot += q;
+      }
+      uint64_t run = off + inc - s;
+#pragma unroll
+      for (int k = 0; k < kItems; ++k) {
+        const uint64_t e = e0 + k;
+        run += w[k];
+        if (e < m) S.pre[e] = run;
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    block_sync_global();  // the draws read the prefix written by every wave
+    const uint64_t W = carry;
+    if (W == 0) break;  // only loops left: more than t components
+    // 2. draws
+    const uint32_t D = nv;
+    for (uint32_t d = threadIdx.x; d < D; d += kSmpBlock) {
+      const u32x4 rnd = philox4x32(trial, round, d, 0u, seed, kDomSample);
+      const uint64_t r = uniform1(lo64(rnd), W);
+      uint64_t lo = 0, hi = m - 1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (S.pre[mid] >= r) hi = mid;
+        else lo = mid + 1;
+      }
+      const cac_edge x = edges[lo];
+      S.draws[d] = make_uint2(S.lab[x.u], S.lab[x.v]);
+    }
+    block_sync_global();
+    // 3. shortest draw prefix that leaves t super-vertices
+    uint32_t L = D;
+    const uint32_t all = probe(S.draws, D, S.par, n, lds);
+    if (nv - all <= t) {
+      uint32_t lo = 1, hi = D;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (nv - probe(S.draws, mid, S.par, n, lds) <= t) hi = mid;
+        else lo = mid + 1;
+      }
+      L = lo;
+    }
+    const uint32_t hooks = probe(S.draws, L, S.par, n, lds);
+    // 4. canonical relabel
+    for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = uf_root(S.par, S.lab[v]);
+    block_sync_global();
+    nv -= hooks;
+  }
+  if (threadIdx.x == 0) *S.nv_out = nv;
+}
+
+// Rank the canonical labels, accumulate the surviving edges into the k x k
+// matrix (k == t unless the sampling stopped early on a disconnected graph) and
+// compute the trial's bound (minimum weighted degree).
+__global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const cac_edge* __restrict__ edges, uint64_t m, uint32_t n,
+                                                              uint32_t t, const TrialScratch* __restrict__ scratch,
+                                                              const uint32_t* __restrict__ slots, int64_t* bounds,
+                                                              int64_t* results) {
+  __shared__ uint32_t wtot[kSmpWaves];
+  __shared__ uint64_t lds[kSmpWaves + 2];
+  const TrialScratch S = scratch[blockIdx.x];
+  const uint32_t slot = slots[blockIdx.x];
+  const uint32_t k = *S.nv_out;
+  if (k > t) {  // disconnected: a cut of weight 0 exists
+    if (threadIdx.x == 0) {
+      bounds[slot] = 0;
+      results[slot] = 0;
+    }
+    return;
+  }
+  uint32_t* rank = S.par;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kSmpBlock) {
+    const uint32_t x = base + threadIdx.x;
+    const uint32_t live = (x < n && S.lab[x] == x) ? 1u : 0u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = live;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    uint32_t off = carry, tot = 0;
+    for (int v = 0; v < kSmpWaves; ++v) {
+      if (v < wave) off += wtot[v];
+      tot += wtot[v];
+    }
+    if (live) rank[x] = off + inc - 1;
+    carry += tot;
+    __syncthreads();
+  }
+  for (uint64_t i = threadIdx.x; i < (uint64_t)k * k; i += kSmpBlock) S.mat[i] = 0;
+  block_sync_global();  // ranks and zeroed rows before the accumulation
+  for (uint64_t e = threadIdx.x; e < m; e += kSmpBlock) {
+    const cac_edge x = edges[e];
+    const uint32_t a = rank[S.lab[x.u]], b = rank[S.lab[x.v]];
+    if (a == b) continue;
+    atomicAdd((unsigned long long*)&S.mat[(uint64_t)a * k + b], (unsigned long long)x.w);
+    atomicAdd((unsigned long long*)&S.mat[(uint64_t)b * k + a], (unsigned long long)x.w);
+  }
+  block_sync_global();
+  // bound = min row sum: one wave per row
+  uint64_t best = ~0ull;
+  for (uint32_t r = threadIdx.x >> 6; r < k; r += kSmpWaves) {
+    uint64_t acc = 0;
+    for (uint32_t x = threadIdx.x & 63; x < k; x += 64) acc += (uint64_t)S.mat[(uint64_t)r * k + x];
+    acc = wsum64(acc);
+    best = acc < best ? acc : best;
+  }
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t b = ~0ull;
+    for (int w = 0; w < kSmpWaves; ++w) b = lds[w] < b ? lds[w] : b;
+    bounds[slot] = (int64_t)b;
+    results[slot] = (int64_t)b;
+  }
+}
+
+}  // namespace
+
+// Host rules, same floating point as the reference (SquareRootCut.cpp:63-83).
+uint32_t sqrt_intermediate_size(uint32_t n, uint64_t m) {
+  uint32_t c = (uint32_t)std::ceil(2.0 * std::sqrt((double)(uint32_t)m) + 1);
+  return c > n ? n : c;
+}
+
+static double ks_min_success_host(uint32_t s) {
+  if (s <= kKsBase) return 1.0;
+  double r = ks_min_success_host(ks_target_host(s));
+  return 1.0 - std::pow((1 - 0.5 * r), 2);
+}
+
+uint32_t ks_trials_host(uint32_t n, double p) {
+  if (n <= kKsBase || p == 0) return 1;
+  double s = ks_min_success_host(n), cur = 1.0, target = 1.0 - p;
+  uint32_t k = 0;
+  while (cur > target) {
+    cur = cur * (1.0 - s);
+    ++k;
+  }
+  return k;
+}
+
+uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p) {
+  double pk = ks_min_success_host(sqrt_intermediate_size(n, m));
+  double cp = 1.0 / (1 - p);
+  return (uint32_t)std::ceil((n * (n * std::log(cp))) / (2.0 * 2.0 * ((uint32_t)m * pk)));
+}
+
+// Runs sqrt-cut trials trial_ids on a device edge list; raw cut per trial to
+// d_results[0..count).  t: intermediate size.
+int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                 const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s) {
+  const uint32_t count = (uint32_t)trial_ids.size();
+  if (count == 0) return CAC_OK;
+  const size_t per = carve_size((size_t)n * 4) * 2 + carve_size(std::max<uint64_t>(m, 1) * 8) +
+                     carve_size((size_t)n * 8) + carve_size((size_t)t * t * 8) + 256;
+  size_t free_b = 0, total_b = 0;
+  CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+  size_t batch = std::max<size_t>(1, (size_t)(0.4 * (double)free_b) / per);
+  batch = std::min<size_t>({batch, (size_t)count, (size_t)4096});
+  int rc = CAC_OK;
+  for (uint32_t b0 = 0; b0 < count && rc == CAC_OK; b0 += (uint32_t)batch) {
+    const uint32_t nb = std::min<uint32_t>(count - b0, (uint32_t)batch);
+    const size_t bytes = per * nb + carve_size(nb * sizeof(TrialScratch)) + 3 * carve_size(nb * 8) + 1024;
+    void* arena = nullptr;
+    hipError_t e = hipMalloc(&arena, bytes);
+    if (e != hipSuccess) return hip_fail(ctx, e, "sqrt-cut trial scratch");
+    Carver cv(arena);
+    TrialScratch* d_scr = cv.take<TrialScratch>(nb);
+    uint32_t* d_nv = cv.take<uint32_t>(nb);
+    uint32_t* d_trials = cv.take<uint32_t>(nb);
+    uint32_t* d_slots = cv.take<uint32_t>(nb);
+    int64_t* d_bounds = cv.take<int64_t>(nb);
+    std::vector<TrialScratch> scr(nb);
+    std::vector<uint32_t> tids(nb), slots(nb);
+    for (uint32_t i = 0; i < nb; ++i) {
+      scr[i].lab = cv.take<uint32_t>(n);
+      scr[i].par = cv.take<uint32_t>(n);
+      scr[i].pre = cv.take<uint64_t>(std::max<uint64_t>(m, 1));
+      scr[i].draws = cv.take<uint2>(n);
+      scr[i].mat = cv.take<int64_t>((size_t)t * t);
+      scr[i].nv_out = d_nv + i;
+      tids[i] = trial_ids[b0 + i];
+      slots[i] = b0 + i;
+    }
+    (void)hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(d_trials, tids.data(), nb * 4, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(d_slots, slots.data(), nb * 4, hipMemcpyHostToDevice, s);
+    prof_begin(ctx, "sample_trials", s);
+    hipLaunchKernelGGL(k_sample_trials, dim3(nb), dim3(kSmpBlock), 0, s, d_edges, m, n, t, seed,
+                       (const uint32_t*)d_trials, (const TrialScratch*)d_scr);
+    prof_end(ctx, s);
+    // results and bounds are indexed by batch-local slots here
+    prof_begin(ctx, "densify_trials", s);
+    hipLaunchKernelGGL(k_densify_trials, dim3(nb), dim3(kSmpBlock), 0, s, d_edges, m, n, t,
+                       (const TrialScratch*)d_scr, (const uint32_t*)d_slots, d_bounds - b0, d_results);
+    prof_end(ctx, s);
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "sampling kernels");
+    std::vector<uint32_t> nvs(nb);
+    std::vector<int64_t> hb(nb);
+    if (rc == CAC_OK) {
+      (void)hipMemcpyAsync(nvs.data(), d_nv, nb * 4, hipMemcpyDeviceToHost, s);
+      (void)hipMemcpyAsync(hb.data(), d_bounds, nb * 8, hipMemcpyDeviceToHost, s);
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) rc = hip_fail(ctx, e, "sampling kernels");
+    }
+    if (rc == CAC_OK) {
+      std::vector<KsRoot> roots;
+      for (uint32_t i = 0; i < nb; ++i)
+        if (nvs[i] >= 2 && nvs[i] <= t) roots.push_back({scr[i].mat, nvs[i], tids[i], b0 + i, hb[i]});
+      rc = ks_run(ctx, roots, seed, d_results, s);
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(arena);
+  }
+  return rc;
+}
+
+}  // namespace cac
+
+namespace cac {
+
+// One trial's shrink phase for cac_sample_contract: labels (canonical) and the
+// merged k x k matrix, k = super-vertices left (t unless disconnected).
+int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                        uint32_t trial, SampleOut* out, hipStream_t s) {
+  *out = SampleOut{};
+  const size_t bytes = carve_size((size_t)n * 4) * 2 + carve_size(std::max<uint64_t>(m, 1) * 8) +
+                       carve_size((size_t)n * 8) + 4 * carve_size(64) + carve_size(sizeof(TrialScratch));
+  CAC_HIP(ctx, hipMalloc(&out->arena, bytes));
+  Carver cv(out->arena);
+  TrialScratch h;
+  TrialScratch* d_scr = cv.take<TrialScratch>(1);
+  h.lab = cv.take<uint32_t>(n);
+  h.par = cv.take<uint32_t>(n);
+  h.pre = cv.take<uint64_t>(std::max<uint64_t>(m, 1));
+  h.draws = cv.take<uint2>(n);
+  h.nv_out = cv.take<uint32_t>(1);
+  uint32_t* d_trial = cv.take<uint32_t>(1);
+  uint32_t* d_slot = cv.take<uint32_t>(1);
+  int64_t* d_misc = cv.take<int64_t>(2);
+  h.mat = nullptr;
+  const uint32_t zero = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(d_trial, &trial, 4, hipMemcpyHostToDevice, s));
+  CAC_HIP(ctx, hipMemcpyAsync(d_slot, &zero, 4, hipMemcpyHostToDevice, s));
+  CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, d_edges, m, n, t, seed, (const uint32_t*)d_trial,
+                     (const TrialScratch*)d_scr);
+  CAC_LAUNCH_CHECK(ctx, "sample trial");
+  uint32_t k = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&k, h.nv_out, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  CAC_HIP(ctx, hipMalloc((void**)&h.mat, std::max<size_t>((size_t)k * k, 1) * 8));
+  CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), 0, s, d_edges, m, n, k, (const TrialScratch*)d_scr,
+                     (const uint32_t*)d_slot, d_misc, d_misc + 1);
+  CAC_LAUNCH_CHECK(ctx, "densify trial");
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  out->lab = h.lab;
+  out->mat = h.mat;
+  out->k = k;
+  return CAC_OK;
+}
+
+void sample_out_free(SampleOut* o) {
+  if (o->mat) (void)hipFree(o->mat);
+  if (o->arena) (void)hipFree(o->arena);
+  *o = SampleOut{};
+}
+
+}  // namespace cac

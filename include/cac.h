@@ -110,6 +110,49 @@ int cac_ks_dense(cac_ctx* ctx, const int64_t* adj, uint32_t n, uint32_t seed, ui
 int cac_ks_dense_dev(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, uint32_t first_trial,
                      uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut, void* stream);
 
+/* ---- sizes and trial counts ------------------------------------------------
+ * Same floating point as the reference: comincut::number_of_trials
+ * (src/karger-stein/co_mincut.cpp:43-74), SquareRootCut::intermediate_size and
+ * numberOfTrials (src/SquareRootCut.cpp:63-83, base_case_multiplier 2).  The
+ * reference parses p with std::stof (src/executables/square_root.cpp:12); pass
+ * the float value to reproduce its trial counts. */
+uint32_t cac_ks_trials(uint32_t n, double p);
+uint32_t cac_intermediate_size(uint32_t n, uint64_t m);
+uint32_t cac_sqrt_trials(uint32_t n, uint64_t m, double p);
+
+/* ---- weighted iterated sampling ---------------------------------------------
+ * Replaces SequentialSquareRootCut::iteratedSampling + AdjacencyListGraph::
+ * finalizeContractionPhase / compact (src/SequentialSquareRootCut.cpp:21-67,
+ * src/AdjacencyListGraph.cpp:34-112) for one trial: rounds of |V| draws
+ * proportional to weight among the edges joining different super-vertices, draw d
+ * of round r from Philox-4x32-10(ctr = {trial, r, d, 0}, key = {seed, 'SAMP'}),
+ * r = 1 + floor(x * W / 2^64) on the 64-bit word x, edge = first inclusive weight
+ * prefix >= r; contraction in draw order stops as soon as |V| == t.
+ * vmap_out[v] (n entries) = id in [0, *n_out) of v's super-vertex, ordered by the
+ * smallest original vertex of each super-vertex; merged_out receives the merged
+ * edge list (u < v, sorted by (u, v), parallel weights summed; *m_out entries;
+ * CAC_ERANGE if merged_cap is smaller).  *n_out == t unless the graph has more
+ * than t components. */
+int cac_sample_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                        uint32_t trial, uint32_t* vmap_out, uint32_t* n_out, cac_edge* merged_out,
+                        uint64_t merged_cap, uint64_t* m_out);
+
+/* ---- square-root min cut ----------------------------------------------------
+ * One low-concurrency trial = SequentialSquareRootCut(graph, engine, t).compute()
+ * (src/SequentialSquareRootCut.cpp:4-19): the sampling contraction above, then
+ * one dense Karger-Stein trial (same seed, same trial id) on the merged graph.
+ * cac_sqrt_cut_trials returns the raw cut of trials first_trial.. (ntrials entries). */
+int cac_sqrt_cut_trials(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                        uint32_t first_trial, uint32_t ntrials, int64_t* raw_cuts);
+/* Replaces SquareRootCut::runMaster / runLowConcurrencyMaster (src/SquareRootCut.cpp:15-25,
+ * 86-148): t = cac_intermediate_size(n, m), T = cac_sqrt_trials(n, m, p) trials
+ * (ids 0..T-1) on one GPU, *cut = minimum raw cut, *trials = T.  Edges must have
+ * ids < n and weights whose sum stays below 2^61. */
+int cac_mincut(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed, int64_t* cut,
+               uint32_t* trials);
+int cac_mincut_dev(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, double p, uint32_t seed,
+                   int64_t* cut, uint32_t* trials, void* stream);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * R-MAT edge list on the device (the reference generates R-MAT inputs with
  * PaRMAT, input_generators/rmat_driver.sh:13-16).  Draw d makes 'scale' quadrant

@@ -78,3 +78,20 @@ def test_ks_rejects_bad_matrix(ctx):
     with pytest.raises(cac.CacError) as ei:
         ctx.ks_dense(A, 0, 0, 1)
     assert ei.value.code == cac.CAC_EINVAL
+
+
+@pytest.mark.parametrize("n", [300, 600, 1500])
+def test_ks_varying_trial_cuts(ctx, orc, n):
+    # weighted cycles: many near-minimum cuts, so raw trial cuts differ between
+    # trials and bit-exactness pins the whole pick sequence, not just the answer
+    rng = np.random.default_rng(n)
+    e = np.array([[i, (i + 1) % n, int(rng.integers(10**6, 2 * 10**6))] for i in range(n)], dtype=np.int64)
+    A = dense(n, e)
+    T = 10
+    _say(f"ks varying cycle n {n}")
+    raw, mn = ctx.ks_dense(A, 3, 0, T)
+    want = [orc.ks_trial(A, 3, t) for t in range(T)]
+    assert len(set(want)) > 1 or n < 400
+    assert raw.tolist() == want
+    raw2, _ = ctx.ks_dense(A, 3, 0, T)  # same stream, same cuts: no run-to-run drift
+    assert raw2.tolist() == want

@@ -1,0 +1,106 @@
+"""Sampling contraction and square-root min cut on the device, bit-exact against
+the oracle: the vertex map and merged edge list of one trial, every trial's raw
+cut, and the final cut against exact Stoer-Wagner."""
+import sys
+
+import numpy as np
+import pytest
+from graphgen import dense, erdos_renyi, planted
+
+import cac
+
+pytestmark = pytest.mark.gpu
+F32_95 = float(np.float32(0.95))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n,t,seed", [(60, 20, 0), (300, 40, 1), (1000, 318, 2), (2000, 130, 3)])
+def test_sample_contract_matches_oracle(ctx, orc, n, t, seed):
+    rng = np.random.default_rng(seed)
+    e = erdos_renyi(rng, n, min(1.0, 12.0 / n), wmax=50)
+    _say(f"sample_contract n {n} t {t} seed {seed}")
+    vmap, k, merged = ctx.sample_contract(e, n, t, 9, 4)
+    lab, nv = orc.sample_contract(n, e, t, 9, 4)
+    assert k == nv == t
+    roots = np.flatnonzero(lab == np.arange(n))
+    rank = np.zeros(n, dtype=np.int64)
+    rank[roots] = np.arange(len(roots))
+    assert np.array_equal(vmap, rank[lab])
+    M = orc.densify(n, e, lab)
+    iu = np.triu_indices(k, 1)
+    nz = M[iu] != 0
+    want = np.stack([iu[0][nz], iu[1][nz], M[iu][nz]], axis=1).astype(np.uint64)
+    assert np.array_equal(merged, want)
+
+
+@pytest.mark.parametrize("n,seed", [(200, 0), (1000, 1)])
+def test_sqrt_trials_bit_exact(ctx, orc, n, seed):
+    rng = np.random.default_rng(seed)
+    e = planted(rng, n, deg=12, wmax=40, c=3)
+    m = len(e)
+    t = orc.intermediate_size(n, m)
+    T = 12
+    _say(f"sqrt trials n {n} t {t}")
+    raw = ctx.sqrt_cut_trials(e, n, t, 7, 0, T)
+    want = [orc.sqrt_trial(n, e, t, 7, k) for k in range(T)]
+    assert raw.tolist() == want
+
+
+def test_sqrt_trials_varying_cuts(ctx, orc):
+    # weighted cycle with chords: trial cuts differ, so the comparison pins the
+    # draw sequence, the contraction prefix and the KS picks
+    rng = np.random.default_rng(5)
+    n = 700
+    e = [[i, (i + 1) % n, int(rng.integers(10**6, 2 * 10**6))] for i in range(n)]
+    e += [[int(a), int(b), int(rng.integers(1, 10**5))] for a, b in rng.integers(0, n, size=(n // 3, 2)) if a != b]
+    e = np.array(e, dtype=np.int64)
+    t = orc.intermediate_size(n, len(e))
+    _say(f"sqrt varying n {n} t {t}")
+    raw = ctx.sqrt_cut_trials(e, n, t, 1, 0, 16)
+    want = [orc.sqrt_trial(n, e, t, 1, k) for k in range(16)]
+    assert len(set(want)) > 1
+    assert raw.tolist() == want
+
+
+def test_mincut_er1000(ctx, orc):
+    # the square_root configuration: ER(1000, 0.05), unit weights, p = 0.95, seed 0
+    rng = np.random.default_rng(0)
+    n = 1000
+    e = erdos_renyi(rng, n, 0.05, wmax=1)
+    _say(f"mincut ER1000 m {len(e)}")
+    cut, T = ctx.mincut(e, n, F32_95, 0)
+    assert T == orc.sqrt_trials(n, len(e), F32_95)
+    assert cut == orc.stoer_wagner(dense(n, e))
+
+
+def test_mincut_planted(ctx, orc):
+    rng = np.random.default_rng(12)
+    n = 1500
+    e = planted(rng, n, deg=16, wmax=100, c=5)
+    _say("mincut planted n 1500")
+    cut, T = ctx.mincut(e, n, F32_95, 3)
+    assert cut == 5
+
+
+def test_mincut_reference_fixtures(ctx, orc):
+    from pathlib import Path
+
+    from graphfile import read_graph
+
+    gold = Path(__file__).resolve().parent / "golden" / "reference_inputs"
+    for name, want in (("cycle_4.graph", 200), ("cycle_6.graph", 10000), ("cycle_10.graph", 10),
+                       ("cycle_20.graph", 100), ("complete_4.graph", 85)):
+        n, e = read_graph(gold / name)
+        _say(f"mincut fixture {name}")
+        cut, _ = ctx.mincut(e.astype(np.int64), n, F32_95, 0)
+        assert cut == want, name
+
+
+def test_mincut_rejects_bad_ids(ctx):
+    with pytest.raises(cac.CacError) as ei:
+        ctx.mincut(np.array([[0, 7, 1]]), 5, 0.9, 0)
+    assert ei.value.code == cac.CAC_EINVAL
