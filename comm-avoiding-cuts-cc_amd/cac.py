@@ -65,6 +65,10 @@ SIGNATURES = {
     "cac_sqrt_cut_trials": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _u32, _pi64]),
     "cac_mincut": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
     "cac_mincut_dev": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32, _vp]),
+    "cac_approx_trials": (_u32, [ctypes.c_double]),
+    "cac_approx_cut": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pu64, _pu32]),
+    "cac_gen_complete_dev": (_i32, [_vp, _u32, _u32, _u32, _vp]),
+    "cac_gen_er": (_i32, [_u32, _u32, _u32, _u32, _vp, _u64, _pu64]),
 }
 
 _lib = None
@@ -243,7 +247,19 @@ class Context:
         self._check(self.lib.cac_mincut(self.handle, _ptr(e), len(e), n, p, seed, ctypes.byref(cut), ctypes.byref(T)))
         return int(cut.value), int(T.value)
 
+    def approx_cut(self, edges, n: int, p: float, seed: int):
+        """Returns (2^k approximation, trials) with ApproximateCut semantics."""
+        e = to_edges(edges)
+        a = ctypes.c_uint64()
+        T = ctypes.c_uint32()
+        self._check(self.lib.cac_approx_cut(self.handle, _ptr(e), len(e), n, p, seed, ctypes.byref(a),
+                                            ctypes.byref(T)))
+        return int(a.value), int(T.value)
+
     # ---- generators -------------------------------------------------------
+    def gen_complete_dev(self, n: int, seed: int, wmax: int, d_adj: int):
+        self._check(self.lib.cac_gen_complete_dev(self.handle, n, seed, wmax, d_adj))
+
     def gen_rmat_dev(self, scale: int, m: int, seed: int, thresholds, d_edges: int) -> int:
         ta, tab, tabc = thresholds
         used = ctypes.c_uint64()
@@ -275,6 +291,44 @@ def intermediate_size(n: int, m: int) -> int:
 
 def sqrt_trials(n: int, m: int, p: float) -> int:
     return int(load_library().cac_sqrt_trials(n, m, p))
+
+
+def gen_er(n: int, p: float, seed: int, wmax: int = 1):
+    """G(n, p) edge list (EDGE records), redrawn with seed+1, seed+2, ... until
+    connected, as the square_root configuration prescribes.  Returns (edges, seed used)."""
+    from fractions import Fraction
+
+    lib = load_library()
+    thr = int(Fraction(str(p)) * (1 << 32))
+    while True:
+        m = ctypes.c_uint64()
+        lib.cac_gen_er(n, thr, seed, wmax, None, 0, ctypes.byref(m))
+        out = np.empty(max(m.value, 1), dtype=EDGE)
+        rc = lib.cac_gen_er(n, thr, seed, wmax, _ptr(out), m.value, ctypes.byref(m))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_gen_er")
+        out = out[: m.value]
+        if _connected(n, out):
+            return out, seed
+        seed += 1
+
+
+def _connected(n: int, e: np.ndarray) -> bool:
+    par = np.arange(n)
+
+    def find(x):
+        while par[x] != x:
+            par[x] = par[par[x]]
+            x = par[x]
+        return x
+
+    comps = n
+    for u, v in zip(e["u"].tolist(), e["v"].tolist()):
+        a, b = find(u), find(v)
+        if a != b:
+            par[a] = b
+            comps -= 1
+    return comps == 1
 
 
 def rmat_thresholds(a: float, b: float, c: float):

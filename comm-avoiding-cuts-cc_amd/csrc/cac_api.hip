@@ -239,3 +239,41 @@ int cac_sample_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
 }
 
 }  // extern "C"
+
+namespace cac {
+int approx_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t copies, uint32_t iters,
+                   uint32_t seed, uint32_t* first_disconnected, hipStream_t s);
+}
+
+extern "C" {
+
+uint32_t cac_approx_trials(double p) {
+  double alpha = std::log(1 - std::exp(-(double)0.75));
+  return (uint32_t)std::ceil(std::log((double)1.0 - p) / alpha);
+}
+
+int cac_approx_cut(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed,
+                   uint64_t* approx, uint32_t* trials) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!edges || !approx || m == 0 || n < 2) return cac::fail(ctx, CAC_EINVAL, "need n >= 2 and at least one edge");
+  if (!(p > 0.0 && p < 1.0)) return cac::fail(ctx, CAC_EINVAL, "success probability must be in (0, 1)");
+  hipStream_t s = ctx->stream;
+  long double total = 0;
+  for (uint64_t e = 0; e < m; ++e) total += edges[e].w;
+  if (total < 1) return cac::fail(ctx, CAC_EINVAL, "total weight must be positive");
+  const uint32_t iters = (uint32_t)std::ceil(std::log((double)total));
+  const uint32_t T = cac_approx_trials(p);
+  cac_edge* d = nullptr;
+  rc = cac::upload_edges(ctx, edges, m, n, &d, s);
+  if (rc) return rc;
+  uint32_t first = 0;
+  rc = cac::approx_cut_run(ctx, d, m, n, T, iters, seed, &first, s);
+  (void)hipFree(d);
+  if (rc) return rc;
+  *approx = 1ull << first;
+  if (trials) *trials = T * first;
+  return CAC_OK;
+}
+
+}  // extern "C"

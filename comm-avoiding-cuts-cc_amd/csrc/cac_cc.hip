@@ -194,6 +194,41 @@ __global__ void __launch_bounds__(kCcBlock) k_check_ids(const cac_uedge* e, uint
   if (b) atomicOr(bad, 1u);
 }
 
+// ---- approximate min cut (ApproximateCut::run) -------------------------------
+// (1 - 2^-i)^w in 0.64 fixed point, floor(a*b/2^64) per multiply; ~0 stands for 1.
+__device__ inline uint64_t keep_threshold(uint32_t i, uint64_t w) {
+  if (w == 0) return ~0ull;
+  uint64_t base = (i >= 64) ? ~0ull : 0ull - (1ull << (64 - i));
+  bool have = false;
+  uint64_t acc = 0;
+  while (w) {
+    if (w & 1) {
+      acc = have ? __umul64hi(acc, base) : base;
+      have = true;
+    }
+    w >>= 1;
+    if (w) base = __umul64hi(base, base);
+  }
+  return acc;
+}
+
+// Edge e is kept in copy j at level i iff lo64(Philox(i, j, e)) >= threshold;
+// kept edges are hooked into copy j's slice [j n, (j + 1) n) of the forest.
+__global__ void __launch_bounds__(kCcBlock) k_approx_hook(const cac_edge* __restrict__ edges, uint64_t m, uint32_t n,
+                                                          uint32_t copies, uint32_t level, uint32_t seed,
+                                                          uint32_t* par) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; e < m; e += (uint64_t)gridDim.x * kCcBlock) {
+    const cac_edge x = edges[e];
+    if (x.w == 0 || x.u == x.v) continue;
+    const uint64_t q = keep_threshold(level, x.w);
+    for (uint32_t j = 0; j < copies; ++j) {
+      const u32x4 r = philox4x32(level, j, (uint32_t)e, (uint32_t)(e >> 32), seed, kDomApprox);
+      if (lo64(r) < q) continue;
+      uf_link(par, x.u + j * n, x.v + j * n);
+    }
+  }
+}
+
 }  // namespace
 
 // Launch sequence shared by cac_cc and cac_cc_dev.  'counter' is device memory
@@ -305,3 +340,36 @@ int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int lab
 }
 
 }  // extern "C"
+
+namespace cac {
+int approx_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t copies, uint32_t iters,
+                   uint32_t seed, uint32_t* first_disconnected, hipStream_t s) {
+  const uint64_t N = (uint64_t)n * copies;
+  if (N >= 0xFFFFFFFFull) return fail(ctx, CAC_EINVAL, "n * trials must fit in 32-bit vertex ids");
+  void* base = nullptr;
+  int rc = scratch_reserve(ctx, carve_size(N * 4) + 512, &base);
+  if (rc) return rc;
+  Carver cv(base);
+  uint32_t* par = cv.take<uint32_t>(N);
+  uint32_t* counter = cv.take<uint32_t>(64);
+  *first_disconnected = 0;
+  for (uint32_t i = 1; i <= iters; ++i) {
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(N, kCcBlock, 256 * 64)), dim3(kCcBlock), 0, s, par, (uint32_t)N);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1);
+    prof_begin(ctx, "approx_hook", s);
+    hipLaunchKernelGGL(k_approx_hook, dim3(grid_for(m, kCcBlock, 256 * 32)), dim3(kCcBlock), 0, s, d_edges, m, n,
+                       copies, i, seed, par);
+    prof_end(ctx, s);
+    hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(N, kCcBlock, 4096)), dim3(kCcBlock), 0, s, par, (uint32_t)N,
+                       counter);
+    CAC_LAUNCH_CHECK(ctx, "approx kernels");
+    CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, counter, 4, hipMemcpyDeviceToHost, s));
+    CAC_HIP(ctx, hipStreamSynchronize(s));
+    if (*(uint32_t*)ctx->pinned > copies) {
+      *first_disconnected = i;
+      break;
+    }
+  }
+  return CAC_OK;
+}
+}  // namespace cac

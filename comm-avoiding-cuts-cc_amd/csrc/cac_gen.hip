@@ -166,3 +166,57 @@ extern "C" int cac_gen_rmat_dev(cac_ctx* ctx, uint32_t scale, uint64_t m, uint32
   }
   return fail(ctx, CAC_EINVAL, "R-MAT: could not reach m distinct edges");
 }
+
+// ---- dense complete graph (C3 workload) and Erdos-Renyi edge lists ----------
+namespace cac {
+namespace {
+
+// w(u, v) = 1 + floor(wmax * x / 2^32), x = word 0 of Philox(ctr = {pair_lo, pair_hi,
+// 0, 0}, key = {seed, 'GENR'}), pair = u * n + v for u < v; symmetric, zero diagonal.
+__global__ void __launch_bounds__(256) k_gen_complete(uint32_t n, uint32_t seed, uint32_t wmax, int64_t* A) {
+  const uint64_t total = (uint64_t)n * n;
+  for (uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (uint64_t)gridDim.x * 256) {
+    const uint32_t r = (uint32_t)(q / n), c = (uint32_t)(q % n);
+    if (r == c) {
+      A[q] = 0;
+      continue;
+    }
+    const uint32_t u = r < c ? r : c, v = r < c ? c : r;
+    const uint64_t pair = (uint64_t)u * n + v;
+    const u32x4 x = philox4x32((uint32_t)pair, (uint32_t)(pair >> 32), 0u, 0u, seed, kDomGen);
+    A[q] = 1 + (int64_t)(((uint64_t)wmax * x.x) >> 32);
+  }
+}
+
+}  // namespace
+}  // namespace cac
+
+extern "C" int cac_gen_complete_dev(cac_ctx* ctx, uint32_t n, uint32_t seed, uint32_t wmax, int64_t* d_adj) {
+  using namespace cac;
+  int rc = bind(ctx);
+  if (rc) return rc;
+  if (!d_adj || n < 2 || wmax < 1) return fail(ctx, CAC_EINVAL, "need n >= 2, wmax >= 1");
+  hipLaunchKernelGGL(k_gen_complete, dim3(grid_for((uint64_t)n * n, 256, 256 * 64)), dim3(256), 0, ctx->stream, n, seed,
+                     wmax, d_adj);
+  CAC_LAUNCH_CHECK(ctx, "complete graph generator");
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+extern "C" int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_t wmax, cac_edge* out, uint64_t cap,
+                          uint64_t* m_out) {
+  // pair (u < v) with index u*n+v is kept iff word 0 of Philox(pair; seed,'GENR') <
+  // threshold; weight 1 + floor(wmax * word1 / 2^32).  Host-side (input generation).
+  if (!m_out || n < 2) return CAC_EINVAL;
+  uint64_t m = 0;
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t v = u + 1; v < n; ++v) {
+      const uint64_t pair = (uint64_t)u * n + v;
+      const cac::u32x4 x = cac::philox4x32((uint32_t)pair, (uint32_t)(pair >> 32), 0u, 0u, seed, cac::kDomGen);
+      if (x.x >= threshold) continue;
+      if (out && m < cap) out[m] = cac_edge{u, v, 1 + (((uint64_t)wmax * x.y) >> 32)};
+      ++m;
+    }
+  *m_out = m;
+  return (out && m > cap) ? CAC_ERANGE : CAC_OK;
+}

@@ -21,6 +21,11 @@
 namespace cac {
 namespace {
 
+// Byte offset of the 2 x kLeafMax int64 leaf-certificate scratch in dynamic LDS.
+__host__ __device__ inline size_t leaf_scratch_offset(uint32_t s) {
+  return ((13 * (size_t)s + 15) & ~(size_t)15) + 16 * kKsWaves + 64;
+}
+
 struct KsLds {
   int64_t* cap;      // [s]  capacities; rank/colmap scratch during compaction
   uint32_t* rep;     // [s]  representatives; colmap during compaction
@@ -28,6 +33,79 @@ struct KsLds {
   uint64_t* red;     // [kKsWaves] reduction scratch
   uint32_t* idx;     // [kKsWaves] index scratch
 };
+
+// ---- leaves --------------------------------------------------------------------
+// One wavefront per leaf: lane l owns vertices l and l + 64.
+constexpr int kLeafMax = 128;
+
+__device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    int64_t k2 = __shfl_xor(key, d, 64);
+    uint32_t i2 = __shfl_xor(idx, d, 64);
+    if (k2 > key || (k2 == key && i2 < idx)) {
+      key = k2;
+      idx = i2;
+    }
+  }
+}
+
+__device__ inline int64_t wave_sum_i64(int64_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// d - (q-1) M, saturating to a large negative value instead of overflowing
+// (a negative term only weakens the bound).
+__device__ inline int64_t leaf_f(int64_t d, int64_t M, uint32_t q) {
+  const int64_t k = (int64_t)q - 1;
+  if (M != 0 && k > INT64_MAX / 256 / M) return INT64_MIN / 256;
+  return d - k * M;
+}
+
+// Lower bound on the minimum cut (see leaf kernel): for |S| = q <= V/2,
+// cut(S) >= sum of the q smallest f_q(u) = d(u) - (q-1) M(u).
+__device__ int64_t leaf_lower_bound(const int64_t d[2], const int64_t M[2], const bool alive[2], uint32_t V,
+                                    int64_t stop_at) {
+  const int lane = threadIdx.x & 63;
+  int64_t best = INT64_MAX;
+  for (uint32_t q = 1; q <= V / 2; ++q) {
+    int64_t f0 = alive[0] ? leaf_f(d[0], M[0], q) : INT64_MAX;
+    int64_t f1 = alive[1] ? leaf_f(d[1], M[1], q) : INT64_MAX;
+    // bitonic sort of 128 keys: element e = lane + 64 * r lives in register r
+#pragma unroll
+    for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        if (j == 64) {
+          // partner of lane's element 0 is its element 1
+          bool up = (lane & k) == 0;  // k == 128 -> up for all
+          if ((f0 > f1) == up) {
+            int64_t t = f0;
+            f0 = f1;
+            f1 = t;
+          }
+        } else {
+          int64_t p0 = __shfl_xor(f0, j, 64), p1 = __shfl_xor(f1, j, 64);
+          bool lower = (lane & j) == 0;
+          bool up0 = ((lane) & k) == 0, up1 = ((lane + 64) & k) == 0;
+          // keep min when (lower == up)
+          int64_t mn0 = f0 < p0 ? f0 : p0, mx0 = f0 < p0 ? p0 : f0;
+          int64_t mn1 = f1 < p1 ? f1 : p1, mx1 = f1 < p1 ? p1 : f1;
+          f0 = (lower == up0) ? mn0 : mx0;
+          f1 = (lower == up1) ? mn1 : mx1;
+        }
+      }
+    }
+    // sum of the q smallest: elements 0..q-1 (element e in register e / 64 of lane e % 64)
+    int64_t part = ((uint32_t)lane < q ? f0 : 0) + ((uint32_t)(lane + 64) < q ? f1 : 0);
+    const int64_t lb = wave_sum_i64(part);
+    if (lb < best) best = lb;
+    if (best < stop_at) break;
+  }
+  return best;
+}
 
 __device__ inline const int64_t* row_of(const KsChild& c, const KsLds& L, uint32_t r) {
   return (L.flags[r] & 1u) ? c.work + (size_t)r * c.s : c.parent + (size_t)r * c.s;
@@ -46,6 +124,7 @@ __device__ int64_t ks_merge(const KsChild& c, const KsLds& L, uint32_t a, uint32
   const int64_t* rb = row_of(c, L, b);
   int64_t* dst = c.work + (size_t)b * s;
   uint64_t sum = 0;
+#pragma unroll 8
   for (uint32_t x = threadIdx.x; x < s; x += kKsBlock) {
     int64_t v = rb[x] + ra[x];
     if (L.rep[x] == b) v = 0;
@@ -113,7 +192,7 @@ __global__ void __launch_bounds__(kKsBlock) k_ks_contract(const KsChild* __restr
     }
     const int64_t* rowi = row_of(c, L, i);
     const uint64_t ci = (uint64_t)L.cap[i];
-    const uint32_t k = block_select([&](uint32_t x) { return rowi[x]; }, s, uniform1(hi64(rnd), ci), L.red, L.idx);
+    const uint32_t k = block_select_row(rowi, s, uniform1(hi64(rnd), ci), L.red, L.idx);
     if (k >= s) {
       bad = true;
       break;
@@ -210,80 +289,45 @@ __global__ void __launch_bounds__(kKsBlock) k_ks_contract(const KsChild* __restr
     __syncthreads();
     ++out_row;
   }
-  if (threadIdx.x == 0) k_out[blockIdx.x] = kk;
-}
-
-// ---- leaves --------------------------------------------------------------------
-// One wavefront per leaf: lane l owns vertices l and l + 64.
-constexpr int kLeafMax = 128;
-
-__device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
+  // A child that is already a leaf is checked right here: row sums and maxima of
+  // its kk x kk output feed the lower-bound certificate; when it proves every cut
+  // of the leaf is >= bound the leaf contributes exactly the bound and needs no
+  // Stoer-Wagner pass (flag bit 31 of k_out).
+  uint32_t flag = 0;
+  if (kk <= kLeafMax && kk >= 2) {
+    block_sync_global();
+    int64_t* dl = (int64_t*)(smem + leaf_scratch_offset(s));
+    int64_t* ml = dl + kLeafMax;
+    for (uint32_t r = threadIdx.x >> 6; r < kk; r += kKsWaves) {
+      const int64_t* row = c.out + (size_t)r * kk;
+      int64_t sm = 0, mx = 0;
+      for (uint32_t x = threadIdx.x & 63; x < kk; x += 64) {
+        const int64_t v = row[x];
+        sm += v;
+        mx = v > mx ? v : mx;
+      }
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    int64_t k2 = __shfl_xor(key, d, 64);
-    uint32_t i2 = __shfl_xor(idx, d, 64);
-    if (k2 > key || (k2 == key && i2 < idx)) {
-      key = k2;
-      idx = i2;
-    }
-  }
-}
-
-__device__ inline int64_t wave_sum_i64(int64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
-
-// d - (q-1) M, saturating to a large negative value instead of overflowing
-// (a negative term only weakens the bound).
-__device__ inline int64_t leaf_f(int64_t d, int64_t M, uint32_t q) {
-  const int64_t k = (int64_t)q - 1;
-  if (M != 0 && k > INT64_MAX / 256 / M) return INT64_MIN / 256;
-  return d - k * M;
-}
-
-// Lower bound on the minimum cut (see leaf kernel): for |S| = q <= V/2,
-// cut(S) >= sum of the q smallest f_q(u) = d(u) - (q-1) M(u).
-__device__ int64_t leaf_lower_bound(const int64_t d[2], const int64_t M[2], const bool alive[2], uint32_t V,
-                                    int64_t stop_at) {
-  const int lane = threadIdx.x & 63;
-  int64_t best = INT64_MAX;
-  for (uint32_t q = 1; q <= V / 2; ++q) {
-    int64_t f0 = alive[0] ? leaf_f(d[0], M[0], q) : INT64_MAX;
-    int64_t f1 = alive[1] ? leaf_f(d[1], M[1], q) : INT64_MAX;
-    // bitonic sort of 128 keys: element e = lane + 64 * r lives in register r
-#pragma unroll
-    for (int k = 2; k <= 128; k <<= 1) {
-#pragma unroll
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        if (j == 64) {
-          // partner of lane's element 0 is its element 1
-          bool up = (lane & k) == 0;  // k == 128 -> up for all
-          if ((f0 > f1) == up) {
-            int64_t t = f0;
-            f0 = f1;
-            f1 = t;
-          }
-        } else {
-          int64_t p0 = __shfl_xor(f0, j, 64), p1 = __shfl_xor(f1, j, 64);
-          bool lower = (lane & j) == 0;
-          bool up0 = ((lane) & k) == 0, up1 = ((lane + 64) & k) == 0;
-          // keep min when (lower == up)
-          int64_t mn0 = f0 < p0 ? f0 : p0, mx0 = f0 < p0 ? p0 : f0;
-          int64_t mn1 = f1 < p1 ? f1 : p1, mx1 = f1 < p1 ? p1 : f1;
-          f0 = (lower == up0) ? mn0 : mx0;
-          f1 = (lower == up1) ? mn1 : mx1;
-        }
+      for (int d = 32; d > 0; d >>= 1) {
+        sm += __shfl_xor(sm, d, 64);
+        const int64_t o = __shfl_xor(mx, d, 64);
+        mx = o > mx ? o : mx;
+      }
+      if ((threadIdx.x & 63) == 0) {
+        dl[r] = sm;
+        ml[r] = mx;
       }
     }
-    // sum of the q smallest: elements 0..q-1 (element e in register e / 64 of lane e % 64)
-    int64_t part = ((uint32_t)lane < q ? f0 : 0) + ((uint32_t)(lane + 64) < q ? f1 : 0);
-    const int64_t lb = wave_sum_i64(part);
-    if (lb < best) best = lb;
-    if (best < stop_at) break;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      const bool alive[2] = {(uint32_t)lane < kk, (uint32_t)lane + 64 < kk};
+      const int64_t d2[2] = {alive[0] ? dl[lane] : 0, alive[1] ? dl[lane + 64] : 0};
+      const int64_t m2[2] = {alive[0] ? ml[lane] : 0, alive[1] ? ml[lane + 64] : 0};
+      const int64_t lb = leaf_lower_bound(d2, m2, alive, kk, c.bound);
+      if (lb >= c.bound) flag = 0x80000000u;
+    }
   }
-  return best;
+  if (threadIdx.x == 0) k_out[blockIdx.x] = kk | flag;
 }
 
 __global__ void __launch_bounds__(64) k_ks_leaf(const KsLeaf* __restrict__ leaves, int64_t* __restrict__ results) {
@@ -400,9 +444,6 @@ __global__ void __launch_bounds__(256) k_min_degree(const int64_t* __restrict__ 
 __global__ void k_fill_i64(int64_t* p, uint32_t count, int64_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = v;
 }
-__global__ void k_copy_i64(int64_t* dst, const int64_t* src, uint32_t count) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) dst[i] = src[i];
-}
 
 }  // namespace
 
@@ -413,7 +454,7 @@ uint32_t ks_target_host(uint32_t s) {
 }
 
 namespace {
-size_t contract_lds_bytes(uint32_t s) { return ((13 * (size_t)s + 15) & ~(size_t)15) + 16 * kKsWaves + 64; }
+size_t contract_lds_bytes(uint32_t s) { return leaf_scratch_offset(s) + 2 * kLeafMax * 8; }
 inline uint32_t ks_target(uint32_t s) { return ks_target_host(s); }
 }  // namespace
 
@@ -515,6 +556,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     for (size_t i = 0; i < nchild; ++i) {
       if (kout[i] == 0xFFFFFFFFu) rc = fail(ctx, CAC_EDEVICE, "Karger-Stein selection out of range");
       if (kout[i] == 0 || kout[i] == 0xFFFFFFFFu) continue;  // disconnected child: its 0 is already recorded
+      if (kout[i] & 0x80000000u) continue;                     // certified leaf: contributes the bound
       next.push_back({ch[i].out, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound});
     }
     level.swap(next);

@@ -126,4 +126,51 @@ __device__ inline uint32_t block_select(Get get, uint32_t s, uint64_t r, uint64_
   return res;
 }
 
+// Same contract as block_select for a row in global memory: thread t owns the
+// contiguous chunk [t*c, (t+1)*c), c = ceil(s / kKsBlock) <= kKsMaxChunk, and keeps
+// it in registers, so one round of loads, one block scan and an in-register scan
+// by the owning thread replace the serial wave walk.  lds_u64: >= kKsWaves + 1.
+constexpr uint32_t kKsMaxChunk = kKsMaxLdsS / kKsBlock;
+__device__ inline uint32_t block_select_row(const int64_t* row, uint32_t s, uint64_t r, uint64_t* lds_u64,
+                                            uint32_t* lds_idx) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t c = (s + kKsBlock - 1) / kKsBlock;
+  const uint32_t lo = threadIdx.x * c;
+  if (threadIdx.x == 0) *lds_idx = 0xFFFFFFFFu;
+  uint64_t v[kKsMaxChunk];
+  uint64_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kKsMaxChunk; ++k) {
+    const uint32_t x = lo + k;
+    v[k] = (k < c && x < s) ? (uint64_t)row[x] : 0ull;
+    sum += v[k];
+  }
+  // block exclusive scan of the chunk sums
+  uint64_t inc = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) lds_u64[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0;
+  for (int w = 0; w < wave; ++w) before += lds_u64[w];
+  const uint64_t ex = before + inc - sum;
+  if (sum != 0 && ex < r && r <= ex + sum) {
+    uint64_t acc = ex;
+    uint32_t found = 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t k = 0; k < kKsMaxChunk; ++k) {
+      acc += v[k];
+      if (found == 0xFFFFFFFFu && v[k] != 0 && acc >= r) found = lo + k;
+    }
+    *lds_idx = found;
+  }
+  __syncthreads();
+  const uint32_t res = *lds_idx;
+  __syncthreads();
+  return res;
+}
+
 }  // namespace cac

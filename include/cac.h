@@ -153,6 +153,19 @@ int cac_mincut(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, doub
 int cac_mincut_dev(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, double p, uint32_t seed,
                    int64_t* cut, uint32_t* trials, void* stream);
 
+/* ---- approximate minimum cut ----------------------------------------------
+ * Replaces ApproximateCut::run (src/ApproximateCut.cpp:10-79) and numberOfTrials
+ * (:82-87).  For i = 1 .. ceil(ln W) (W = total weight): T = cac_approx_trials(p)
+ * disjoint copies each keep edge e iff binomial(w_e, 2^-i) > 0, decided exactly as
+ * x >= Q_i(w_e) with x = lo64 of Philox-4x32-10(ctr = {i, copy, e_lo, e_hi},
+ * key = {seed, 'APRX'}) and Q_i(w) = (1 - 2^-i)^w in 0.64 fixed point (square
+ * and multiply, each product floor(a*b / 2^64)); connected components of the
+ * union; stop at the first i with more than T components.
+ * *approx = 2^i (1 if no level disconnects), *trials = T * i.  n * T < 2^32. */
+uint32_t cac_approx_trials(double p);
+int cac_approx_cut(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed,
+                   uint64_t* approx, uint32_t* trials);
+
 /* ---- synthetic inputs ----------------------------------------------------
  * R-MAT edge list on the device (the reference generates R-MAT inputs with
  * PaRMAT, input_generators/rmat_driver.sh:13-16).  Draw d makes 'scale' quadrant
@@ -162,6 +175,19 @@ int cac_mincut_dev(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n
  * draw order kept, and d_edges (m entries) is written sorted by (u, v). */
 int cac_gen_rmat_dev(cac_ctx* ctx, uint32_t scale, uint64_t m, uint32_t seed, uint32_t ta,
                      uint32_t tab, uint32_t tabc, cac_uedge* d_edges, uint64_t* draws_used);
+
+/* Dense complete graph on the device (the dense Karger-Stein workload; the
+ * reference's complete_graph_generator.cpp writes such graphs as text):
+ * w(u, v) = 1 + floor(wmax * x / 2^32), x = word 0 of Philox-4x32-10(ctr =
+ * {p_lo, p_hi, 0, 0}, key = {seed, 'GENR'}), p = u * n + v for u < v; d_adj is n x n
+ * int64, symmetric, zero diagonal. */
+int cac_gen_complete_dev(cac_ctx* ctx, uint32_t n, uint32_t seed, uint32_t wmax, int64_t* d_adj);
+/* Erdos-Renyi G(n, threshold / 2^32) edge list on the host (the reference's
+ * er_generator.cpp): pair (u < v) kept iff word 0 of the same Philox draw <
+ * threshold, weight 1 + floor(wmax * word1 / 2^32).  Writes up to cap edges
+ * sorted by (u, v); *m_out = edges generated (CAC_ERANGE if more than cap). */
+int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_t wmax, cac_edge* out, uint64_t cap,
+               uint64_t* m_out);
 
 #ifdef __cplusplus
 }

@@ -559,3 +559,64 @@ void orc_ks_first_pick(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t t
     *b = i > k ? k : i;
     free(cap);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Approximate minimum cut                                                   */
+/* ------------------------------------------------------------------------- */
+#define DOM_APPROX 0x41505258u /* 'APRX' */
+
+static uint64_t mulhi64(uint64_t a, uint64_t b) { return (uint64_t)(((__uint128_t)a * b) >> 64); }
+
+/* (1 - 2^-i)^w as a 0.64 fixed-point fraction; UINT64_MAX stands for 1 (w == 0). */
+uint64_t orc_keep_threshold(uint32_t i, uint64_t w) {
+    if (w == 0) return UINT64_MAX;
+    uint64_t base = (i >= 64) ? UINT64_MAX : (uint64_t)0 - ((uint64_t)1 << (64 - i));
+    int have = 0;
+    uint64_t acc = 0;
+    while (w) {
+        if (w & 1) {
+            acc = have ? mulhi64(acc, base) : base;
+            have = 1;
+        }
+        w >>= 1;
+        if (w) base = mulhi64(base, base);
+    }
+    return acc;
+}
+
+void orc_approx_cut(uint32_t n, uint64_t m, const uint64_t* edges, double p, uint32_t seed, uint64_t* approx,
+                    uint32_t* trials) {
+    uint64_t W = 0;
+    for (uint64_t e = 0; e < m; ++e) W += edges[3 * e + 2];
+    uint32_t iters = (uint32_t)ceil(log((double)W));
+    uint32_t T = orc_approx_trials(p);
+    uint32_t first = 0;
+    uint64_t N = (uint64_t)n * T;
+    uint32_t* lab = (uint32_t*)malloc(sizeof(uint32_t) * (N ? N : 1));
+    uint32_t* sub = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (m ? m : 1) * T);
+    uint32_t key[2] = {seed, DOM_APPROX};
+    for (uint32_t i = 1; i <= iters; ++i) {
+        uint64_t k = 0;
+        for (uint32_t j = 0; j < T; ++j) {
+            for (uint64_t e = 0; e < m; ++e) {
+                uint32_t ctr[4] = {i, j, (uint32_t)e, (uint32_t)(e >> 32)}, o[4];
+                orc_philox4x32_10(ctr, key, o);
+                uint64_t x = ((uint64_t)o[1] << 32) | o[0];
+                uint64_t Q = orc_keep_threshold(i, edges[3 * e + 2]);
+                if (edges[3 * e + 2] == 0 || x < Q) continue;
+                sub[2 * k] = (uint32_t)edges[3 * e] + j * n;
+                sub[2 * k + 1] = (uint32_t)edges[3 * e + 1] + j * n;
+                ++k;
+            }
+        }
+        uint32_t comps = orc_cc((uint32_t)N, k, sub, 0, lab);
+        if (comps > T) {
+            first = i;
+            break;
+        }
+    }
+    *approx = (uint64_t)1 << first;
+    *trials = T * first;
+    free(lab);
+    free(sub);
+}

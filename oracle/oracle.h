@@ -113,6 +113,18 @@ uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32
  * src/SequentialSquareRootCut.cpp:4-19). */
 int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial);
 
+/* ---- approximate minimum cut ------------------------------------------------
+ * Restates ApproximateCut::run (src/ApproximateCut.cpp:10-79): for i = 1 ..
+ * ceil(ln W) (W = total weight), copies j < T (T = orc_approx_trials(p)) keep edge e
+ * iff binomial(w_e, 2^-i) > 0, here exactly: x >= Q_i(w_e), x = lo64 of
+ * Philox(ctr={i, j, e_lo, e_hi}, key={seed,'APRX'}), Q_i(w) = (1 - 2^-i)^w in
+ * 0.64 fixed point by square-and-multiply with floor(a*b/2^64); stop at the first
+ * i whose union of copies has more than T components.  *approx = 2^i (1 if none),
+ * *trials = T * i. */
+uint64_t orc_keep_threshold(uint32_t i, uint64_t w);
+void orc_approx_cut(uint32_t n, uint64_t m, const uint64_t* edges, double p, uint32_t seed, uint64_t* approx,
+                    uint32_t* trials);
+
 #ifdef __cplusplus
 }
 #endif

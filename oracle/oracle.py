@@ -40,6 +40,8 @@ SIGS = {
     "orc_densify": (_u32, [_u32, _u64, _vp, _vp, _vp]),
     "orc_sqrt_trial": (_i64, [_u32, _u64, _vp, _u32, _u32, _u32]),
     "orc_ks_first_pick": (None, [_u32, _vp, _u32, _u32, _u32, _vp, _vp]),
+    "orc_keep_threshold": (_u64, [_u32, _u64]),
+    "orc_approx_cut": (None, [_u32, _u64, _vp, ctypes.c_double, _u32, _vp, _vp]),
 }
 
 _lib = None
@@ -177,6 +179,18 @@ def densify(n: int, edges, labels):
 def sqrt_trial(n: int, edges, t: int, seed: int, trial: int) -> int:
     e = _edges3(edges)
     return int(lib().orc_sqrt_trial(n, e.shape[0], _p(e), t, seed, trial))
+
+
+def keep_threshold(i: int, w: int) -> int:
+    return int(lib().orc_keep_threshold(i, w))
+
+
+def approx_cut(n: int, edges, p: float, seed: int):
+    e = _edges3(edges)
+    a = np.zeros(1, np.uint64)
+    t = np.zeros(1, np.uint32)
+    lib().orc_approx_cut(n, e.shape[0], _p(e), p, seed, _p(a), _p(t))
+    return int(a[0]), int(t[0])
 
 
 def edges_to_dense(n: int, edges: np.ndarray) -> np.ndarray:

@@ -152,3 +152,35 @@ def test_sqrt_trials_find_planted_cut(orc):
     cuts = [orc.sqrt_trial(n, e, t, 4, k) for k in range(T)]
     assert min(cuts) == 3
     assert all(c >= 3 for c in cuts)
+
+
+def test_keep_threshold_closed_forms(orc):
+    # Q_i(w) ~ (1 - 2^-i)^w * 2^64 from below; floor rounding in the squarings
+    # compounds to a relative error far below 2^-40 (a wrong base or a missed
+    # multiply would be off by orders of magnitude)
+    from fractions import Fraction
+
+    assert orc.keep_threshold(1, 1) == 1 << 63
+    assert orc.keep_threshold(2, 1) == 3 << 62
+    assert orc.keep_threshold(5, 0) == (1 << 64) - 1
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 80
+    for i in (1, 2, 3, 7, 20, 40, 63):
+        for w in (1, 2, 3, 5, 100, 12345, 10**6):
+            q = orc.keep_threshold(i, w)
+            if w <= 100:
+                exact = Fraction(2**i - 1, 2**i) ** w * 2**64
+                assert q <= exact and exact - q <= exact / 2**40 + 4
+            else:  # (1 - 2^-i)^w = exp(w ln(1 - 2^-i)) to 80 digits
+                exact = (Decimal(w) * (Decimal(1) - Decimal(2) ** -i).ln()).exp() * (Decimal(2) ** 64)
+                assert Decimal(q) <= exact and exact - Decimal(q) <= exact / Decimal(2) ** 40 + 4
+
+
+def test_approx_oracle_is_log_approximation(orc):
+    rng = np.random.default_rng(4)
+    for c in (1, 3, 8):
+        e = planted(rng, 300, deg=10, wmax=100, c=c)
+        a, t = orc.approx_cut(300, e, F32_95, 1)
+        assert a & (a - 1) == 0 and t % 5 == 0
+        assert c / 64 <= a <= 64 * c
