@@ -49,6 +49,7 @@ SIGNATURES = {
     "cac_dev_free": (_i32, [_vp, _vp]),
     "cac_memcpy_h2d": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
     "cac_memcpy_d2h": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "cac_memcpy_d2d": (_i32, [_vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "cac_prof_enable": (_i32, [_vp, _i32]),
     "cac_prof_reset": (_i32, [_vp]),
     "cac_prof_count": (_i32, [_vp]),
@@ -166,6 +167,10 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.cac_ctx_synchronize(self.handle))
+
+    def copy_dev(self, dst: int, src: int, nbytes: int):
+        """Enqueue a device-to-device copy kernel on the context's stream."""
+        self._check(self.lib.cac_memcpy_d2d(self.handle, dst, src, nbytes, None))
 
     def prof_enable(self, on: bool = True):
         self._check(self.lib.cac_prof_enable(self.handle, 1 if on else 0))

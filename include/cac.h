@@ -68,6 +68,11 @@ int cac_dev_alloc(cac_ctx* ctx, size_t bytes, void** dptr);
 int cac_dev_free(cac_ctx* ctx, void* dptr);
 int cac_memcpy_h2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes);
 int cac_memcpy_d2h(cac_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a streaming kernel (16-byte aligned pointers),
+ * enqueued on `stream` (NULL = the context's stream) without synchronising.
+ * Reads and writes exactly `bytes` bytes, so it also calibrates HBM byte
+ * counters.  No counterpart in the reference. */
+int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 
 /* Per-kernel timing with HIP events on the stream each kernel is launched on.
  * No counterpart in the reference beyond TimeUtils::measure (src/utils.hpp:21-40). */

@@ -62,6 +62,34 @@ def test_cc_large_sampled_path(ctx, orc):
     assert np.array_equal(lab, lab2)  # canonical labels are run-to-run deterministic
 
 
+@pytest.mark.parametrize("shape", ["sorted", "islands", "one_run", "ragged_tail"])
+def test_cc_sorted_runs(ctx, orc, shape):
+    """The run-sampling + giant-bitmap path on u-sorted lists: long runs, no
+    giant component (many equal-size islands), a single run of one u, and a
+    length that leaves a partial last tile."""
+    rng = np.random.default_rng(["sorted", "islands", "one_run", "ragged_tail"].index(shape) + 100)
+    if shape == "sorted":
+        n, m = 1 << 18, (1 << 19) + 3
+        e = rng.integers(0, n, size=(m, 2)).astype(np.uint32)
+    elif shape == "islands":  # 2^16 islands of 8 vertices, no component dominates
+        n, m = 1 << 19, 1 << 19
+        isl = rng.integers(0, n // 8, size=m).astype(np.uint32)
+        e = np.stack([isl * 8 + rng.integers(0, 8, size=m), isl * 8 + rng.integers(0, 8, size=m)], 1).astype(np.uint32)
+    elif shape == "one_run":  # one star: every edge has u = 5
+        n, m = 1 << 18, 1 << 18
+        e = np.stack([np.full(m, 5), rng.integers(0, n, size=m)], 1).astype(np.uint32)
+    else:
+        n, m = 3000, (1 << 18) + 511
+        e = rng.integers(0, n, size=(m, 2)).astype(np.uint32)
+    e = e[np.lexsort((e[:, 1], e[:, 0]))]
+    _say(f"cc sorted runs {shape} n {n} m {m}")
+    for mode in (cac.LABEL_MIN_ID, cac.LABEL_RANK):
+        lab, nc = ctx.cc(e, n, mode)
+        want, wn = orc.cc(e, n, mode)
+        assert nc == wn
+        assert np.array_equal(lab, want)
+
+
 def test_cc_edge_cases(ctx):
     _say("cc edge cases")
     lab, nc = ctx.cc(np.zeros((0, 2), np.uint32), 4)
@@ -106,3 +134,24 @@ def test_cc_rmat_full_size(ctx, orc):
     assert nc == wn
     assert np.array_equal(lab, want)
     _say(f"rmat full size: {nc} components")
+
+
+@pytest.mark.parametrize("nbytes", [16, 1000, (1 << 20) + 7])
+def test_copy_kernel(ctx, nbytes):
+    """The byte-counter calibration copy moves exactly nbytes (vector body + tail)."""
+    _say(f"copy kernel {nbytes} bytes")
+    rng = np.random.default_rng(nbytes)
+    src = rng.integers(0, 256, size=nbytes, dtype=np.uint8)
+    a = ctx.alloc(nbytes)
+    b = ctx.alloc(nbytes + 64)
+    try:
+        a.upload(src)
+        b.upload(np.full(nbytes + 64, 0xAB, dtype=np.uint8))
+        ctx.copy_dev(b.ptr, a.ptr, nbytes)
+        ctx.synchronize()
+        got = b.download(np.uint8, nbytes + 64)
+        assert np.array_equal(got[:nbytes], src)
+        assert np.all(got[nbytes:] == 0xAB)
+    finally:
+        a.free()
+        b.free()
