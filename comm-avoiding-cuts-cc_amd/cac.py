@@ -28,6 +28,8 @@ LABEL_RANK = 1
 # Edge records, identical to cac_uedge / cac_edge.
 UEDGE = np.dtype([("u", "<u4"), ("v", "<u4")])
 EDGE = np.dtype([("u", "<u4"), ("v", "<u4"), ("w", "<u8")])
+# cac_graph_desc (24 bytes): one graph of a batch.
+GRAPH_DESC = np.dtype([("edge_offset", "<u8"), ("m", "<u8"), ("n", "<u4"), ("trials", "<u4")])
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -70,6 +72,11 @@ SIGNATURES = {
     "cac_approx_cut": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pu64, _pu32]),
     "cac_gen_complete_dev": (_i32, [_vp, _u32, _u32, _u32, _vp]),
     "cac_gen_er": (_i32, [_u32, _u32, _u32, _u32, _vp, _u64, _pu64]),
+    "cac_mincut_batch": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
+    "cac_mincut_batch_dev": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32, _vp]),
+    "cac_gen_planted_sizes": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _pu32, _pu64]),
+    "cac_gen_planted": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _u32, _vp, _vp, _u64, _pu64,
+                               _pu32]),
 }
 
 _lib = None
@@ -252,6 +259,25 @@ class Context:
         self._check(self.lib.cac_mincut(self.handle, _ptr(e), len(e), n, p, seed, ctypes.byref(cut), ctypes.byref(T)))
         return int(cut.value), int(T.value)
 
+    def mincut_batch(self, edges, descs: np.ndarray, p: float, seed: int):
+        """Batched square-root cut over graphs descs (GRAPH_DESC records) whose
+        edges lie back to back in edges.  Returns (cuts, trials) arrays."""
+        e = to_edges(edges)
+        d = np.ascontiguousarray(descs, dtype=GRAPH_DESC)
+        cuts = np.empty(len(d), dtype=np.int64)
+        trials = np.empty(len(d), dtype=np.uint32)
+        self._check(self.lib.cac_mincut_batch(self.handle, _ptr(e), _ptr(d), len(d), p, seed,
+                                              cuts.ctypes.data_as(_pi64), trials.ctypes.data_as(_pu32)))
+        return cuts, trials
+
+    def mincut_batch_dev(self, d_edges: int, descs: np.ndarray, p: float, seed: int, stream=None):
+        d = np.ascontiguousarray(descs, dtype=GRAPH_DESC)
+        cuts = np.empty(len(d), dtype=np.int64)
+        trials = np.empty(len(d), dtype=np.uint32)
+        self._check(self.lib.cac_mincut_batch_dev(self.handle, d_edges, _ptr(d), len(d), p, seed,
+                                                  cuts.ctypes.data_as(_pi64), trials.ctypes.data_as(_pu32), stream))
+        return cuts, trials
+
     def approx_cut(self, edges, n: int, p: float, seed: int):
         """Returns (2^k approximation, trials) with ApproximateCut semantics."""
         e = to_edges(edges)
@@ -316,6 +342,28 @@ def gen_er(n: int, p: float, seed: int, wmax: int = 1):
         if _connected(n, out):
             return out, seed
         seed += 1
+
+
+def gen_planted(first: int, count: int, seed: int, nmin: int = 256, nmax: int = 65536, avg_degree: float = 16.0,
+                wmax: int = 100, cmax: int = 8):
+    """Planted-cut graphs g = first .. first+count-1 (see cac_gen_planted).
+    Returns (descs GRAPH_DESC[count], edges EDGE[m_total], planted uint32[count])."""
+    lib = load_library()
+    ns = np.empty(max(count, 1), dtype=np.uint32)
+    bound = ctypes.c_uint64()
+    rc = lib.cac_gen_planted_sizes(first, count, seed, nmin, nmax, avg_degree, cmax, ns.ctypes.data_as(_pu32),
+                                   ctypes.byref(bound))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_planted_sizes")
+    descs = np.empty(max(count, 1), dtype=GRAPH_DESC)
+    edges = np.empty(max(bound.value, 1), dtype=EDGE)
+    planted = np.empty(max(count, 1), dtype=np.uint32)
+    m = ctypes.c_uint64()
+    rc = lib.cac_gen_planted(first, count, seed, nmin, nmax, avg_degree, wmax, cmax, _ptr(descs), _ptr(edges),
+                             len(edges), ctypes.byref(m), planted.ctypes.data_as(_pu32))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_planted")
+    return descs[:count], edges[: m.value], planted[:count]
 
 
 def _connected(n: int, e: np.ndarray) -> bool:

@@ -28,6 +28,16 @@ uint32_t ks_target_host(uint32_t s);
 uint32_t ks_trials_host(uint32_t n, double p);
 uint32_t sqrt_intermediate_size(uint32_t n, uint64_t m);
 uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p);
+// One sqrt-cut job: trial 'trial' of the graph (edges, m, n) shrunk to t
+// super-vertices, raw cut to d_results[slot].
+struct SqrtJob {
+  const cac_edge* edges;
+  uint64_t m;
+  uint32_t n, t, trial, slot;
+};
+constexpr size_t kMaxJobsPerChunk = 8192;
+// Jobs of any mix of graphs, in memory-sized chunks of shared launches.
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s);
 // Sqrt-cut trials on a device edge list: raw cut of trial_ids[i] to d_results[i].
 int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
                  const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s);

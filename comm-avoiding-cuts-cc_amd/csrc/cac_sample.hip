@@ -34,13 +34,19 @@ namespace {
 constexpr int kSmpBlock = 1024;
 constexpr int kSmpWaves = kSmpBlock / 64;
 
+// One sampling job = one sqrt-cut trial of one graph; a launch runs one
+// workgroup per job, so trials of different graphs share a launch.
 struct TrialScratch {
-  uint32_t* lab;    // [n]
-  uint32_t* par;    // [n]
-  uint64_t* pre;    // [m]
-  uint2* draws;     // [n]
-  int64_t* mat;     // [t x t]
-  uint32_t* nv_out; // [1]
+  const cac_edge* edges;  // the job's graph (ids local to it)
+  uint64_t m;
+  uint32_t n, t;          // vertices, intermediate size
+  uint32_t trial, slot;   // Philox trial id, result slot
+  uint32_t* lab;          // [n]
+  uint32_t* par;          // [n]
+  uint64_t* pre;          // [m]
+  uint2* draws;           // [n]
+  int64_t* mat;           // [t x t]
+  uint32_t* nv_out;       // [1]
 };
 
 __device__ inline uint64_t wsum64(uint64_t x) {
@@ -93,12 +99,12 @@ __device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t* par, uint32_
   return (uint32_t)bsum64(mine, lds);
 }
 
-__global__ void __launch_bounds__(kSmpBlock) k_sample_trials(const cac_edge* __restrict__ edges, uint64_t m, uint32_t n,
-                                                             uint32_t t, uint32_t seed, const uint32_t* __restrict__ trials,
-                                                             const TrialScratch* __restrict__ scratch) {
+__global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, const TrialScratch* __restrict__ scratch) {
   __shared__ uint64_t lds[kSmpWaves + 2];
   const TrialScratch S = scratch[blockIdx.x];
-  const uint32_t trial = trials[blockIdx.x];
+  const cac_edge* __restrict__ edges = S.edges;
+  const uint64_t m = S.m;
+  const uint32_t n = S.n, t = S.t, trial = S.trial;
   for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = v;
   block_sync_global();
   uint32_t nv = n;
@@ -188,18 +194,19 @@ __global__ void __launch_bounds__(kSmpBlock) k_sample_trials(const cac_edge* __r
 // Rank the canonical labels, accumulate the surviving edges into the k x k
 // matrix (k == t unless the sampling stopped early on a disconnected graph) and
 // compute the trial's bound (minimum weighted degree).
-__global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const cac_edge* __restrict__ edges, uint64_t m, uint32_t n,
-                                                              uint32_t t, const TrialScratch* __restrict__ scratch,
-                                                              const uint32_t* __restrict__ slots, int64_t* bounds,
+// bounds is indexed by job (blockIdx.x), results by the job's slot.
+__global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch* __restrict__ scratch, int64_t* bounds,
                                                               int64_t* results) {
   __shared__ uint32_t wtot[kSmpWaves];
   __shared__ uint64_t lds[kSmpWaves + 2];
   const TrialScratch S = scratch[blockIdx.x];
-  const uint32_t slot = slots[blockIdx.x];
+  const cac_edge* __restrict__ edges = S.edges;
+  const uint64_t m = S.m;
+  const uint32_t n = S.n, t = S.t, slot = S.slot;
   const uint32_t k = *S.nv_out;
   if (k > t) {  // disconnected: a cut of weight 0 exists
     if (threadIdx.x == 0) {
-      bounds[slot] = 0;
+      bounds[blockIdx.x] = 0;
       results[slot] = 0;
     }
     return;
@@ -250,7 +257,7 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const cac_edge* __
   if (threadIdx.x == 0) {
     uint64_t b = ~0ull;
     for (int w = 0; w < kSmpWaves; ++w) b = lds[w] < b ? lds[w] : b;
-    bounds[slot] = (int64_t)b;
+    bounds[blockIdx.x] = (int64_t)b;
     results[slot] = (int64_t)b;
   }
 }
@@ -286,57 +293,75 @@ uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p) {
   return (uint32_t)std::ceil((n * (n * std::log(cp))) / (2.0 * 2.0 * ((uint32_t)m * pk)));
 }
 
-// Runs sqrt-cut trials trial_ids on a device edge list; raw cut per trial to
-// d_results[0..count).  t: intermediate size.
-int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
-                 const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s) {
-  const uint32_t count = (uint32_t)trial_ids.size();
-  if (count == 0) return CAC_OK;
-  const size_t per = carve_size((size_t)n * 4) * 2 + carve_size(std::max<uint64_t>(m, 1) * 8) +
-                     carve_size((size_t)n * 8) + carve_size((size_t)t * t * 8) + 256;
+// Device bytes one sampling job needs beside the Karger-Stein levels.
+static size_t job_sample_bytes(const SqrtJob& j) {
+  return carve_size((size_t)j.n * 4) * 2 + carve_size(std::max<uint64_t>(j.m, 1) * 8) + carve_size((size_t)j.n * 8) +
+         carve_size((size_t)j.t * j.t * 8) + 256;
+}
+// Upper estimate of the Karger-Stein level arenas of one root of size t: each
+// level holds about t^2 entries in copy-on-write work rows plus the contracted
+// children, and two levels are alive at once.
+static size_t job_ks_bytes(const SqrtJob& j) { return (size_t)j.t * j.t * 8 * 8 + 4096; }
+
+// Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
+// one sampling launch, one densify launch and one breadth-first Karger-Stein
+// pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
+// i goes to d_results[jobs[i].slot].
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s) {
+  if (jobs.empty()) return CAC_OK;
   size_t free_b = 0, total_b = 0;
   CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-  size_t batch = std::max<size_t>(1, (size_t)(0.4 * (double)free_b) / per);
-  batch = std::min<size_t>({batch, (size_t)count, (size_t)4096});
+  const size_t budget = std::max<size_t>((size_t)(0.4 * (double)free_b), (size_t)1 << 28);
   int rc = CAC_OK;
-  for (uint32_t b0 = 0; b0 < count && rc == CAC_OK; b0 += (uint32_t)batch) {
-    const uint32_t nb = std::min<uint32_t>(count - b0, (uint32_t)batch);
-    const size_t bytes = per * nb + carve_size(nb * sizeof(TrialScratch)) + 3 * carve_size(nb * 8) + 1024;
+  size_t b0 = 0;
+  while (b0 < jobs.size() && rc == CAC_OK) {
+    size_t nb = 0, sample_bytes = 0, ks_bytes = 0;
+    while (b0 + nb < jobs.size() && nb < kMaxJobsPerChunk) {
+      const SqrtJob& j = jobs[b0 + nb];
+      const size_t a = job_sample_bytes(j), k = job_ks_bytes(j);
+      if (nb > 0 && sample_bytes + a + ks_bytes + k > budget) break;
+      sample_bytes += a;
+      ks_bytes += k;
+      ++nb;
+    }
+    const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) + 1024;
     void* arena = nullptr;
     hipError_t e = hipMalloc(&arena, bytes);
-    if (e != hipSuccess) return hip_fail(ctx, e, "sqrt-cut trial scratch");
+    if (e != hipSuccess) return hip_fail(ctx, e, "sqrt-cut job scratch");
     Carver cv(arena);
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
     uint32_t* d_nv = cv.take<uint32_t>(nb);
-    uint32_t* d_trials = cv.take<uint32_t>(nb);
-    uint32_t* d_slots = cv.take<uint32_t>(nb);
     int64_t* d_bounds = cv.take<int64_t>(nb);
     std::vector<TrialScratch> scr(nb);
-    std::vector<uint32_t> tids(nb), slots(nb);
-    for (uint32_t i = 0; i < nb; ++i) {
-      scr[i].lab = cv.take<uint32_t>(n);
-      scr[i].par = cv.take<uint32_t>(n);
-      scr[i].pre = cv.take<uint64_t>(std::max<uint64_t>(m, 1));
-      scr[i].draws = cv.take<uint2>(n);
-      scr[i].mat = cv.take<int64_t>((size_t)t * t);
-      scr[i].nv_out = d_nv + i;
-      tids[i] = trial_ids[b0 + i];
-      slots[i] = b0 + i;
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      TrialScratch& x = scr[i];
+      x.edges = j.edges;
+      x.m = j.m;
+      x.n = j.n;
+      x.t = j.t;
+      x.trial = j.trial;
+      x.slot = j.slot;
+      x.lab = cv.take<uint32_t>(j.n);
+      x.par = cv.take<uint32_t>(j.n);
+      x.pre = cv.take<uint64_t>(std::max<uint64_t>(j.m, 1));
+      x.draws = cv.take<uint2>(j.n);
+      x.mat = cv.take<int64_t>((size_t)j.t * j.t);
+      x.nv_out = d_nv + i;
     }
-    (void)hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(d_trials, tids.data(), nb * 4, hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(d_slots, slots.data(), nb * 4, hipMemcpyHostToDevice, s);
-    prof_begin(ctx, "sample_trials", s);
-    hipLaunchKernelGGL(k_sample_trials, dim3(nb), dim3(kSmpBlock), 0, s, d_edges, m, n, t, seed,
-                       (const uint32_t*)d_trials, (const TrialScratch*)d_scr);
-    prof_end(ctx, s);
-    // results and bounds are indexed by batch-local slots here
-    prof_begin(ctx, "densify_trials", s);
-    hipLaunchKernelGGL(k_densify_trials, dim3(nb), dim3(kSmpBlock), 0, s, d_edges, m, n, t,
-                       (const TrialScratch*)d_scr, (const uint32_t*)d_slots, d_bounds - b0, d_results);
-    prof_end(ctx, s);
-    e = hipGetLastError();
-    if (e != hipSuccess) rc = hip_fail(ctx, e, "sampling kernels");
+    e = hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
+    if (rc == CAC_OK) {
+      prof_begin(ctx, "sample_trials", s);
+      hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
+      prof_end(ctx, s);
+      prof_begin(ctx, "densify_trials", s);
+      hipLaunchKernelGGL(k_densify_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, (const TrialScratch*)d_scr,
+                         d_bounds, d_results);
+      prof_end(ctx, s);
+      e = hipGetLastError();
+      if (e != hipSuccess) rc = hip_fail(ctx, e, "sampling kernels");
+    }
     std::vector<uint32_t> nvs(nb);
     std::vector<int64_t> hb(nb);
     if (rc == CAC_OK) {
@@ -347,14 +372,23 @@ int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, 
     }
     if (rc == CAC_OK) {
       std::vector<KsRoot> roots;
-      for (uint32_t i = 0; i < nb; ++i)
-        if (nvs[i] >= 2 && nvs[i] <= t) roots.push_back({scr[i].mat, nvs[i], tids[i], b0 + i, hb[i]});
+      for (size_t i = 0; i < nb; ++i)
+        if (nvs[i] >= 2 && nvs[i] <= scr[i].t) roots.push_back({scr[i].mat, nvs[i], scr[i].trial, scr[i].slot, hb[i]});
       rc = ks_run(ctx, roots, seed, d_results, s);
     }
     (void)hipStreamSynchronize(s);
     (void)hipFree(arena);
+    b0 += nb;
   }
   return rc;
+}
+
+// Trials trial_ids of one graph; raw cut of trial_ids[i] to d_results[i].
+int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
+                 const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s) {
+  std::vector<SqrtJob> jobs(trial_ids.size());
+  for (size_t i = 0; i < trial_ids.size(); ++i) jobs[i] = {d_edges, m, n, t, trial_ids[i], (uint32_t)i};
+  return sqrt_cut_batch(ctx, jobs, seed, d_results, s);
 }
 
 }  // namespace cac
@@ -371,30 +405,31 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   CAC_HIP(ctx, hipMalloc(&out->arena, bytes));
   Carver cv(out->arena);
   TrialScratch h;
+  h.edges = d_edges;
+  h.m = m;
+  h.n = n;
+  h.t = t;
+  h.trial = trial;
+  h.slot = 0;
   TrialScratch* d_scr = cv.take<TrialScratch>(1);
   h.lab = cv.take<uint32_t>(n);
   h.par = cv.take<uint32_t>(n);
   h.pre = cv.take<uint64_t>(std::max<uint64_t>(m, 1));
   h.draws = cv.take<uint2>(n);
   h.nv_out = cv.take<uint32_t>(1);
-  uint32_t* d_trial = cv.take<uint32_t>(1);
-  uint32_t* d_slot = cv.take<uint32_t>(1);
   int64_t* d_misc = cv.take<int64_t>(2);
   h.mat = nullptr;
-  const uint32_t zero = 0;
-  CAC_HIP(ctx, hipMemcpyAsync(d_trial, &trial, 4, hipMemcpyHostToDevice, s));
-  CAC_HIP(ctx, hipMemcpyAsync(d_slot, &zero, 4, hipMemcpyHostToDevice, s));
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, d_edges, m, n, t, seed, (const uint32_t*)d_trial,
-                     (const TrialScratch*)d_scr);
+  hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
   CAC_LAUNCH_CHECK(ctx, "sample trial");
   uint32_t k = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&k, h.nv_out, 4, hipMemcpyDeviceToHost, s));
   CAC_HIP(ctx, hipStreamSynchronize(s));
   CAC_HIP(ctx, hipMalloc((void**)&h.mat, std::max<size_t>((size_t)k * k, 1) * 8));
+  h.t = k;
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), 0, s, d_edges, m, n, k, (const TrialScratch*)d_scr,
-                     (const uint32_t*)d_slot, d_misc, d_misc + 1);
+  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), 0, s, (const TrialScratch*)d_scr, d_misc,
+                     d_misc + 1);
   CAC_LAUNCH_CHECK(ctx, "densify trial");
   CAC_HIP(ctx, hipStreamSynchronize(s));
   out->lab = h.lab;

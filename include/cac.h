@@ -32,7 +32,7 @@ extern "C" {
 #define CAC_EINVAL (-1)   /* bad argument: null pointer, vertex id >= n, size out of range */
 #define CAC_ENOMEM (-2)   /* device or host allocation failed */
 #define CAC_EDEVICE (-3)  /* HIP runtime error */
-#define CAC_ENOTCONN (-4) /* graph is not connected (min-cut precondition) */
+#define CAC_ENOTCONN (-4) /* reserved (a disconnected graph is reported as cut 0) */
 #define CAC_ERANGE (-5)   /* a weight sum would overflow int64 */
 #define CAC_EIO (-6)      /* edge-list file cannot be opened or parsed */
 
@@ -158,6 +158,31 @@ int cac_mincut(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, doub
 int cac_mincut_dev(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, double p, uint32_t seed,
                    int64_t* cut, uint32_t* trials, void* stream);
 
+/* ---- batched square-root min cut ---------------------------------------------
+ * Many independent graphs in one call.  The reference solves one graph per MPI
+ * job (src/SquareRootCut.cpp:15-25, src/executables/square_root.cpp); there is
+ * no batch interface to replace.  Graph g's edges are
+ * d_edges[edge_offset .. edge_offset + m) with ids local to the graph (< n).
+ * Graph g runs square-root-cut trials 0 .. T_g - 1 exactly as cac_mincut would
+ * on that graph alone (t = cac_intermediate_size(n, m)), so every cut equals the
+ * one-graph result with the same T_g:
+ *   trials > 0:  T_g = trials;   trials == 0:  T_g = cac_sqrt_trials(n, m, p).
+ * cuts[g] = minimum raw cut over the graph's trials; trials_out[g] (optional) =
+ * T_g.  The (graph, trial) jobs of all graphs share launches, in chunks sized to
+ * the free device memory. */
+typedef struct cac_graph_desc {
+  uint64_t edge_offset;
+  uint64_t m;
+  uint32_t n;
+  uint32_t trials;
+} cac_graph_desc;
+int cac_mincut_batch(cac_ctx* ctx, const cac_edge* edges, const cac_graph_desc* descs, uint32_t count, double p,
+                     uint32_t seed, int64_t* cuts, uint32_t* trials_out);
+/* Device variant: d_edges in device memory (ids not validated); descs, cuts and
+ * trials_out on the host; returns when the cuts are on the host. */
+int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_desc* descs, uint32_t count, double p,
+                         uint32_t seed, int64_t* cuts, uint32_t* trials_out, void* stream);
+
 /* ---- approximate minimum cut ----------------------------------------------
  * Replaces ApproximateCut::run (src/ApproximateCut.cpp:10-79) and numberOfTrials
  * (:82-87).  For i = 1 .. ceil(ln W) (W = total weight): T = cac_approx_trials(p)
@@ -193,6 +218,25 @@ int cac_gen_complete_dev(cac_ctx* ctx, uint32_t n, uint32_t seed, uint32_t wmax,
  * sorted by (u, v); *m_out = edges generated (CAC_ERANGE if more than cap). */
 int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_t wmax, cac_edge* out, uint64_t cap,
                uint64_t* m_out);
+
+/* Planted-cut graphs for the batched min cut (no counterpart in the reference).
+ * Graph g (g = first .. first + count - 1) has n_g = 2 floor(e^x / 2) vertices,
+ * x uniform in [ln nmin, ln nmax] (word 0 of Philox(ctr = {g, ~0, 0, 0},
+ * key = {seed, 'PLNT'})), clamped to even values in [nmin, nmax].  Its halves
+ * [0, n/2) and [n/2, n) are G(n/2, avg_degree / (n/2 - 1)) drawn by geometric
+ * skipping, with weights uniform in [1, wmax].  c uniform in [1, cmax] distinct
+ * unit-weight edges (x < n/2 <= y) join them.  A graph is redrawn until both
+ * halves are connected and every weighted degree exceeds c, so c is the
+ * weight of the cut between the halves and below every single-vertex cut.
+ * cac_gen_planted_sizes gives n_g and an upper bound on the total edge count.
+ * cac_gen_planted writes descs[i] (edge_offset, m, n; trials = 0),
+ * planted[i] = c_i and the edges of all graphs back to back (*m_total edges;
+ * CAC_ERANGE if more than edge_cap).  Host-side, multithreaded. */
+int cac_gen_planted_sizes(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
+                          double avg_degree, uint32_t cmax, uint32_t* n_out, uint64_t* edge_bound);
+int cac_gen_planted(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax, double avg_degree,
+                    uint32_t wmax, uint32_t cmax, cac_graph_desc* descs, cac_edge* edges, uint64_t edge_cap,
+                    uint64_t* m_total, uint32_t* planted);
 
 #ifdef __cplusplus
 }

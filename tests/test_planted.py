@@ -1,0 +1,75 @@
+"""The planted-cut batch generator (cac_gen_planted, host-side): sizes, the
+certificate it promises (connected halves, every weighted degree above c, c
+unit cross edges), determinism, and the planted cut against exact
+Stoer-Wagner (the oracle) on small graphs."""
+import numpy as np
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+
+import cac
+
+
+def _graph(descs, edges, i):
+    d = descs[i]
+    return int(d["n"]), edges[int(d["edge_offset"]): int(d["edge_offset"] + d["m"])]
+
+
+def test_sizes_and_certificate():
+    descs, edges, planted = cac.gen_planted(0, 48, 3, 256, 4096)
+    n = descs["n"].astype(np.int64)
+    assert np.all(n % 2 == 0) and n.min() >= 256 and n.max() <= 4096
+    assert np.all(descs["trials"] == 0)
+    assert np.all((planted >= 1) & (planted <= 8))
+    assert descs["edge_offset"][0] == 0
+    assert np.all(descs["edge_offset"][1:] == descs["edge_offset"][:-1] + descs["m"][:-1])
+    avg = (2 * descs["m"].sum()) / n.sum()
+    assert 15.0 < avg < 17.0
+    for i in range(len(descs)):
+        nv, e = _graph(descs, edges, i)
+        h = nv // 2
+        u, v, w = e["u"].astype(np.int64), e["v"].astype(np.int64), e["w"].astype(np.int64)
+        assert np.all(u < v) and np.all(v < nv)
+        cross = (u < h) & (v >= h)
+        assert cross.sum() == planted[i] and np.all(w[cross] == 1)
+        assert np.all((w[~cross] >= 1) & (w[~cross] <= 100))
+        assert len(np.unique(u * nv + v)) == len(u)  # simple graph
+        deg = np.bincount(u, w, nv) + np.bincount(v, w, nv)
+        assert deg.min() > planted[i]
+        inner = ~cross
+        g = coo_matrix((np.ones(inner.sum()), (u[inner], v[inner])), shape=(nv, nv))
+        nc, lab = connected_components(g, directed=False)
+        assert nc == 2 and len(set(lab[:h])) == 1 and len(set(lab[h:])) == 1
+
+
+def test_deterministic_and_addressable():
+    a = cac.gen_planted(0, 10, 7, 256, 2048)
+    b = cac.gen_planted(0, 10, 7, 256, 2048)
+    c = cac.gen_planted(6, 3, 7, 256, 2048)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    for k in range(3):
+        na, ea = _graph(a[0], a[1], 6 + k)
+        nc_, ec = _graph(c[0], c[1], k)
+        assert na == nc_ and np.array_equal(ea, ec) and a[2][6 + k] == c[2][k]
+
+
+def test_planted_cut_is_the_min_cut(orc):
+    descs, edges, planted = cac.gen_planted(0, 12, 11, 256, 600)
+    for i in range(len(descs)):
+        nv, e = _graph(descs, edges, i)
+        adj = np.zeros((nv, nv), dtype=np.int64)
+        np.add.at(adj, (e["u"], e["v"]), e["w"].astype(np.int64))
+        adj += adj.T
+        assert orc.stoer_wagner(adj) == planted[i]
+
+
+def test_size_distribution_is_log_uniform():
+    ns = np.empty(4096, dtype=np.uint32)
+    import ctypes
+
+    b = ctypes.c_uint64()
+    assert cac.load_library().cac_gen_planted_sizes(0, 4096, 0, 256, 65536, 16.0, 8,
+                                                    ns.ctypes.data_as(cac._pu32), ctypes.byref(b)) == 0
+    x = (np.log(ns.astype(np.float64)) - np.log(256)) / (np.log(65536) - np.log(256))
+    hist = np.histogram(x, bins=8, range=(0, 1))[0]
+    assert hist.min() > 4096 / 8 * 0.8 and hist.max() < 4096 / 8 * 1.2
+    assert b.value >= 8 * ns.astype(np.int64).sum()
