@@ -30,6 +30,14 @@ UEDGE = np.dtype([("u", "<u4"), ("v", "<u4")])
 EDGE = np.dtype([("u", "<u4"), ("v", "<u4"), ("w", "<u8")])
 # cac_graph_desc (24 bytes): one graph of a batch.
 GRAPH_DESC = np.dtype([("edge_offset", "<u8"), ("m", "<u8"), ("n", "<u4"), ("trials", "<u4")])
+# cac_query (48 bytes) and cac_query_result (64 bytes) of the serving loop.
+QUERY_CC = 0
+QUERY_MINCUT = 1
+QUERY = np.dtype([("kind", "<i4"), ("n", "<u4"), ("m", "<u8"), ("edges", "<u8"), ("p", "<f8"), ("seed", "<u4"),
+                  ("reserved", "<u4"), ("labels_out", "<u8")])
+QUERY_RESULT = np.dtype([("id", "<u8"), ("status", "<i4"), ("ncomp", "<u4"), ("cut", "<i8"), ("t", "<u4"),
+                         ("trials", "<u4"), ("submit_s", "<f8"), ("admit_s", "<f8"), ("done_s", "<f8"),
+                         ("batch_size", "<u4"), ("reserved", "<u4")])
 
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
@@ -74,6 +82,14 @@ SIGNATURES = {
     "cac_gen_er": (_i32, [_u32, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_mincut_batch": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
     "cac_mincut_batch_dev": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32, _vp]),
+    "cac_serve_mincut_plan": (_i32, [_u32, _u64, ctypes.c_double, _pu32, _pu32]),
+    "cac_server_create": (_i32, [_i32, _u32, _u64, _u32, ctypes.POINTER(_vp)]),
+    "cac_server_submit": (_i32, [_vp, _vp, _pu64]),
+    "cac_server_poll": (_i32, [_vp, _vp, _u32, _pu32, _i32]),
+    "cac_server_clock": (ctypes.c_double, [_vp]),
+    "cac_server_replay": (_i32, [_vp, _vp, _vp, _u32, _vp]),
+    "cac_server_destroy": (_i32, [_vp]),
+    "cac_gen_gnp": (_i32, [_u32, ctypes.c_double, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_gen_planted_sizes": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _pu32, _pu64]),
     "cac_gen_planted": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _u32, _vp, _vp, _u64, _pu64,
                                _pu32]),
@@ -299,6 +315,78 @@ class Context:
         return int(used.value)
 
 
+class Server:
+    """cac_server: worker threads with their own contexts serving CC and min-cut
+    queries with footprint-based admission and per-kind launch groups."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 64 << 30, max_batch: int = 64):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.cac_server_create(device, workers, hbm_budget, max_batch, ctypes.byref(h))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_create")
+        self.handle = h
+        self._keep = []
+
+    @staticmethod
+    def query(kind: int, edges, n: int, p: float = 0.95, seed: int = 0, labels: np.ndarray | None = None):
+        """One QUERY record; returns (record, arrays to keep alive)."""
+        e = to_edges(edges)
+        q = np.zeros(1, dtype=QUERY)
+        q["kind"], q["n"], q["m"] = kind, n, len(e)
+        q["edges"] = e.ctypes.data if len(e) else 0
+        q["p"], q["seed"] = p, seed
+        q["labels_out"] = labels.ctypes.data if labels is not None else 0
+        return q[0], (e, labels)
+
+    def replay(self, queries, arrivals) -> np.ndarray:
+        """queries: list of (record, keep) from query(); arrivals: seconds."""
+        recs = np.array([q for q, _ in queries], dtype=QUERY)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        out = np.zeros(len(recs), dtype=QUERY_RESULT)
+        rc = self.lib.cac_server_replay(self.handle, _ptr(recs), _ptr(arr), len(recs), _ptr(out))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_replay")
+        return out
+
+    def submit(self, record, keep) -> int:
+        rec = np.array([record], dtype=QUERY)
+        i = ctypes.c_uint64()
+        rc = self.lib.cac_server_submit(self.handle, _ptr(rec), ctypes.byref(i))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_submit")
+        self._keep.append(keep)
+        return int(i.value)
+
+    def poll(self, cap: int = 256, wait: bool = True) -> np.ndarray:
+        out = np.zeros(cap, dtype=QUERY_RESULT)
+        k = ctypes.c_uint32()
+        rc = self.lib.cac_server_poll(self.handle, _ptr(out), cap, ctypes.byref(k), 1 if wait else 0)
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_poll")
+        return out[: k.value]
+
+    def close(self):
+        if self.handle:
+            self.lib.cac_server_destroy(self.handle)
+            self.handle = None
+            self._keep = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def serve_mincut_plan(n: int, m: int, p: float):
+    t, T = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = load_library().cac_serve_mincut_plan(n, m, p, ctypes.byref(t), ctypes.byref(T))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_serve_mincut_plan")
+    return int(t.value), int(T.value)
+
+
 def to_edges(edges) -> np.ndarray:
     """(m, 3) [u, v, w] integers (or EDGE records) -> contiguous EDGE records."""
     a = np.asarray(edges)
@@ -364,6 +452,22 @@ def gen_planted(first: int, count: int, seed: int, nmin: int = 256, nmax: int = 
     if rc != CAC_OK:
         raise CacError(rc, "cac_gen_planted")
     return descs[:count], edges[: m.value], planted[:count]
+
+
+def gen_gnp(n: int, avg_degree: float, seed: int, stream: int, wmax: int = 100) -> np.ndarray:
+    """G(n, avg_degree/(n-1)) as EDGE records (see cac_gen_gnp); may be disconnected."""
+    lib = load_library()
+    mean = n * avg_degree / 2.0
+    cap = int(mean + 8.0 * mean ** 0.5 + 64)
+    out = np.empty(max(cap, 1), dtype=EDGE)
+    m = ctypes.c_uint64()
+    rc = lib.cac_gen_gnp(n, avg_degree, seed, stream, wmax, _ptr(out), cap, ctypes.byref(m))
+    if rc == CAC_ERANGE:
+        out = np.empty(m.value, dtype=EDGE)
+        rc = lib.cac_gen_gnp(n, avg_degree, seed, stream, wmax, _ptr(out), m.value, ctypes.byref(m))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_gnp")
+    return out[: m.value]
 
 
 def _connected(n: int, e: np.ndarray) -> bool:

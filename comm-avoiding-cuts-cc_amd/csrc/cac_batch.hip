@@ -19,6 +19,7 @@ namespace cac {
 namespace {
 
 constexpr uint32_t kDomPlanted = 0x504C4E54u;  // 'PLNT'
+constexpr uint32_t kDomGnp = 0x474E5047u;      // 'GNPG'
 constexpr uint32_t kMaxPlantedAttempts = 64;
 
 uint32_t planted_n(uint32_t g, uint32_t seed, uint32_t nmin, uint32_t nmax) {
@@ -158,6 +159,32 @@ int cac_gen_planted(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin
   if (off > edge_cap) return CAC_ERANGE;
   for (uint32_t i = 0; i < count; ++i) std::copy(per[i].begin(), per[i].end(), edges + descs[i].edge_offset);
   return CAC_OK;
+}
+
+int cac_gen_gnp(uint32_t n, double avg_degree, uint32_t seed, uint32_t stream, uint32_t wmax, cac_edge* out,
+                uint64_t cap, uint64_t* m_out) {
+  if (!m_out || n < 2 || !(avg_degree > 0) || wmax == 0) return CAC_EINVAL;
+  const double p = std::min(1.0, avg_degree / (double)(n - 1));
+  const double l1mp = std::log1p(-p);
+  uint64_t m = 0;
+  for (uint32_t u = 0; u + 1 < n; ++u) {
+    uint64_t v = u;
+    for (uint32_t j = 0;; ++j) {
+      const cac::u32x4 r = cac::philox4x32(stream, u, j, 0u, seed, cac::kDomGnp);
+      double skip = 0.0;
+      if (p < 1.0) {
+        const double U = ((double)r.x + 0.5) / 4294967296.0;
+        skip = std::floor(std::log(U) / l1mp);
+        if (skip >= (double)n) break;
+      }
+      v += 1 + (uint64_t)skip;
+      if (v >= n) break;
+      if (out && m < cap) out[m] = cac_edge{u, (uint32_t)v, 1 + (((uint64_t)wmax * r.y) >> 32)};
+      ++m;
+    }
+  }
+  *m_out = m;
+  return (out && m > cap) ? CAC_ERANGE : CAC_OK;
 }
 
 int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_desc* descs, uint32_t count, double p,

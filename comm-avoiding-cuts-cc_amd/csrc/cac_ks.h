@@ -36,8 +36,13 @@ struct SqrtJob {
   uint32_t n, t, trial, slot;
 };
 constexpr size_t kMaxJobsPerChunk = 8192;
-// Jobs of any mix of graphs, in memory-sized chunks of shared launches.
-int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s);
+// Jobs of any mix of graphs, in chunks of shared launches whose estimated
+// device bytes stay within budget (0: 40% of the free device memory).
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s,
+                   size_t budget = 0);
+// Estimated device bytes of one job: sampling scratch, Karger-Stein levels.
+size_t job_sample_bytes(const SqrtJob& j);
+size_t job_ks_bytes(const SqrtJob& j);
 // Sqrt-cut trials on a device edge list: raw cut of trial_ids[i] to d_results[i].
 int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
                  const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s);

@@ -12,6 +12,7 @@
 // results to the depth-first CPU restatement.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "cac_internal.h"
@@ -536,12 +537,13 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
       }
     }
     CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
-    static bool lds_attr = false;
-    if (!lds_attr) {
-      CAC_HIP(ctx, hipFuncSetAttribute((const void*)k_ks_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)contract_lds_bytes(kKsMaxLdsS)));
-      lds_attr = true;
-    }
+    static std::once_flag lds_attr;
+    hipError_t attr_err = hipSuccess;
+    std::call_once(lds_attr, [&] {
+      attr_err = hipFuncSetAttribute((const void*)k_ks_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)contract_lds_bytes(kKsMaxLdsS));
+    });
+    CAC_HIP(ctx, attr_err);
     prof_begin(ctx, "ks_contract", s);
     hipLaunchKernelGGL(k_ks_contract, dim3((unsigned)nchild), dim3(kKsBlock), contract_lds_bytes(smax), s, d_children,
                        seed, d_kout, d_results);

@@ -294,24 +294,27 @@ uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p) {
 }
 
 // Device bytes one sampling job needs beside the Karger-Stein levels.
-static size_t job_sample_bytes(const SqrtJob& j) {
+size_t job_sample_bytes(const SqrtJob& j) {
   return carve_size((size_t)j.n * 4) * 2 + carve_size(std::max<uint64_t>(j.m, 1) * 8) + carve_size((size_t)j.n * 8) +
          carve_size((size_t)j.t * j.t * 8) + 256;
 }
 // Upper estimate of the Karger-Stein level arenas of one root of size t: each
 // level holds about t^2 entries in copy-on-write work rows plus the contracted
 // children, and two levels are alive at once.
-static size_t job_ks_bytes(const SqrtJob& j) { return (size_t)j.t * j.t * 8 * 8 + 4096; }
+size_t job_ks_bytes(const SqrtJob& j) { return (size_t)j.t * j.t * 8 * 8 + 4096; }
 
 // Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
 // one sampling launch, one densify launch and one breadth-first Karger-Stein
 // pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
 // i goes to d_results[jobs[i].slot].
-int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s) {
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s,
+                   size_t budget) {
   if (jobs.empty()) return CAC_OK;
-  size_t free_b = 0, total_b = 0;
-  CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-  const size_t budget = std::max<size_t>((size_t)(0.4 * (double)free_b), (size_t)1 << 28);
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+    budget = std::max<size_t>((size_t)(0.4 * (double)free_b), (size_t)1 << 28);
+  }
   int rc = CAC_OK;
   size_t b0 = 0;
   while (b0 < jobs.size() && rc == CAC_OK) {

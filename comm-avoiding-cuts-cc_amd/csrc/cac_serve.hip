@@ -1,0 +1,476 @@
+// cac_serve.hip -- a serving loop for connected-components and min-cut queries
+// on one GPU (no counterpart in the reference, whose executables solve one
+// graph per MPI job).
+//
+// Design: W worker threads, each with its own context (HIP stream + scratch).
+// A free worker takes the oldest pending query, then every other pending query
+// of the same kind (min cut: same seed) that still fits the HBM budget, up to
+// max_batch, and runs them as one launch group:
+//   - CC: the graphs are laid side by side (vertex ids offset) and one CC
+//     pipeline labels all of them; labels are canonical minimum ids, so each
+//     query's labels and component count are exactly its alone result;
+//   - min cut: every (query, trial) job of the group shares the sampling,
+//     densify and Karger-Stein launches (sqrt_cut_batch).
+// Groups on different workers overlap on the device.  A query is admitted only
+// when its footprint estimate fits beside the groups in flight, so requests
+// that arrive while others run are admitted as memory and workers free up.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "cac_common.h"
+#include "cac_internal.h"
+#include "cac_ks.h"
+
+namespace cac {
+
+size_t cc_tmp_words(uint64_t m, uint32_t n, int label_mode);
+int cc_launch(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode, uint32_t* d_lab,
+              uint32_t* d_tmp, hipStream_t s);
+
+namespace {
+
+constexpr uint32_t kServeMaxT = 4096;              // dense base size cap for served min cuts
+constexpr uint64_t kServeMaxGroupEdges = 1ull << 27;  // edges per CC launch group
+
+struct Pending {
+  uint64_t id;
+  cac_query q;
+  double submit_s;
+  size_t bytes;
+};
+
+void mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* T) {
+  *t = std::min<uint32_t>(sqrt_intermediate_size(n, m), kServeMaxT);
+  *T = std::max<uint32_t>(2, ks_trials_host(*t, p));
+}
+
+size_t footprint(const cac_query& q) {
+  if (q.kind == CAC_QUERY_CC)
+    return carve_size(q.m * sizeof(cac_edge)) + carve_size(q.m * sizeof(cac_uedge)) + carve_size((size_t)q.n * 4) +
+           cc_tmp_words(q.m, q.n, 0) * 4 + 4096;
+  uint32_t t, T;
+  mincut_plan(q.n, q.m, q.p, &t, &T);
+  SqrtJob j{nullptr, q.m, q.n, t, 0, 0};
+  return carve_size(q.m * sizeof(cac_edge)) + (size_t)T * (job_sample_bytes(j) + job_ks_bytes(j)) + 4096;
+}
+
+}  // namespace
+}  // namespace cac
+
+struct cac_server {
+  int device = 0;
+  uint64_t budget = 0;
+  uint32_t max_batch = 1;
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<cac::Pending> pending;
+  std::vector<cac_query_result> done;
+  uint64_t inflight = 0, next_id = 0;
+  bool stop = false;
+  std::vector<std::thread> workers;
+  std::vector<cac_ctx*> ctxs;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace cac {
+namespace {
+
+cac_query_result blank(const Pending& p) {
+  cac_query_result r{};
+  r.id = p.id;
+  r.status = CAC_OK;
+  r.submit_s = p.submit_s;
+  return r;
+}
+
+// Cheap checks at submit time; edge ids and weights are checked on the device.
+int validate(const cac_query& q) {
+  if (q.n < 1 || (q.m && !q.edges)) return CAC_EINVAL;
+  if (q.kind == CAC_QUERY_MINCUT && (q.n < 2 || q.m == 0 || !(q.p >= 0.0 && q.p < 1.0))) return CAC_EINVAL;
+  return CAC_OK;
+}
+
+// CC group staging: query edges (cac_edge) -> side-by-side cac_uedge with the
+// query's vertex offset.  An edge with an id >= n becomes the loop (off, off)
+// and flags the query, so no kernel of the group can index outside it.
+__global__ void __launch_bounds__(256) k_serve_prep_cc(const cac_edge* __restrict__ in, uint64_t m, uint32_t n,
+                                                       uint32_t off, cac_uedge* __restrict__ out, uint32_t* bad) {
+  uint32_t b = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < m; e += (uint64_t)gridDim.x * 256) {
+    const cac_edge x = in[e];
+    const bool ok = x.u < n && x.v < n;
+    out[e] = ok ? cac_uedge{x.u + off, x.v + off} : cac_uedge{off, off};
+    b |= ok ? 0u : 1u;
+  }
+  if (b) atomicOr(bad, 1u);
+}
+
+// Min-cut group staging: flags ids >= n (chk[0] bit 0) and weights >= 2^40
+// (bit 1); such an edge becomes a weight-0 loop.  Sums the weights, plus the
+// weights' high words so a wrapped 64-bit sum is still caught (chk[1], chk[2]).
+__global__ void __launch_bounds__(256) k_serve_check_w(cac_edge* __restrict__ e, uint64_t m, uint32_t n,
+                                                       unsigned long long* chk) {
+  __shared__ unsigned long long red[2][4];
+  unsigned long long s = 0, hs = 0;
+  uint32_t b = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256) {
+    const cac_edge x = e[i];
+    const uint32_t f = (x.u >= n || x.v >= n ? 1u : 0u) | (x.w >= (1ull << 40) ? 2u : 0u);
+    if (f) {
+      e[i] = cac_edge{0, 0, 0};
+      b |= f;
+    } else {
+      s += x.w;
+      hs += x.w >> 32;
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    s += __shfl_down(s, d, 64);
+    hs += __shfl_down(hs, d, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = s;
+    red[1][threadIdx.x >> 6] = hs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      s += red[0][w];
+      hs += red[1][w];
+    }
+    if (s) atomicAdd(&chk[1], s);
+    if (hs) atomicAdd(&chk[2], hs);
+  }
+  if (b) atomicOr(&chk[0], (unsigned long long)b);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t count) {
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+__global__ void k_zero_u32s(uint32_t* p, uint32_t count) {
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0;
+}
+
+void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query_result>& res) {
+  uint64_t M = 0, N = 0;
+  std::vector<uint64_t> voff(grp.size()), eoff(grp.size());
+  for (size_t i = 0; i < grp.size(); ++i) {
+    voff[i] = N;
+    eoff[i] = M;
+    N += grp[i].q.n;
+    M += grp[i].q.m;
+  }
+  const size_t tw = cc_tmp_words(M, (uint32_t)N, CAC_LABEL_MIN_ID);
+  void* base = nullptr;
+  int rc = CAC_OK;
+  hipError_t e = hipMalloc(&base, carve_size(M * sizeof(cac_edge)) + carve_size(M * sizeof(cac_uedge)) +
+                                      carve_size(N * 4) + carve_size(tw * 4) + carve_size(grp.size() * 4) + 2048);
+  if (e != hipSuccess) rc = hip_fail(ctx, e, "serve cc buffers");
+  std::vector<uint32_t> lab(N), bad(grp.size(), 0);
+  hipStream_t s = ctx->stream;
+  if (rc == CAC_OK) {
+    Carver cv(base);
+    cac_edge* d_in = cv.take<cac_edge>(M);
+    cac_uedge* d_e = cv.take<cac_uedge>(M);
+    uint32_t* d_l = cv.take<uint32_t>(N);
+    uint32_t* d_t = cv.take<uint32_t>(tw);
+    uint32_t* d_bad = cv.take<uint32_t>(grp.size());
+    hipLaunchKernelGGL(k_zero_u32s, dim3(1), dim3(256), 0, s, d_bad, (uint32_t)grp.size());
+    for (size_t i = 0; i < grp.size() && e == hipSuccess; ++i) {
+      const cac_query& q = grp[i].q;
+      if (!q.m) continue;
+      e = hipMemcpyAsync(d_in + eoff[i], q.edges, q.m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
+      hipLaunchKernelGGL(k_serve_prep_cc, dim3(grid_for(q.m, 256, 2048)), dim3(256), 0, s,
+                         (const cac_edge*)(d_in + eoff[i]), q.m, q.n, (uint32_t)voff[i], d_e + eoff[i], d_bad + i);
+    }
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "serve cc upload");
+    if (rc == CAC_OK) rc = cc_launch(ctx, d_e, M, (uint32_t)N, CAC_LABEL_MIN_ID, d_l, d_t, s);
+    if (rc == CAC_OK) {
+      e = hipMemcpyAsync(lab.data(), d_l, N * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(bad.data(), d_bad, grp.size() * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) rc = hip_fail(ctx, e, "serve cc results");
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (base) (void)hipFree(base);
+  for (size_t i = 0; i < grp.size(); ++i) {
+    cac_query_result& r = res[i];
+    r.status = rc != CAC_OK ? rc : (bad[i] ? CAC_EINVAL : CAC_OK);
+    if (r.status != CAC_OK) continue;
+    uint32_t nc = 0;
+    const uint32_t off = (uint32_t)voff[i];
+    for (uint32_t v = 0; v < grp[i].q.n; ++v) {
+      const uint32_t l = lab[off + v] - off;  // canonical ids stay inside the query's range
+      nc += (l == v);
+      if (grp[i].q.labels_out) grp[i].q.labels_out[v] = l;
+    }
+    r.ncomp = nc;
+  }
+}
+
+void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query_result>& res, size_t budget) {
+  uint64_t M = 0;
+  for (auto& p : grp) M += p.q.m;
+  int rc = CAC_OK;
+  hipStream_t s = ctx->stream;
+  cac_edge* d_e = nullptr;
+  int64_t* d_res = nullptr;
+  unsigned long long* d_chk = nullptr;
+  hipError_t e = hipMalloc((void**)&d_e, std::max<uint64_t>(M, 1) * sizeof(cac_edge));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_chk, 24 * grp.size());
+  if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut buffers");
+  std::vector<uint64_t> eoff(grp.size());
+  std::vector<unsigned long long> chk(3 * grp.size());
+  if (rc == CAC_OK) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, s, d_chk, (uint32_t)(3 * grp.size()));
+    uint64_t off = 0;
+    for (size_t i = 0; i < grp.size() && e == hipSuccess; ++i) {
+      const cac_query& q = grp[i].q;
+      eoff[i] = off;
+      e = hipMemcpyAsync(d_e + off, q.edges, q.m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
+      hipLaunchKernelGGL(k_serve_check_w, dim3(grid_for(q.m, 256, 2048)), dim3(256), 0, s, d_e + off, q.m, q.n,
+                         d_chk + 3 * i);
+      off += q.m;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(chk.data(), d_chk, 24 * grp.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // bad queries leave the group before any sampling
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut upload");
+  }
+  std::vector<SqrtJob> jobs;
+  std::vector<size_t> first(grp.size() + 1, 0);
+  for (size_t i = 0; i < grp.size() && rc == CAC_OK; ++i) {
+    const cac_query& q = grp[i].q;
+    first[i] = jobs.size();
+    if (chk[3 * i] & 1) {
+      res[i].status = CAC_EINVAL;
+      continue;
+    }
+    if ((chk[3 * i] & 2) || chk[3 * i + 1] >= (1ull << 61) || chk[3 * i + 2] >= (1ull << 29)) {
+      res[i].status = CAC_ERANGE;
+      continue;
+    }
+    uint32_t t, T;
+    mincut_plan(q.n, q.m, q.p, &t, &T);
+    res[i].t = t;
+    res[i].trials = T;
+    for (uint32_t k = 0; k < T; ++k) jobs.push_back({d_e + eoff[i], q.m, q.n, t, k, (uint32_t)jobs.size()});
+  }
+  first[grp.size()] = jobs.size();
+  if (rc == CAC_OK && !jobs.empty()) {
+    e = hipMalloc((void**)&d_res, 8 * jobs.size());
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
+    if (rc == CAC_OK) rc = sqrt_cut_batch(ctx, jobs, grp[0].q.seed, d_res, s, budget);
+  }
+  std::vector<int64_t> raw(jobs.size());
+  if (rc == CAC_OK && !jobs.empty()) {
+    e = hipMemcpyAsync(raw.data(), d_res, 8 * raw.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
+  }
+  (void)hipStreamSynchronize(s);
+  if (d_res) (void)hipFree(d_res);
+  if (d_chk) (void)hipFree(d_chk);
+  if (d_e) (void)hipFree(d_e);
+  for (size_t i = 0; i < grp.size(); ++i) {
+    if (rc != CAC_OK) res[i].status = rc;
+    if (res[i].status == CAC_OK)
+      res[i].cut = *std::min_element(raw.begin() + first[i], raw.begin() + first[i + 1]);
+  }
+}
+
+void worker_loop(cac_server* S, cac_ctx* ctx) {
+  for (;;) {
+    std::vector<Pending> grp;
+    size_t bytes = 0;
+    double admit = 0;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (;;) {
+        if (S->stop) return;
+        if (!S->pending.empty() && S->inflight + S->pending.front().bytes <= S->budget) break;
+        S->cv.wait(lk);
+      }
+      Pending head = S->pending.front();
+      S->pending.pop_front();
+      bytes = head.bytes;
+      uint64_t edges = head.q.m;
+      grp.push_back(head);
+      for (auto it = S->pending.begin(); it != S->pending.end() && grp.size() < S->max_batch;) {
+        const bool same = it->q.kind == head.q.kind && (head.q.kind == CAC_QUERY_CC || it->q.seed == head.q.seed);
+        const bool fits = S->inflight + bytes + it->bytes <= S->budget &&
+                          (head.q.kind != CAC_QUERY_CC || edges + it->q.m <= kServeMaxGroupEdges);
+        const bool ids_fit = head.q.kind != CAC_QUERY_CC || [&] {
+          uint64_t nsum = 0;
+          for (auto& g : grp) nsum += g.q.n;
+          return nsum + it->q.n < (1ull << 32);
+        }();
+        if (same && fits && ids_fit) {
+          bytes += it->bytes;
+          edges += it->q.m;
+          grp.push_back(*it);
+          it = S->pending.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      S->inflight += bytes;
+      admit = S->now();
+    }
+    std::vector<cac_query_result> res(grp.size());
+    for (size_t i = 0; i < grp.size(); ++i) {
+      res[i] = blank(grp[i]);
+      res[i].admit_s = admit;
+      res[i].batch_size = (uint32_t)grp.size();
+    }
+    (void)bind(ctx);
+    if (grp[0].q.kind == CAC_QUERY_CC)
+      run_cc_group(ctx, grp, res);
+    else
+      run_mincut_group(ctx, grp, res, bytes);
+    const double fin = S->now();
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->inflight -= bytes;
+      for (auto& r : res) {
+        r.done_s = fin;
+        S->done.push_back(r);
+      }
+    }
+    S->cv.notify_all();
+  }
+}
+
+}  // namespace
+}  // namespace cac
+
+extern "C" {
+
+int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* trials) {
+  if (!t || !trials || n < 2 || m == 0) return CAC_EINVAL;
+  cac::mincut_plan(n, m, p, t, trials);
+  return CAC_OK;
+}
+
+int cac_server_create(int device, uint32_t workers, uint64_t hbm_budget, uint32_t max_batch, cac_server** out) {
+  if (!out || workers == 0 || workers > 16 || hbm_budget == 0 || max_batch == 0) return CAC_EINVAL;
+  *out = nullptr;
+  cac_server* S = new (std::nothrow) cac_server();
+  if (!S) return CAC_ENOMEM;
+  S->device = device;
+  S->budget = hbm_budget;
+  S->max_batch = max_batch;
+  S->t0 = std::chrono::steady_clock::now();
+  for (uint32_t w = 0; w < workers; ++w) {
+    cac_ctx* c = nullptr;
+    int rc = cac_ctx_create(device, &c);
+    if (rc != CAC_OK) {
+      for (auto x : S->ctxs) cac_ctx_destroy(x);
+      delete S;
+      return rc;
+    }
+    S->ctxs.push_back(c);
+  }
+  for (uint32_t w = 0; w < workers; ++w) S->workers.emplace_back(cac::worker_loop, S, S->ctxs[w]);
+  *out = S;
+  return CAC_OK;
+}
+
+double cac_server_clock(cac_server* S) { return S ? S->now() : 0.0; }
+
+int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
+  if (!S || !q) return CAC_EINVAL;
+  if (q->kind != CAC_QUERY_CC && q->kind != CAC_QUERY_MINCUT) return CAC_EINVAL;
+  cac::Pending p;
+  p.q = *q;
+  p.bytes = cac::footprint(*q);
+  const int v = cac::validate(*q);
+  std::lock_guard<std::mutex> lk(S->mu);
+  p.id = S->next_id++;
+  p.submit_s = S->now();
+  if (id_out) *id_out = p.id;
+  if (v != CAC_OK || p.bytes > S->budget) {  // rejected now: reported through poll
+    cac_query_result r = cac::blank(p);
+    r.status = v != CAC_OK ? v : CAC_ENOMEM;
+    r.admit_s = r.done_s = p.submit_s;
+    S->done.push_back(r);
+    S->cv.notify_all();
+    return CAC_OK;
+  }
+  S->pending.push_back(p);
+  S->cv.notify_all();
+  return CAC_OK;
+}
+
+int cac_server_poll(cac_server* S, cac_query_result* out, uint32_t cap, uint32_t* count_out, int wait) {
+  if (!S || !count_out || (cap && !out)) return CAC_EINVAL;
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (wait)
+    S->cv.wait(lk, [&] { return !S->done.empty() || (S->pending.empty() && S->inflight == 0); });
+  const uint32_t k = (uint32_t)std::min<size_t>(cap, S->done.size());
+  for (uint32_t i = 0; i < k; ++i) out[i] = S->done[i];
+  S->done.erase(S->done.begin(), S->done.begin() + k);
+  *count_out = k;
+  return CAC_OK;
+}
+
+int cac_server_destroy(cac_server* S) {
+  if (!S) return CAC_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  for (auto& t : S->workers) t.join();
+  for (auto c : S->ctxs) cac_ctx_destroy(c);
+  delete S;
+  return CAC_OK;
+}
+
+int cac_server_replay(cac_server* S, const cac_query* q, const double* arrival_s, uint32_t count,
+                      cac_query_result* out) {
+  if (!S || (count && (!q || !arrival_s || !out))) return CAC_EINVAL;
+  const double start = S->now();
+  std::vector<uint64_t> ids(count);
+  std::vector<char> got(count, 0);
+  uint32_t received = 0;
+  uint64_t id0 = 0;
+  std::vector<cac_query_result> buf(256);
+  auto drain = [&](int wait) {
+    uint32_t k = 0;
+    cac_server_poll(S, buf.data(), (uint32_t)buf.size(), &k, wait);
+    for (uint32_t i = 0; i < k; ++i) {
+      const uint64_t idx = buf[i].id - id0;
+      if (idx < count && !got[idx]) {
+        out[idx] = buf[i];
+        got[idx] = 1;
+        ++received;
+      }
+    }
+  };
+  for (uint32_t i = 0; i < count; ++i) {
+    for (;;) {  // wait for the arrival time, collecting finished results meanwhile
+      const double dt = start + arrival_s[i] - S->now();
+      if (dt <= 0) break;
+      std::this_thread::sleep_for(std::chrono::duration<double>(std::min(dt, 0.0005)));
+      drain(0);
+    }
+    uint64_t id = 0;
+    int rc = cac_server_submit(S, &q[i], &id);
+    if (rc != CAC_OK) return rc;
+    if (i == 0) id0 = id;
+    ids[i] = id;
+    if (id != id0 + i) return CAC_EINVAL;  // another thread submitted meanwhile
+  }
+  while (received < count) drain(1);
+  return CAC_OK;
+}
+
+}  // extern "C"

@@ -183,6 +183,61 @@ int cac_mincut_batch(cac_ctx* ctx, const cac_edge* edges, const cac_graph_desc* 
 int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_desc* descs, uint32_t count, double p,
                          uint32_t seed, int64_t* cuts, uint32_t* trials_out, void* stream);
 
+/* ---- serving loop ------------------------------------------------------------
+ * A query server on one GPU (no counterpart in the reference, whose executables
+ * solve one graph per MPI job).  W worker threads, each with its own context
+ * and stream, admit queries first come first served when their device-memory
+ * footprint fits beside the work in flight (hbm_budget bytes in total), and
+ * run up to max_batch pending queries of one kind (min cut: same seed) as one
+ * launch group: CC graphs side by side in one CC pipeline, min-cut trials of
+ * all queries in shared sampling / Karger-Stein launches.  Groups on different
+ * workers overlap on the device.  Results do not depend on batching:
+ *   CC:      ncomp and labels (canonical minimum ids) equal cac_cc's;
+ *   min cut: cut = min over trials 0 .. T-1 of the square-root cut with
+ *            (t, T) from cac_serve_mincut_plan, i.e. the minimum of
+ *            cac_sqrt_cut_trials(edges, n, t, seed, 0, T).
+ * A query's edges (and labels_out) are host memory that must stay valid until
+ * its result is polled.  Invalid queries (ids >= n, weights >= 2^40, weight sum
+ * >= 2^61, footprint > budget) come back with a non-zero status. */
+#define CAC_QUERY_CC 0
+#define CAC_QUERY_MINCUT 1
+typedef struct cac_query {
+  int32_t kind;            /* CAC_QUERY_CC or CAC_QUERY_MINCUT */
+  uint32_t n;
+  uint64_t m;
+  const cac_edge* edges;   /* m records; CC ignores w */
+  double p;                /* min cut: success probability */
+  uint32_t seed;           /* min cut: Philox seed */
+  uint32_t reserved;
+  uint32_t* labels_out;    /* CC: optional, n labels (minimum id of the component) */
+} cac_query;
+typedef struct cac_query_result {
+  uint64_t id;             /* as returned by cac_server_submit */
+  int32_t status;          /* CAC_OK or a CAC_E* code */
+  uint32_t ncomp;          /* CC: number of components */
+  int64_t cut;             /* min cut */
+  uint32_t t, trials;      /* min cut: intermediate size and trials run */
+  double submit_s, admit_s, done_s;  /* server clock (seconds since create) */
+  uint32_t batch_size;     /* queries in the launch group it ran in */
+  uint32_t reserved;
+} cac_query_result;
+typedef struct cac_server cac_server;
+/* Served min cut: t = min(cac_intermediate_size(n, m), 4096) (the dense base
+ * stays within one GPU's fast path), T = max(2, cac_ks_trials(t, p)). */
+int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* trials);
+int cac_server_create(int device, uint32_t workers, uint64_t hbm_budget, uint32_t max_batch, cac_server** out);
+int cac_server_submit(cac_server* srv, const cac_query* q, uint64_t* id_out);
+/* Moves up to cap finished results to out; with wait != 0 blocks until at least
+ * one is ready or nothing is pending or running. */
+int cac_server_poll(cac_server* srv, cac_query_result* out, uint32_t cap, uint32_t* count_out, int wait);
+double cac_server_clock(cac_server* srv);
+/* Submits q[i] at arrival_s[i] seconds after the call (open-loop replay of an
+ * arrival schedule, non-decreasing times) and returns every result, out[i]
+ * for q[i].  No other thread may submit meanwhile. */
+int cac_server_replay(cac_server* srv, const cac_query* q, const double* arrival_s, uint32_t count,
+                      cac_query_result* out);
+int cac_server_destroy(cac_server* srv);
+
 /* ---- approximate minimum cut ----------------------------------------------
  * Replaces ApproximateCut::run (src/ApproximateCut.cpp:10-79) and numberOfTrials
  * (:82-87).  For i = 1 .. ceil(ln W) (W = total weight): T = cac_approx_trials(p)
@@ -232,6 +287,14 @@ int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_t wmax, cac
  * cac_gen_planted writes descs[i] (edge_offset, m, n; trials = 0),
  * planted[i] = c_i and the edges of all graphs back to back (*m_total edges;
  * CAC_ERANGE if more than edge_cap).  Host-side, multithreaded. */
+/* G(n, avg_degree / (n - 1)) by geometric skipping (no counterpart in the
+ * reference): row u draws skips from word 0 of Philox(ctr = {stream, u, j, 0},
+ * key = {seed, 'GNPG'}) and weights 1 + floor(wmax * word1 / 2^32); edges u < v
+ * sorted by (u, v), loops and duplicates impossible, connectivity not enforced.
+ * Writes up to cap edges (*m_out = edges generated; CAC_ERANGE if more).
+ * Host-side, single-threaded, reentrant. */
+int cac_gen_gnp(uint32_t n, double avg_degree, uint32_t seed, uint32_t stream, uint32_t wmax, cac_edge* out,
+                uint64_t cap, uint64_t* m_out);
 int cac_gen_planted_sizes(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
                           double avg_degree, uint32_t cmax, uint32_t* n_out, uint64_t* edge_bound);
 int cac_gen_planted(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax, double avg_degree,

@@ -73,3 +73,18 @@ def test_size_distribution_is_log_uniform():
     hist = np.histogram(x, bins=8, range=(0, 1))[0]
     assert hist.min() > 4096 / 8 * 0.8 and hist.max() < 4096 / 8 * 1.2
     assert b.value >= 8 * ns.astype(np.int64).sum()
+
+
+def test_gnp_generator():
+    e = cac.gen_gnp(5000, 12.0, 3, 7, wmax=100)
+    u, v = e["u"].astype(np.int64), e["v"].astype(np.int64)
+    assert np.all(u < v) and np.all(v < 5000)
+    assert np.all(np.diff(u * 5000 + v) > 0)  # sorted, no duplicates
+    assert abs(2 * len(e) / 5000 - 12.0) < 0.5
+    assert e["w"].min() >= 1 and e["w"].max() <= 100
+    assert np.array_equal(e, cac.gen_gnp(5000, 12.0, 3, 7, wmax=100))
+    assert not np.array_equal(e[:100], cac.gen_gnp(5000, 12.0, 3, 8, wmax=100)[:100])
+    # pair frequencies: each of the n(n-1)/2 pairs with probability d/(n-1)
+    counts = [len(cac.gen_gnp(400, 8.0, 11, s)) for s in range(40)]
+    mean = 400 * 8.0 / 2
+    assert abs(np.mean(counts) - mean) < 4 * np.sqrt(mean / 40)
