@@ -1,11 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark: connected components (parallel_cc path) on one MI355X.
+"""Benchmark: connected components (parallel_cc path) on one MI355X, plus the
+min-cut and serving figures of the same metric.
 
-Workload: R-MAT graph, 2^24 vertices, 2^28 undirected edges, (a,b,c,d) =
-(0.45, 0.22, 0.22, 0.11), uint32 ids, loops and duplicates removed, generated in
-HBM before timing.  A step = one cac_cc_dev call: HBM-resident edge list in,
+Headline workload: R-MAT graph, 2^24 vertices, 2^28 undirected edges, (a,b,c,d)
+= (0.45, 0.22, 0.22, 0.11), uint32 ids, loops and duplicates removed, generated
+in HBM before timing.  A step = one cac_cc_dev call: HBM-resident edge list in,
 canonical (minimum-id) labels in HBM out, component count on the host.
 value = whole-job GTEPS = 2^28 edges / seconds per step.
+
+The "extras" object carries the min-cut solves/sec and served-latency parts of
+the metric, each on its own workload:
+  c1  square-root min cut, Erdos-Renyi G(1000, 0.05), unit weights, p = 0.95;
+  c3  dense Karger-Stein, complete graph n = 8192, weights U[1, 42], log2(n)^2 trials;
+  c4  4096 planted-cut graphs, n log-uniform in [256, 65536], batched in one call
+      vs the same kernels one graph per call;
+  c5  served Poisson stream, 70% CC / 30% min cut, G(n, d/n), n log-uniform in
+      [10^3, 10^6], d uniform in [8, 64], weights U[1, 100].
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle's single-thread union-find on the same edge list).
@@ -41,6 +51,13 @@ def parse():
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC byte-counter scaling)")
+    ap.add_argument("--extras", default="c1,c3,c4,c5",
+                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5) or 'none'")
+    ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
+    ap.add_argument("--c4-graphs", type=int, default=4096)
+    ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
+    ap.add_argument("--c5-queries", type=int, default=200)
+    ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
     return ap.parse_args()
 
 
@@ -101,6 +118,151 @@ def copy_probe(ctx, mib: int):
     a.free()
     b.free()
     print(f"# copy probe: 3 copies of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
+F32_95 = float(np.float32(0.95))  # the reference parses p with std::stof
+
+
+def bench_c1(ctx, cac, cpu_budget_s: float):
+    """Square-root min cut on ER(1000, 0.05), unit weights, seed 0 (redrawn until connected)."""
+    edges, seed_used = cac.gen_er(1000, 0.05, 0)
+    n = 1000
+    ctx.mincut(edges, n, F32_95, 0)
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cut, T = ctx.mincut(edges, n, F32_95, 0)
+    dt = (time.perf_counter() - t0) / reps
+    deg = np.bincount(edges["u"], edges["w"], n) + np.bincount(edges["v"], edges["w"], n)
+    out = {"workload": f"square_root p=0.95 on G(1000, 0.05), unit weights, seed {seed_used}, m={len(edges)}",
+           "solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3, "cut": cut, "trials": T,
+           "t": cac.intermediate_size(n, len(edges)), "min_degree": int(deg.min())}
+    if cpu_budget_s > 0:
+        import oracle  # the CPU restatement: baseline only
+        e3 = np.stack([edges["u"], edges["v"], edges["w"]], axis=1).astype(np.uint64)
+        t = out["t"]
+        t0 = time.perf_counter()
+        k = 0
+        while k < T and time.perf_counter() - t0 < cpu_budget_s:
+            oracle.sqrt_trial(n, e3, t, 0, k)
+            k += 1
+        per_trial = (time.perf_counter() - t0) / k
+        out["cpu_baseline"] = {"value": 1.0 / (per_trial * T), "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} of the {T} trials timed single-threaded, scaled to one solve"}
+    return out
+
+
+def bench_c3(ctx, cac, trials: int):
+    """Dense Karger-Stein on the complete graph n = 8192, weights U[1, 42]."""
+    n = 8192
+    d_adj = ctx.alloc(n * n * 8)
+    ctx.gen_complete_dev(n, 0, 42, d_adj.ptr)
+    ctx.ks_dense_dev(d_adj.ptr, n, 0, 1000, 2)  # warm-up (other trial ids)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    raw, cut = ctx.ks_dense_dev(d_adj.ptr, n, 0, 0, trials)
+    dt = time.perf_counter() - t0
+    deg_min = None
+    # the bound every trial starts from: minimum weighted degree (row sums on the host, one row at a time)
+    import ctypes
+    for r in range(0, n, 512):
+        blk = np.empty((min(512, n - r), n), dtype=np.int64)
+        ctx._check(ctx.lib.cac_memcpy_d2h(ctx.handle, blk.ctypes.data_as(ctypes.c_void_p), d_adj.ptr + r * n * 8,
+                                          blk.nbytes))
+        s = int(blk.sum(axis=1).min())
+        deg_min = s if deg_min is None else min(deg_min, s)
+    d_adj.free()
+    bytes_per_trial = 22.88e9  # sum over recursion children of 8 (s^2 + t'^2) + 8 * 128^2 per leaf
+    return {"workload": f"dense Karger-Stein, complete graph n={n}, weights U[1,42], {trials} trials, seed 0",
+            "seconds_per_solve": dt, "trials_per_s": trials / dt, "cut": int(cut), "min_degree": deg_min,
+            "raw_cuts_equal_bound": bool(np.all(raw == deg_min)),
+            "roofline": {"bound": "hbm", "algorithmic_bytes_per_trial": bytes_per_trial,
+                         "achieved_gbs": bytes_per_trial * trials / dt / 1e9, "peak": HBM_PEAK_GBS,
+                         "frac": bytes_per_trial * trials / dt / 1e9 / HBM_PEAK_GBS}}
+
+
+def bench_c4(ctx, cac, count: int, stride: int):
+    """Batched square-root min cut over planted-cut graphs vs one graph per call."""
+    t0 = time.perf_counter()
+    descs, edges, planted = cac.gen_planted(0, count, 0)
+    gen_s = time.perf_counter() - t0
+    n = descs["n"].astype(np.int64)
+    m = descs["m"].astype(np.int64)
+    t = np.array([cac.intermediate_size(int(a), int(b)) for a, b in zip(n, m)])
+    descs["trials"] = np.maximum(2, [cac.ks_trials(int(x), F32_95) for x in t])
+    buf = ctx.alloc(edges.nbytes)
+    buf.upload(edges)
+    m_total = len(edges)
+    del edges
+    ctx.mincut_batch_dev(buf.ptr, descs[:32].copy(), F32_95, 0)  # warm-up
+    t0 = time.perf_counter()
+    cuts, trials = ctx.mincut_batch_dev(buf.ptr, descs, F32_95, 0)
+    dt = time.perf_counter() - t0
+    sub = np.arange(0, count, stride)
+    t0 = time.perf_counter()
+    single = [ctx.mincut_batch_dev(buf.ptr, descs[g:g + 1], F32_95, 0)[0][0] for g in sub]
+    dt1 = time.perf_counter() - t0
+    buf.free()
+    batched_rate, single_rate = count / dt, len(sub) / dt1
+    return {"workload": f"{count} planted-cut graphs, n log-uniform in [256, 65536] (mean {n.mean():.0f}), "
+                        f"halves G(n/2, 16/(n/2-1)) weights U[1,100], c in [1,8] unit cross edges, "
+                        f"{m_total} edges, generated in {gen_s:.1f} s",
+            "trial_rule": "T_g = max(2, ks_trials(t_g, 0.95)), t_g = intermediate_size(n_g, m_g)",
+            "total_trials": int(trials.sum()), "solves_per_s": batched_rate, "seconds": dt,
+            "one_graph_per_call_solves_per_s": single_rate,
+            "one_graph_per_call_sample": f"every {stride}th graph ({len(sub)} graphs)",
+            "speedup": batched_rate / single_rate,
+            "success_fraction": float((cuts == planted).mean()),
+            "batched_equals_single": bool(np.array_equal(np.array(single), cuts[sub]))}
+
+
+def bench_c5(cac, count: int, rate: float):
+    """Served Poisson stream of CC (70%) and min-cut (30%) queries."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(0)
+    ns = np.exp(rng.uniform(np.log(1e3), np.log(1e6), size=count)).astype(np.int64)
+    ds = rng.uniform(8, 64, size=count)
+    kinds = np.where(rng.random(count) < 0.7, cac.QUERY_CC, cac.QUERY_MINCUT)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    with ThreadPoolExecutor(16) as ex:
+        edges = list(ex.map(lambda i: cac.gen_gnp(int(ns[i]), float(ds[i]), 0, i, 100), range(count)))
+    recs = [cac.Server.query(int(kinds[i]), edges[i], int(ns[i]), F32_95, 0) for i in range(count)]
+    budget = 160 << 30
+    with cac.Server(0, workers=4, hbm_budget=budget, max_batch=64) as srv:
+        t0 = time.perf_counter()
+        res = srv.replay(recs, arrivals)
+        wall = time.perf_counter() - t0
+        peak, groups = srv.stats()
+    lat = res["done_s"] - res["submit_s"]
+    out = {"workload": f"{count} queries, Poisson arrivals at {rate}/s, 70% CC / 30% min cut (p=0.95), "
+                       f"G(n, d/n), n log-uniform in [1e3, 1e6], d uniform in [8, 64], weights U[1,100], "
+                       f"{sum(len(e) for e in edges)} edges in total; host edge lists, upload included",
+           "offered_qps": rate, "throughput_qps": count / wall, "all_ok": bool(np.all(res["status"] == 0)),
+           "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+           "hbm_budget_gib": budget / 2**30, "peak_admitted_gib": peak / 2**30, "launch_groups": groups,
+           "mean_group_size": float(res["batch_size"].mean())}
+    for k, name in ((cac.QUERY_CC, "cc"), (cac.QUERY_MINCUT, "mincut")):
+        lk = lat[kinds == k]
+        out[f"{name}_latency_ms"] = {"count": int(len(lk)), "p50": float(np.percentile(lk, 50) * 1e3),
+                                     "p99": float(np.percentile(lk, 99) * 1e3)}
+    # served min-cut throughput: the stream's min-cut queries all at once, grouped on 4 workers,
+    # vs the same queries one graph per launch group on one worker
+    mc = [recs[i] for i in range(count) if kinds[i] == cac.QUERY_MINCUT][:24]
+    rates = {}
+    for name, w, mb in (("grouped", 4, 64), ("one_graph_per_launch", 1, 1)):
+        with cac.Server(0, workers=w, hbm_budget=budget, max_batch=mb) as srv:
+            t0 = time.perf_counter()
+            r = srv.replay(mc, np.zeros(len(mc)))
+            rates[name] = len(mc) / (time.perf_counter() - t0)
+            assert np.all(r["status"] == 0)
+            if name == "grouped":
+                ref = r["cut"].copy()
+            else:
+                out["mincut_grouped_equals_single"] = bool(np.array_equal(ref, r["cut"]))
+    out["mincut_closed_loop_qps"] = rates
+    out["mincut_speedup"] = rates["grouped"] / rates["one_graph_per_launch"]
+    return out
 
 
 def main():
@@ -209,6 +371,19 @@ def main():
     }
     d_edges.free()
     d_lab.free()
+
+    wanted = set() if args.extras == "none" else set(args.extras.split(","))
+    extras = {}
+    for key, fn in (("c1", lambda: bench_c1(ctx, cac, min(args.cpu_baseline_seconds, 10.0))),
+                    ("c3", lambda: bench_c3(ctx, cac, args.c3_trials)),
+                    ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride)),
+                    ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate))):
+        if key in wanted:
+            t0 = time.perf_counter()
+            extras[key] = fn()
+            print(f"# {key} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if extras:
+        out["extras"] = extras
     ctx.close()
     print(json.dumps(out), flush=True)
 

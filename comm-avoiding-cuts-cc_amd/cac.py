@@ -88,6 +88,7 @@ SIGNATURES = {
     "cac_server_poll": (_i32, [_vp, _vp, _u32, _pu32, _i32]),
     "cac_server_clock": (ctypes.c_double, [_vp]),
     "cac_server_replay": (_i32, [_vp, _vp, _vp, _u32, _vp]),
+    "cac_server_stats": (_i32, [_vp, _pu64, _pu64]),
     "cac_server_destroy": (_i32, [_vp]),
     "cac_gen_gnp": (_i32, [_u32, ctypes.c_double, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_gen_planted_sizes": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _pu32, _pu64]),
@@ -365,6 +366,14 @@ class Server:
         if rc != CAC_OK:
             raise CacError(rc, "cac_server_poll")
         return out[: k.value]
+
+    def stats(self):
+        """(peak admitted footprint in bytes, launch groups run)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        rc = self.lib.cac_server_stats(self.handle, ctypes.byref(a), ctypes.byref(b))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_stats")
+        return int(a.value), int(b.value)
 
     def close(self):
         if self.handle:

@@ -72,6 +72,7 @@ struct cac_server {
   std::deque<cac::Pending> pending;
   std::vector<cac_query_result> done;
   uint64_t inflight = 0, next_id = 0;
+  uint64_t peak_inflight = 0, groups = 0;
   bool stop = false;
   std::vector<std::thread> workers;
   std::vector<cac_ctx*> ctxs;
@@ -322,6 +323,8 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
         }
       }
       S->inflight += bytes;
+      S->peak_inflight = std::max(S->peak_inflight, S->inflight);
+      ++S->groups;
       admit = S->now();
     }
     std::vector<cac_query_result> res(grp.size());
@@ -384,6 +387,14 @@ int cac_server_create(int device, uint32_t workers, uint64_t hbm_budget, uint32_
 }
 
 double cac_server_clock(cac_server* S) { return S ? S->now() : 0.0; }
+
+int cac_server_stats(cac_server* S, uint64_t* peak_inflight_bytes, uint64_t* groups) {
+  if (!S || !peak_inflight_bytes || !groups) return CAC_EINVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *peak_inflight_bytes = S->peak_inflight;
+  *groups = S->groups;
+  return CAC_OK;
+}
 
 int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
   if (!S || !q) return CAC_EINVAL;

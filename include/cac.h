@@ -231,6 +231,9 @@ int cac_server_submit(cac_server* srv, const cac_query* q, uint64_t* id_out);
  * one is ready or nothing is pending or running. */
 int cac_server_poll(cac_server* srv, cac_query_result* out, uint32_t cap, uint32_t* count_out, int wait);
 double cac_server_clock(cac_server* srv);
+/* Peak admitted footprint (bytes of device memory the admitted groups were
+ * estimated to hold at once) and the number of launch groups so far. */
+int cac_server_stats(cac_server* srv, uint64_t* peak_inflight_bytes, uint64_t* groups);
 /* Submits q[i] at arrival_s[i] seconds after the call (open-loop replay of an
  * arrival schedule, non-decreasing times) and returns every result, out[i]
  * for q[i].  No other thread may submit meanwhile. */
