@@ -114,6 +114,8 @@ int cac_ctx_destroy(cac_ctx* ctx) {
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  for (int b = 0; b < 2; ++b)
+    if (ctx->ks_buf[b]) (void)hipFree(ctx->ks_buf[b]);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -26,6 +26,10 @@ struct cac_ctx {
   bool profiling = false;
   std::vector<cac_prof_rec> prof;
   std::vector<hipEvent_t> event_pool;
+  // Two grow-only buffers for the levels of the Karger-Stein recursion (parents
+  // in one, children in the other).
+  void* ks_buf[2] = {nullptr, nullptr};
+  size_t ks_bytes[2] = {0, 0};
 };
 
 namespace cac {

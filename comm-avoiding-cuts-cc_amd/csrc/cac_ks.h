@@ -9,16 +9,20 @@
 
 namespace cac {
 
-// One trial's root: a dense n x n device matrix (row stride n) and where its raw
-// cut goes.  Several roots may share one matrix (independent trials).
+// One trial's root: a dense n x n device matrix (row stride n) with its row
+// sums, and where its raw cut goes.  Several roots may share one matrix
+// (independent trials).
 struct KsRoot {
   const int64_t* mat;
+  const int64_t* rsum;
   uint32_t n, trial, slot;
   int64_t bound;  // the trial's upper bound (minimum weighted degree)
 };
 
-// Minimum weighted degree of a dense device matrix into *d_bound (device).
-int ks_min_degree(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_bound, hipStream_t s);
+// Row sums of a dense device matrix into d_rsum and their minimum into *d_min.
+int ks_row_sums(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_rsum, int64_t* d_min, hipStream_t s);
+// Upper estimate of the device bytes one root needs for one level of children.
+size_t ks_level_bytes_per_root(uint32_t n);
 // Runs one trial per root; d_results[slot] (initialised to the root's bound)
 // receives min(bound, leaves).
 int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s);

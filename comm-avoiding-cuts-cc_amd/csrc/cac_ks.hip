@@ -1,15 +1,23 @@
-// cac_ks.hip -- dense Karger-Stein recursive contraction on gfx950.
+// cac_ks.hip -- dense Karger-Stein on gfx950: breadth-first recursive contraction
+// over all trials at once, a many-workgroup compaction per level, and
+// certificate-first leaves.
 //
-// Replaces comincut::minimum_cut / minimum_cut_try on dense graphs
-// (src/karger-stein/co_mincut.h:315-390, 466-490) and the recursive-contract base
-// case mincut::parallel_cut hands to it (src/recursive-contract/recursive_contract.cpp:262-266).
+// Replaces comincut::minimum_cut_trials_start (dense, src/karger-stein/co_mincut.h:370-390),
+// minimum_cut_trials (:315-342), recursive_contraction_inner (:176-223),
+// contract_graph (:38-66) and the base case deterministic_cut_pr
+// (co_mincut_base_case.hpp:164-185).  Sizes follow
+// target_number_of_vertices_for_contraction (co_mincut.cpp:18-20).
 //
-// Execution model: the recursion tree of every trial is walked breadth first.  A
-// level launch contracts every child of every live node at once (one workgroup
-// per child, so trials x 2^level workgroups); the host then gathers the leaves
-// (<= 128 vertices) and launches one wavefront per leaf.  PRNG draws are keyed
-// by (seed, trial, node id, pick), so the breadth-first order gives bit-identical
-// results to the depth-first CPU restatement.
+// Per level of the recursion tree:
+//   1. k_ks_contract<B>: one workgroup per child (B = 64, 256 or 1024 threads by
+//      size) draws s - t' picks, merges rows into pool rows, runs the PR pass and
+//      writes the column map and the live-row list;
+//   2. k_ks_compact: workgroups of a few output rows each sum live rows by column
+//      class into the child's t' x t' matrix, with row sums and row maxima;
+//   3. children of at most 128 vertices are leaves: k_ks_cert settles most of them
+//      from row sums and maxima; k_ks_sw runs Stoer-Wagner on the rest.
+// Device memory for a level comes from two grow-only buffers of the context used
+// alternately (the parents live in one, the children in the other).
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -22,20 +30,7 @@
 namespace cac {
 namespace {
 
-// Byte offset of the 2 x kLeafMax int64 leaf-certificate scratch in dynamic LDS.
-__host__ __device__ inline size_t leaf_scratch_offset(uint32_t s) {
-  return ((13 * (size_t)s + 15) & ~(size_t)15) + 16 * kKsWaves + 64;
-}
-
-struct KsLds {
-  int64_t* cap;      // [s]  capacities; rank/colmap scratch during compaction
-  uint32_t* rep;     // [s]  representatives; colmap during compaction
-  uint8_t* flags;    // [s]  bit0: row lives in work; bit1: live at compaction
-  uint64_t* red;     // [kKsWaves] reduction scratch
-  uint32_t* idx;     // [kKsWaves] index scratch
-};
-
-// ---- leaves --------------------------------------------------------------------
+// ---- leaves ------------------------------------------------------------------
 // One wavefront per leaf: lane l owns vertices l and l + 64.
 constexpr int kLeafMax = 128;
 
@@ -65,8 +60,10 @@ __device__ inline int64_t leaf_f(int64_t d, int64_t M, uint32_t q) {
   return d - k * M;
 }
 
-// Lower bound on the minimum cut (see leaf kernel): for |S| = q <= V/2,
-// cut(S) >= sum of the q smallest f_q(u) = d(u) - (q-1) M(u).
+// Lower bound on the minimum cut of a leaf: for a side S with |S| = q <= V/2,
+// cut(S) = sum over u in S of (d(u) - w(u, S \ u)) >= sum of f_q(u) over S, with
+// f_q(u) = d(u) - (q-1) M(u) and M(u) the heaviest edge at u; so the min over q
+// of the sum of the q smallest f_q bounds every cut from below.
 __device__ int64_t leaf_lower_bound(const int64_t d[2], const int64_t M[2], const bool alive[2], uint32_t V,
                                     int64_t stop_at) {
   const int lane = threadIdx.x & 63;
@@ -91,7 +88,6 @@ __device__ int64_t leaf_lower_bound(const int64_t d[2], const int64_t M[2], cons
           int64_t p0 = __shfl_xor(f0, j, 64), p1 = __shfl_xor(f1, j, 64);
           bool lower = (lane & j) == 0;
           bool up0 = ((lane) & k) == 0, up1 = ((lane + 64) & k) == 0;
-          // keep min when (lower == up)
           int64_t mn0 = f0 < p0 ? f0 : p0, mx0 = f0 < p0 ? p0 : f0;
           int64_t mn1 = f1 < p1 ? f1 : p1, mx1 = f1 < p1 ? p1 : f1;
           f0 = (lower == up0) ? mn0 : mx0;
@@ -108,322 +104,434 @@ __device__ int64_t leaf_lower_bound(const int64_t d[2], const int64_t M[2], cons
   return best;
 }
 
-__device__ inline const int64_t* row_of(const KsChild& c, const KsLds& L, uint32_t r) {
-  return (L.flags[r] & 1u) ? c.work + (size_t)r * c.s : c.parent + (size_t)r * c.s;
+// Certificate first: a leaf whose lower bound reaches the trial's bound
+// contributes the bound; the others are queued for Stoer-Wagner.
+__global__ void __launch_bounds__(64) k_ks_cert(const KsLeaf* __restrict__ leaves, int64_t* __restrict__ results,
+                                                uint32_t* __restrict__ sw_list, uint32_t* __restrict__ sw_count) {
+  const KsLeaf lf = leaves[blockIdx.x];
+  const uint32_t V = lf.k;
+  const int lane = threadIdx.x;
+  const bool alive[2] = {(uint32_t)lane < V, (uint32_t)lane + 64 < V};
+  int64_t d[2] = {0, 0}, M[2] = {0, 0};
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t u = lane + 64 * h;
+    if (!alive[h]) continue;
+    if (lf.rsum && lf.rmax) {
+      d[h] = lf.rsum[u];
+      M[h] = lf.rmax[u];
+    } else {
+      int64_t sm = 0, mx = 0;
+      for (uint32_t x = 0; x < V; ++x) {
+        const int64_t v = lf.mat[(size_t)u * V + x];
+        sm += v;
+        mx = v > mx ? v : mx;
+      }
+      d[h] = sm;
+      M[h] = mx;
+    }
+  }
+  const int64_t lb = V >= 2 ? leaf_lower_bound(d, M, alive, V, lf.bound) : 0;
+  if (lane == 0) {
+    if (lb >= lf.bound) atomicMin((long long*)&results[lf.slot], (long long)lf.bound);
+    else sw_list[atomicAdd(sw_count, 1u)] = blockIdx.x;
+  }
 }
 
-// Merge row a into row b (a dies).  Returns the contracted weight
-// (cap[a] + cap[b] - cap_new) / 2.  Called by the whole block.
-__device__ int64_t ks_merge(const KsChild& c, const KsLds& L, uint32_t a, uint32_t b) {
-  const uint32_t s = c.s;
-  // every wave has read the representatives that chose (a, b) before any is relabelled
+// Merge vertex `last` into `prev` in the LDS matrix (row and column); lanes own
+// columns lane and lane + 64.
+__device__ inline void leaf_merge(int64_t* A, uint32_t V, uint32_t prev, uint32_t last, bool (&alive)[2]) {
+  const int lane = threadIdx.x;
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t x = lane + 64 * h;
+    if (x < V && x != prev && x != last) {
+      const int64_t v = A[prev * kLeafMax + x] + A[last * kLeafMax + x];
+      A[prev * kLeafMax + x] = v;
+      A[x * kLeafMax + prev] = v;
+    }
+  }
   __syncthreads();
-  for (uint32_t x = threadIdx.x; x < s; x += kKsBlock)
+  if (lane == 0) {
+    A[prev * kLeafMax + last] = 0;
+    A[last * kLeafMax + prev] = 0;
+  }
+  if ((uint32_t)lane == last) alive[0] = false;
+  if ((uint32_t)lane + 64 == last) alive[1] = false;
+  __syncthreads();
+}
+
+// The base case of co_mincut_base_case.hpp:164-185 on a leaf in LDS: PR-1 pass
+// at the bound (contracting an edge of weight >= bound keeps every cut below the
+// bound, so min(bound, min cut) is unchanged), then Stoer-Wagner (J. ACM 44(4),
+// 1997): maximum adjacency order, cut of the phase, merge of the last two
+// vertices.  Result min(cut, bound).
+__global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves, const uint32_t* __restrict__ sw_list,
+                                              int64_t* __restrict__ results) {
+  __shared__ int64_t A[kLeafMax * kLeafMax];
+  const KsLeaf lf = leaves[sw_list[blockIdx.x]];
+  const uint32_t V = lf.k;
+  const int lane = threadIdx.x;
+  for (uint32_t r = 0; r < V; ++r)
+    for (uint32_t x = lane; x < V; x += 64) A[r * kLeafMax + x] = lf.mat[(size_t)r * V + x];
+  __syncthreads();
+  bool alive[2] = {(uint32_t)lane < V, (uint32_t)(lane + 64) < V};
+  int64_t best = INT64_MAX;
+  uint32_t nalive = V;
+  // PR-1: repeatedly contract an edge of weight >= bound (the lowest live row
+  // holding one, its lowest live column); merged-away columns hold stale values
+  // and are skipped
+  while (nalive > 1) {
+    const uint64_t live_lo = __ballot(alive[0]), live_hi = __ballot(alive[1]);
+    uint32_t col[2] = {kNoSlot, kNoSlot};
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t u = lane + 64 * h;
+      if (!alive[h]) continue;
+      for (uint32_t x = 0; x < V; ++x) {
+        const bool live_x = ((x < 64 ? live_lo : live_hi) >> (x & 63)) & 1ull;
+        if (live_x && x != u && A[u * kLeafMax + x] >= lf.bound) {
+          col[h] = x;
+          break;
+        }
+      }
+    }
+    const uint64_t lo_mask = __ballot(col[0] != kNoSlot), hi_mask = __ballot(col[1] != kNoSlot);
+    if (!lo_mask && !hi_mask) break;
+    const uint32_t u = lo_mask ? (uint32_t)(__ffsll((long long)lo_mask) - 1)
+                               : 64u + (uint32_t)(__ffsll((long long)hi_mask) - 1);
+    const uint32_t x = __shfl(u < 64 ? col[0] : col[1], (int)(u & 63), 64);
+    leaf_merge(A, V, u < x ? u : x, u < x ? x : u, alive);
+    --nalive;
+  }
+  if (lf.bound <= 0) nalive = 1;  // a non-positive bound is already the answer
+  while (nalive > 1) {
+    int64_t key[2] = {0, 0};
+    bool added[2] = {!alive[0], !alive[1]};
+    // start from the smallest live vertex
+    uint64_t live_lo = __ballot(alive[0]), live_hi = __ballot(alive[1]);
+    uint32_t start =
+        live_lo ? (uint32_t)(__ffsll((long long)live_lo) - 1) : 64u + (uint32_t)(__ffsll((long long)live_hi) - 1);
+    uint32_t prev = start, last = start;
+    if ((uint32_t)lane == start) added[0] = true;
+    if ((uint32_t)lane + 64 == start) added[1] = true;
+    for (int h = 0; h < 2; ++h) key[h] += A[start * kLeafMax + lane + 64 * h];
+    int64_t cut = 0;
+    for (uint32_t step = 1; step < nalive; ++step) {
+      int64_t bk = INT64_MIN;
+      uint32_t bi = 0xFFFFFFFFu;
+      for (int h = 0; h < 2; ++h) {
+        if (!added[h] && (key[h] > bk || (key[h] == bk && (uint32_t)lane + 64 * h < bi))) {
+          bk = key[h];
+          bi = lane + 64 * h;
+        }
+      }
+      wave_argmax(bk, bi);
+      if (bi >= V) break;  // no candidate: cannot happen for step < nalive
+      prev = last;
+      last = bi;
+      if ((uint32_t)lane == bi) added[0] = true;
+      if ((uint32_t)lane + 64 == bi) added[1] = true;
+      if (step + 1 == nalive) {
+        cut = bk;
+        break;
+      }
+      for (int h = 0; h < 2; ++h)
+        if (!added[h]) key[h] += A[bi * kLeafMax + lane + 64 * h];
+    }
+    best = cut < best ? cut : best;
+    leaf_merge(A, V, prev, last, alive);
+    --nalive;
+  }
+  best = best < lf.bound ? best : lf.bound;
+  if (lane == 0) atomicMin((long long*)&results[lf.slot], (long long)best);
+}
+
+// ---- contraction -------------------------------------------------------------
+// LDS layout for a child of s vertices: cap[s] i64 | cbs[nb] u64 | rep[s] u32 |
+// wslot[s] u32 | red[B/64] u64 | misc[8] u32.
+__host__ __device__ inline size_t contract_lds_bytes(uint32_t s, int B) {
+  const size_t nb = (s + 63) / 64;
+  return 8 * (size_t)s + 8 * nb + 8 * (size_t)s + 8 * (size_t)(B / 64) + 32;
+}
+
+struct KsLds {
+  int64_t* cap;
+  uint64_t* cbs;
+  uint32_t* rep;
+  uint32_t* wslot;
+  uint64_t* red;
+  uint32_t* misc;  // [0] i  [1] k  [2] slot of the other row  [3] destination  [4] overflow
+};
+
+__device__ inline const int64_t* ks_row(const KsChild& c, const KsPool& pool, uint32_t slot, uint32_t r) {
+  return slot == kNoSlot ? c.parent + (size_t)r * c.s : pool.base + (size_t)slot * kKsUnit;
+}
+
+__device__ inline void load_chunk(const int64_t* row, uint32_t lo, uint32_t cc, uint32_t s, int64_t (&v)[kKsChunk]) {
+#pragma unroll
+  for (uint32_t k = 0; k < kKsChunk; ++k) v[k] = (k < cc && lo + k < s) ? row[lo + k] : 0;
+}
+
+// Merge row a into row b (a dies, b keeps its index).  The calling threads hold
+// their chunk of row `held` (a or b) in v; the other row is loaded here.  The
+// merged row goes to b's pool row (taken now if b has none).  Returns the
+// contracted weight (cap[a] + cap[b] - cap_new) / 2 in w; false when the pool is
+// exhausted.  Called by the whole block.
+template <int B>
+__device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, uint32_t a, uint32_t b, uint32_t held,
+                         const int64_t (&v)[kKsChunk], uint32_t cc, int64_t& w) {
+  const uint32_t s = c.s;
+  const uint32_t other = held == a ? b : a;
+  if (threadIdx.x == 0) {
+    L.misc[2] = L.wslot[other];
+    uint32_t db = L.wslot[b];
+    uint32_t overflow = 0;
+    if (db == kNoSlot) {
+      const unsigned long long units = (s + kKsUnit - 1) / kKsUnit;
+      const unsigned long long off = atomicAdd(pool.used_units, units);
+      if (off + units > pool.cap_units) {
+        overflow = 1;
+      } else {
+        db = (uint32_t)off;
+        L.wslot[b] = db;
+      }
+    }
+    L.misc[3] = db;
+    L.misc[4] = overflow;
+  }
+  __syncthreads();  // every thread has read rep[k] and the slots before they change
+  if (L.misc[4]) return false;
+  const uint32_t so = L.misc[2], db = L.misc[3];
+  for (uint32_t x = threadIdx.x; x < s; x += B)
     if (L.rep[x] == a) L.rep[x] = b;
   __syncthreads();
-  const int64_t* ra = row_of(c, L, a);
-  const int64_t* rb = row_of(c, L, b);
-  int64_t* dst = c.work + (size_t)b * s;
+  const int64_t* ro = ks_row(c, pool, so, other);
+  int64_t* dst = pool.base + (size_t)db * kKsUnit;
+  const uint32_t lo = threadIdx.x * cc;
+  int64_t u[kKsChunk];
+  load_chunk(ro, lo, cc, s, u);
   uint64_t sum = 0;
-#pragma unroll 8
-  for (uint32_t x = threadIdx.x; x < s; x += kKsBlock) {
-    int64_t v = rb[x] + ra[x];
-    if (L.rep[x] == b) v = 0;
-    dst[x] = v;
-    sum += (uint64_t)v;
+#pragma unroll
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    const uint32_t x = lo + k;
+    if (k < cc && x < s) {
+      int64_t t = v[k] + u[k];
+      if (L.rep[x] == b) t = 0;  // loop of the merged vertex
+      dst[x] = t;                // each thread reads back only its own chunk later
+      sum += (uint64_t)t;
+    }
   }
-  // the new row b is read by other waves in later picks and the compaction
-  block_sync_global();
-  const int64_t newcap = (int64_t)block_sum_u64(sum, L.red);
-  const int64_t w = (L.cap[a] + L.cap[b] - newcap) / 2;
-  __syncthreads();
+  const int64_t ca = L.cap[a], cb = L.cap[b];
+  const int64_t newcap = (int64_t)block_sum_u64<B>(sum, L.red);
+  w = (ca + cb - newcap) / 2;
+  if (B == 64) __syncthreads();  // every lane has read cap[a], cap[b]
   if (threadIdx.x == 0) {
-    L.cap[b] = newcap;
+    L.cbs[a / 64] -= (uint64_t)ca;
     L.cap[a] = 0;
-    L.flags[b] |= 1u;
+    L.cbs[b / 64] += (uint64_t)(newcap - cb);
+    L.cap[b] = newcap;
   }
   __syncthreads();
-  return w;
+  return true;
 }
 
-__global__ void __launch_bounds__(kKsBlock) k_ks_contract(const KsChild* __restrict__ children, uint32_t seed,
-                                                          uint32_t* __restrict__ k_out, int64_t* __restrict__ results) {
+template <int B>
+__global__ void __launch_bounds__(B) k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t seed,
+                                                   uint32_t* __restrict__ k_out, int64_t* __restrict__ results) {
   extern __shared__ __align__(16) unsigned char smem[];
   const KsChild c = children[blockIdx.x];
-  const uint32_t s = c.s;
+  const uint32_t s = c.s, nb = (s + 63) / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   KsLds L;
   L.cap = (int64_t*)smem;
-  L.rep = (uint32_t*)(smem + 8 * (size_t)s);
-  L.flags = smem + 12 * (size_t)s;
-  L.red = (uint64_t*)(smem + ((13 * (size_t)s + 15) & ~(size_t)15));
-  L.idx = (uint32_t*)(L.red + kKsWaves);
+  L.cbs = (uint64_t*)(smem + 8 * (size_t)s);
+  L.rep = (uint32_t*)(smem + 8 * (size_t)s + 8 * nb);
+  L.wslot = L.rep + s;
+  L.red = (uint64_t*)(smem + 16 * (size_t)s + 8 * nb);
+  L.misc = (uint32_t*)(L.red + B / 64);
 
-  // capacities = row sums of the parent; representatives = identity
-  for (uint32_t x = threadIdx.x; x < s; x += kKsBlock) {
+  for (uint32_t x = threadIdx.x; x < s; x += B) {
+    L.cap[x] = c.prsum[x];
     L.rep[x] = x;
-    L.flags[x] = 0;
-  }
-  uint64_t W = 0;
-  for (uint32_t r = threadIdx.x >> 6; r < s; r += kKsWaves) {
-    const int64_t* row = c.parent + (size_t)r * s;
-    uint64_t acc = 0;
-    for (uint32_t x = threadIdx.x & 63; x < s; x += 64) acc += (uint64_t)row[x];
-    acc = wave_sum_u64(acc);
-    if ((threadIdx.x & 63) == 0) L.cap[r] = (int64_t)acc;
+    L.wslot[x] = kNoSlot;
   }
   __syncthreads();
-  {
-    uint64_t part = 0;
-    for (uint32_t x = threadIdx.x; x < s; x += kKsBlock) part += (uint64_t)L.cap[x];
-    W = block_sum_u64(part, L.red);
+  for (uint32_t blk = wave; blk < nb; blk += B / 64) {
+    const uint32_t x = blk * 64 + lane;
+    const uint64_t t = wave_sum_u64(x < s ? (uint64_t)L.cap[x] : 0);
+    if (lane == 0) L.cbs[blk] = t;
   }
+  __syncthreads();
+  uint64_t W = 0;
+  for (uint32_t blk = 0; blk < nb; ++blk) W += L.cbs[blk];
 
+  const uint32_t cc = (s + B - 1) / B;  // entries per thread, <= kKsChunk
+  const uint32_t lo = threadIdx.x * cc;
   uint32_t nv = s, pr1 = 0;
-  bool disconnected = false, bad = false;
+  int status = 0;  // 1 disconnected, 2 bad selection, 3 pool exhausted
+  int64_t v[kKsChunk];
   for (uint32_t p = 0; p + c.tgt < s; ++p) {
     if (W == 0) {
-      disconnected = true;
+      status = 1;
       break;
     }
     const u32x4 rnd = philox4x32(c.trial, c.nid, p, 0u, seed, kDomKs);
-    const uint32_t i = block_select([&](uint32_t x) { return L.cap[x]; }, s, uniform1(lo64(rnd), W), L.red, L.idx);
-    if (i >= s) {  // capacities inconsistent with W: never expected; abandon this child
-      bad = true;
+    if (wave == 0) {
+      const uint32_t i = wave_select_two_level(L.cap, L.cbs, s, nb, uniform1(lo64(rnd), W));
+      if (lane == 0) L.misc[0] = i;
+    }
+    __syncthreads();
+    const uint32_t i = L.misc[0];
+    if (i >= s) {
+      status = 2;
       break;
     }
-    const int64_t* rowi = row_of(c, L, i);
-    const uint64_t ci = (uint64_t)L.cap[i];
-    const uint32_t k = block_select_row(rowi, s, uniform1(hi64(rnd), ci), L.red, L.idx);
+    load_chunk(ks_row(c, pool, L.wslot[i], i), lo, cc, s, v);
+    const uint32_t k = block_select_chunk<B>(v, cc, s, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1);
     if (k >= s) {
-      bad = true;
+      status = 2;
       break;
     }
     const uint32_t j = L.rep[k];
     const uint32_t a = i > j ? i : j, b = i > j ? j : i;
-    const int64_t w = ks_merge(c, L, a, b);
+    int64_t w = 0;
+    if (!ks_merge<B>(c, pool, L, a, b, i, v, cc, w)) {
+      status = 3;
+      break;
+    }
     W -= 2 * (uint64_t)w;
     --nv;
     if (w >= c.bound) ++pr1;
   }
-  if (disconnected || bad) {
+  // PR pass (indexed_adjacency_matrix::pr_pass): contract every edge of weight >= bound
+  if (status == 0 && pr1 > 0 && c.bound > 0) {
+    for (uint32_t i = 0; i < s && nv > 2; ++i) {
+      if (L.rep[i] != i || L.cap[i] < c.bound) continue;
+      load_chunk(ks_row(c, pool, L.wslot[i], i), lo, cc, s, v);
+      uint32_t first = kNoSlot;
+#pragma unroll
+      for (uint32_t k = 0; k < kKsChunk; ++k)
+        if (first == kNoSlot && k < cc && lo + k < s && v[k] >= c.bound) first = lo + k;
+      first = block_min_u32<B>(first, (uint32_t*)L.red);
+      if (first == kNoSlot) continue;
+      int64_t w = 0;
+      if (!ks_merge<B>(c, pool, L, i, L.rep[first], i, v, cc, w)) {
+        status = 3;
+        break;
+      }
+      --nv;
+    }
+  }
+  if (status != 0) {
     if (threadIdx.x == 0) {
-      k_out[blockIdx.x] = bad ? 0xFFFFFFFFu : 0u;
-      if (disconnected) atomicMin((long long*)&results[c.slot], 0ll);
+      k_out[blockIdx.x] = status == 1 ? 0u : status == 2 ? kKsBad : kKsOverflow;
+      if (status == 1) atomicMin((long long*)&results[c.slot], 0ll);
     }
     return;
   }
-
-  // PR pass (indexed_adjacency_matrix::pr_pass): contract edges of weight >= bound
-  if (pr1 > 0 && c.bound > 0) {
-    for (uint32_t i = 0; i < s && nv > 2; ++i) {
-      if (L.rep[i] != i || L.cap[i] < c.bound) continue;
-      const int64_t* row = row_of(c, L, i);
-      uint32_t first = 0xFFFFFFFFu;
-      for (uint32_t x = threadIdx.x; x < s; x += kKsBlock)
-        if (row[x] >= c.bound) {
-          first = x;
-          break;
-        }
-      first = block_min_u32(first, L.idx);
-      if (first != 0xFFFFFFFFu) {
-        const uint32_t b = L.rep[first];
-        (void)ks_merge(c, L, i, b);
-        --nv;
-      }
-    }
-  }
-
-  // ---- compaction: rank live rows, sum columns by class --------------------
-  __syncthreads();  // the PR loop's reads of cap/rep precede the overwrites below
-  uint32_t* rank = (uint32_t*)L.cap;           // [s] in the first half of cap
-  uint32_t* colmap_tmp = rank + s;             // [s] in the second half of cap
-  {
-    // block exclusive scan of live flags, 4-wave chunks of kKsBlock
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < s; base += kKsBlock) {
-      const uint32_t x = base + threadIdx.x;
-      const uint32_t live = (x < s && L.rep[x] == x) ? 1u : 0u;
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      uint32_t inc = live;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-      }
-      if (lane == 63) L.idx[wave] = inc;
-      __syncthreads();
-      uint32_t off = carry, tot = 0;
-      for (int w = 0; w < kKsWaves; ++w) {
-        uint32_t v = L.idx[w];
-        if (w < wave) off += v;
-        tot += v;
-      }
-      if (x < s) {
-        rank[x] = off + inc - live;
-        if (live) L.flags[x] |= 2u;
-      }
-      carry += tot;
-      __syncthreads();
-    }
-    nv = carry;
-  }
-  for (uint32_t y = threadIdx.x; y < s; y += kKsBlock) colmap_tmp[y] = rank[L.rep[y]];
+  // ranks of the live rows (natural order), live-row list, column map
   __syncthreads();
-  uint32_t* colmap = L.rep;
-  for (uint32_t y = threadIdx.x; y < s; y += kKsBlock) colmap[y] = colmap_tmp[y];
-  __syncthreads();
-  unsigned long long* acc = (unsigned long long*)L.cap;  // [nv] accumulators
-  const uint32_t kk = nv;
-  uint32_t out_row = 0;
-  for (uint32_t a = 0; a < s; ++a) {
-    if (!(L.flags[a] & 2u)) continue;
-    for (uint32_t x = threadIdx.x; x < kk; x += kKsBlock) acc[x] = 0ull;
-    __syncthreads();
-    const int64_t* row = row_of(c, L, a);
-    for (uint32_t y = threadIdx.x; y < s; y += kKsBlock) {
-      const int64_t v = row[y];
-      if (v) atomicAdd(&acc[colmap[y]], (unsigned long long)v);
-    }
-    __syncthreads();
-    int64_t* dst = c.out + (size_t)out_row * kk;
-    for (uint32_t x = threadIdx.x; x < kk; x += kKsBlock) dst[x] = (int64_t)acc[x];
-    __syncthreads();
-    ++out_row;
-  }
-  // A child that is already a leaf is checked right here: row sums and maxima of
-  // its kk x kk output feed the lower-bound certificate; when it proves every cut
-  // of the leaf is >= bound the leaf contributes exactly the bound and needs no
-  // Stoer-Wagner pass (flag bit 31 of k_out).
-  uint32_t flag = 0;
-  if (kk <= kLeafMax && kk >= 2) {
-    block_sync_global();
-    int64_t* dl = (int64_t*)(smem + leaf_scratch_offset(s));
-    int64_t* ml = dl + kLeafMax;
-    for (uint32_t r = threadIdx.x >> 6; r < kk; r += kKsWaves) {
-      const int64_t* row = c.out + (size_t)r * kk;
-      int64_t sm = 0, mx = 0;
-      for (uint32_t x = threadIdx.x & 63; x < kk; x += 64) {
-        const int64_t v = row[x];
-        sm += v;
-        mx = v > mx ? v : mx;
-      }
+  uint32_t* rank = (uint32_t*)L.cap;  // capacities are no longer needed
+  uint32_t cnt = 0;
 #pragma unroll
-      for (int d = 32; d > 0; d >>= 1) {
-        sm += __shfl_xor(sm, d, 64);
-        const int64_t o = __shfl_xor(mx, d, 64);
-        mx = o > mx ? o : mx;
-      }
-      if ((threadIdx.x & 63) == 0) {
-        dl[r] = sm;
-        ml[r] = mx;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
-      const bool alive[2] = {(uint32_t)lane < kk, (uint32_t)lane + 64 < kk};
-      const int64_t d2[2] = {alive[0] ? dl[lane] : 0, alive[1] ? dl[lane + 64] : 0};
-      const int64_t m2[2] = {alive[0] ? ml[lane] : 0, alive[1] ? ml[lane + 64] : 0};
-      const int64_t lb = leaf_lower_bound(d2, m2, alive, kk, c.bound);
-      if (lb >= c.bound) flag = 0x80000000u;
+  for (uint32_t k = 0; k < kKsChunk; ++k)
+    if (k < cc && lo + k < s && L.rep[lo + k] == lo + k) ++cnt;
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) L.red[wave] = inc;
+  __syncthreads();
+  uint32_t ex = inc - cnt, total = 0;
+  for (int w2 = 0; w2 < B / 64; ++w2) {
+    if (w2 < wave) ex += (uint32_t)L.red[w2];
+    total += (uint32_t)L.red[w2];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    const uint32_t x = lo + k;
+    if (k < cc && x < s && L.rep[x] == x) {
+      rank[x] = ex;
+      const uint32_t sl = L.wslot[x];
+      c.live[ex] = sl == kNoSlot ? x : (kLiveWork | sl);
+      ++ex;
     }
   }
-  if (threadIdx.x == 0) k_out[blockIdx.x] = kk | flag;
+  __syncthreads();
+  for (uint32_t y = threadIdx.x; y < s; y += B) c.colmap[y] = rank[L.rep[y]];
+  if (threadIdx.x == 0) k_out[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(64) k_ks_leaf(const KsLeaf* __restrict__ leaves, int64_t* __restrict__ results) {
-  __shared__ int64_t A[kLeafMax * kLeafMax];
-  const KsLeaf lf = leaves[blockIdx.x];
-  const uint32_t V = lf.k;
-  const int lane = threadIdx.x;
-  // load (row stride kLeafMax), degrees and row maxima
-  int64_t d[2] = {0, 0}, M[2] = {0, 0};
-  for (uint32_t r = 0; r < V; ++r) {
-    for (uint32_t x = lane; x < V; x += 64) A[r * kLeafMax + x] = lf.mat[(size_t)r * V + x];
+// Rows per compaction workgroup for a child of t' output vertices (the LDS
+// accumulator holds rows x t' int64, at most 32 KiB unless t' alone is larger).
+__host__ __device__ inline uint32_t compact_rows(uint32_t tgt) {
+  const uint32_t r = 4096 / (tgt ? tgt : 1);
+  return r < 1 ? 1 : (r > 64 ? 64 : r);
+}
+
+// One workgroup per (child, group of output rows): sums every live row by column
+// class into LDS, then writes the rows with their sums and maxima.  The work
+// item's child comes from a binary search of the per-child item prefix.
+__global__ void __launch_bounds__(256) k_ks_compact(const KsChild* __restrict__ children,
+                                                    const uint32_t* __restrict__ k_out,
+                                                    const uint32_t* __restrict__ item_prefix, uint32_t nchild,
+                                                    KsPool pool) {
+  extern __shared__ __align__(16) unsigned long long acc[];
+  const uint32_t item = blockIdx.x;
+  uint32_t lo = 0, hi = nchild;  // last child with item_prefix[child] <= item
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (item_prefix[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  const KsChild c = children[lo];
+  const uint32_t kk = k_out[lo];
+  if (kk == 0 || kk > c.tgt) return;  // disconnected child, or an error code
+  const uint32_t R = compact_rows(c.tgt);
+  const uint32_t r0 = (item - item_prefix[lo]) * R;
+  if (r0 >= kk) return;
+  const uint32_t nr = min(R, kk - r0);
+  const uint32_t s = c.s;
+  for (uint32_t x = threadIdx.x; x < nr * kk; x += 256) acc[x] = 0ull;
+  __syncthreads();
+  for (uint32_t rr = 0; rr < nr; ++rr) {
+    const uint32_t e = c.live[r0 + rr];
+    const int64_t* src =
+        (e & kLiveWork) ? pool.base + (size_t)(e & ~kLiveWork) * kKsUnit : c.parent + (size_t)e * s;
+    unsigned long long* a = acc + (size_t)rr * kk;
+    for (uint32_t y = threadIdx.x; y < s; y += 256) {
+      const int64_t v = src[y];
+      if (v) atomicAdd(&a[c.colmap[y]], (unsigned long long)v);
+    }
   }
   __syncthreads();
-  bool alive[2] = {(uint32_t)lane < V, (uint32_t)(lane + 64) < V};
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t u = lane + 64 * h;
-    if (!alive[h]) continue;
-    int64_t s = 0, mx = 0;
-    for (uint32_t x = 0; x < V; ++x) {
-      int64_t v = A[u * kLeafMax + x];
-      s += v;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t rr = wave; rr < nr; rr += 4) {
+    const uint32_t r = r0 + rr;
+    int64_t* dst = c.out + (size_t)r * kk;
+    int64_t sm = 0, mx = 0;
+    for (uint32_t x = lane; x < kk; x += 64) {
+      const int64_t v = (int64_t)acc[(size_t)rr * kk + x];
+      dst[x] = v;
+      sm += v;
       mx = v > mx ? v : mx;
     }
-    d[h] = s;
-    M[h] = mx;
-  }
-  int64_t best;
-  const int64_t lb = V >= 2 ? leaf_lower_bound(d, M, alive, V, lf.bound) : 0;
-  if (lb >= lf.bound) {
-    best = lf.bound;  // every cut of the leaf is at least the bound
-  } else {
-    // Stoer-Wagner: maximum adjacency order, cut of the phase, merge last two
-    best = INT64_MAX;
-    uint32_t nalive = V;
-    while (nalive > 1) {
-      int64_t key[2] = {0, 0};
-      bool added[2] = {!alive[0], !alive[1]};
-      // start from the smallest live vertex
-      uint64_t live_lo = __ballot(alive[0]), live_hi = __ballot(alive[1]);
-      uint32_t start = live_lo ? (uint32_t)(__ffsll((long long)live_lo) - 1)
-                               : 64u + (uint32_t)(__ffsll((long long)live_hi) - 1);
-      uint32_t prev = start, last = start;
-      if ((uint32_t)lane == start) added[0] = true;
-      if ((uint32_t)lane + 64 == start) added[1] = true;
-      for (int h = 0; h < 2; ++h) key[h] += A[start * kLeafMax + lane + 64 * h];
-      int64_t cut = 0;
-      for (uint32_t step = 1; step < nalive; ++step) {
-        int64_t bk = INT64_MIN;
-        uint32_t bi = 0xFFFFFFFFu;
-        for (int h = 0; h < 2; ++h) {
-          if (!added[h] && (key[h] > bk || (key[h] == bk && (uint32_t)lane + 64 * h < bi))) {
-            bk = key[h];
-            bi = lane + 64 * h;
-          }
-        }
-        wave_argmax(bk, bi);
-        if (bi >= V) break;  // no candidate: cannot happen for step < nalive
-        prev = last;
-        last = bi;
-        if ((uint32_t)lane == bi) added[0] = true;
-        if ((uint32_t)lane + 64 == bi) added[1] = true;
-        if (step + 1 == nalive) {
-          cut = bk;
-          break;
-        }
-        for (int h = 0; h < 2; ++h)
-          if (!added[h]) key[h] += A[bi * kLeafMax + lane + 64 * h];
-      }
-      best = cut < best ? cut : best;
-      // merge 'last' into 'prev' (row and column)
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t x = lane + 64 * h;
-        if (x < V && x != prev && x != last) {
-          const int64_t v = A[prev * kLeafMax + x] + A[last * kLeafMax + x];
-          A[prev * kLeafMax + x] = v;
-          A[x * kLeafMax + prev] = v;
-        }
-      }
-      __syncthreads();
-      if (lane == 0) {
-        A[prev * kLeafMax + last] = 0;
-        A[last * kLeafMax + prev] = 0;
-      }
-      if ((uint32_t)lane == last) alive[0] = false;
-      if ((uint32_t)lane + 64 == last) alive[1] = false;
-      __syncthreads();
-      --nalive;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      sm += __shfl_xor(sm, d, 64);
+      const int64_t o = __shfl_xor(mx, d, 64);
+      mx = o > mx ? o : mx;
     }
-    best = best < lf.bound ? best : lf.bound;
+    if (lane == 0) {
+      c.orsum[r] = sm;
+      c.ormax[r] = mx;
+    }
   }
-  if (lane == 0) atomicMin((long long*)&results[lf.slot], (long long)best);
 }
 
-__global__ void __launch_bounds__(256) k_min_degree(const int64_t* __restrict__ A, uint32_t n, int64_t* out_bound) {
-  // one wave per row, block-level min, one atomic per block
+// Row sums of a dense matrix (one wavefront per row) and their minimum.
+__global__ void __launch_bounds__(256) k_row_sums(const int64_t* __restrict__ A, uint32_t n, int64_t* __restrict__ rsum,
+                                                  int64_t* out_min) {
   __shared__ long long wmin[4];
   long long best = INT64_MAX;
   for (uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < n; r += gridDim.x * 4) {
@@ -431,6 +539,7 @@ __global__ void __launch_bounds__(256) k_min_degree(const int64_t* __restrict__ 
     uint64_t acc = 0;
     for (uint32_t x = threadIdx.x & 63; x < n; x += 64) acc += (uint64_t)row[x];
     acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) rsum[r] = (int64_t)acc;
     best = (long long)acc < best ? (long long)acc : best;
   }
   if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
@@ -438,13 +547,39 @@ __global__ void __launch_bounds__(256) k_min_degree(const int64_t* __restrict__ 
   if (threadIdx.x == 0) {
     long long b = wmin[0];
     for (int w = 1; w < 4; ++w) b = wmin[w] < b ? wmin[w] : b;
-    atomicMin((long long*)out_bound, b);
+    atomicMin((long long*)out_min, b);
   }
 }
 
 __global__ void k_fill_i64(int64_t* p, uint32_t count, int64_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_zero_words(unsigned long long* p, uint32_t count) {
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) p[i] = 0ull;
+}
+
+// Block size for a child of s vertices: the smallest power of two from 64 up
+// with at most kKsChunk row entries per thread.
+inline int contract_block(uint32_t s) {
+  int B = 64;
+  while (B < 1024 && (uint32_t)B * kKsChunk < s) B *= 2;
+  return B;
+}
+
+// Bump carving with 64-byte alignment (level buffers hold millions of small arrays).
+struct Carve64 {
+  char* base;
+  size_t off = 0;
+  explicit Carve64(void* b) : base((char*)b) {}
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 63) & ~size_t(63);
+    T* p = (T*)(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+inline size_t c64(size_t bytes) { return (bytes + 63) & ~size_t(63); }
 
 }  // namespace
 
@@ -455,117 +590,231 @@ uint32_t ks_target_host(uint32_t s) {
 }
 
 namespace {
-size_t contract_lds_bytes(uint32_t s) { return leaf_scratch_offset(s) + 2 * kLeafMax * 8; }
 inline uint32_t ks_target(uint32_t s) { return ks_target_host(s); }
-}  // namespace
 
-int ks_min_degree(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_bound, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, d_bound, 1u, (int64_t)INT64_MAX);
-  hipLaunchKernelGGL(k_min_degree, dim3(grid_for(n, 4, 2048)), dim3(256), 0, s, d_adj, n, d_bound);
-  CAC_LAUNCH_CHECK(ctx, "min degree");
+// Grow-only level buffer `which` of the context.
+int ks_reserve(cac_ctx* ctx, int which, size_t bytes, void** out) {
+  if (bytes > ctx->ks_bytes[which]) {
+    if (ctx->ks_buf[which]) CAC_HIP(ctx, hipFree(ctx->ks_buf[which]));
+    ctx->ks_buf[which] = nullptr;
+    ctx->ks_bytes[which] = 0;
+    CAC_HIP(ctx, hipMalloc(&ctx->ks_buf[which], bytes));
+    ctx->ks_bytes[which] = bytes;
+  }
+  *out = ctx->ks_buf[which];
   return CAC_OK;
 }
 
-// Runs one Karger-Stein trial per root.  roots[i]: device matrix (n_i x n_i),
-// trial id, result slot and bound; d_results[slot] must hold the bound already.
+// Device bytes of the per-child arrays of one child (out, sums, maxima, map, list).
+size_t child_bytes(uint32_t s, uint32_t t) {
+  return c64((size_t)t * t * 8) + 2 * c64((size_t)t * 8) + c64((size_t)s * 4) + c64((size_t)t * 4);
+}
+size_t child_pool_units(uint32_t s, uint32_t t) { return (size_t)(s - t) * ((s + kKsUnit - 1) / kKsUnit); }
+
+int run_leaves(cac_ctx* ctx, const std::vector<KsLeaf>& leaves, int buf, int64_t* d_results, hipStream_t s) {
+  const size_t n = leaves.size();
+  void* base = nullptr;
+  int rc = ks_reserve(ctx, buf, c64(n * sizeof(KsLeaf)) + c64(n * 4) + 128, &base);
+  if (rc) return rc;
+  Carve64 cv(base);
+  KsLeaf* d_leaves = cv.take<KsLeaf>(n);
+  uint32_t* d_list = cv.take<uint32_t>(n);
+  uint32_t* d_count = cv.take<uint32_t>(2);
+  CAC_HIP(ctx, hipMemcpyAsync(d_leaves, leaves.data(), n * sizeof(KsLeaf), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned long long*)d_count, 1u);
+  prof_begin(ctx, "ks_cert", s);
+  hipLaunchKernelGGL(k_ks_cert, dim3((unsigned)n), dim3(64), 0, s, (const KsLeaf*)d_leaves, d_results, d_list, d_count);
+  prof_end(ctx, s);
+  CAC_LAUNCH_CHECK(ctx, "ks certificate");
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_count, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  const uint32_t nsw = *(uint32_t*)ctx->pinned;
+  if (nsw) {
+    prof_begin(ctx, "ks_sw", s);
+    hipLaunchKernelGGL(k_ks_sw, dim3(nsw), dim3(64), 0, s, (const KsLeaf*)d_leaves, (const uint32_t*)d_list, d_results);
+    prof_end(ctx, s);
+    CAC_LAUNCH_CHECK(ctx, "ks stoer-wagner");
+  }
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  return CAC_OK;
+}
+
+template <int B>
+void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, KsPool pool, uint32_t seed,
+                     uint32_t* d_kout, int64_t* d_results, hipStream_t s) {
+  hipLaunchKernelGGL(k_ks_contract<B>, dim3(count), dim3(B), contract_lds_bytes(smax, B), s, d_children, pool, seed,
+                     d_kout, d_results);
+}
+
+}  // namespace
+
+int ks_row_sums(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_rsum, int64_t* d_min, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, s, d_min, 1u, (int64_t)INT64_MAX);
+  hipLaunchKernelGGL(k_row_sums, dim3(grid_for(n, 4, 2048)), dim3(256), 0, s, d_adj, n, d_rsum, d_min);
+  CAC_LAUNCH_CHECK(ctx, "row sums");
+  return CAC_OK;
+}
+
+size_t ks_level_bytes_per_root(uint32_t n) {
+  // children of every node at one depth with their pool rows (and 1/8 spare)
+  size_t best = 0;
+  uint64_t nodes = 1;
+  for (uint32_t sz = n; sz > kKsBase; sz = ks_target(sz), nodes *= 2) {
+    const uint32_t t = ks_target(sz);
+    const size_t b =
+        2 * nodes * (child_bytes(sz, t) + sizeof(KsChild) + 16 + child_pool_units(sz, t) * kKsUnit * 8 * 9 / 8);
+    best = std::max(best, b);
+  }
+  return best + 4096;
+}
+
+// Runs one Karger-Stein trial per root.  roots[i]: device matrix (n_i x n_i)
+// with its row sums, trial id, result slot and bound; d_results[slot] must hold
+// the bound already and receives min(bound, every leaf) by atomicMin.
 int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s) {
   if (roots.empty()) return CAC_OK;
   struct Node {
     const int64_t* mat;
+    const int64_t* rsum;
+    const int64_t* rmax;
     uint32_t s, trial, nid, slot;
     int64_t bound;
   };
-  std::vector<Node> level;
-  for (size_t i = 0; i < roots.size(); ++i)
-    level.push_back({roots[i].mat, roots[i].n, roots[i].trial, 1u, roots[i].slot, roots[i].bound});
+  static std::once_flag lds_attr;
+  hipError_t attr_err = hipSuccess;
+  std::call_once(lds_attr, [&] {
+    attr_err = hipFuncSetAttribute((const void*)k_ks_contract<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)contract_lds_bytes(kKsMaxS, 1024));
+    if (attr_err == hipSuccess)
+      attr_err = hipFuncSetAttribute((const void*)k_ks_contract<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)contract_lds_bytes(512 * kKsChunk, 512));
+  });
+  CAC_HIP(ctx, attr_err);
 
-  void* prev_arena = nullptr;  // buffers holding 'level' matrices (except the roots)
-  int rc = CAC_OK;
-  while (!level.empty() && rc == CAC_OK) {
+  std::vector<Node> level;
+  for (const KsRoot& r : roots) {
+    if (r.n > kKsMaxS) return fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
+    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, 1u, r.slot, r.bound});
+  }
+  int buf = 0;  // level buffer for the children of this level; the parents are in the other one
+  double spare = 0.125;
+  while (!level.empty()) {
     std::vector<KsLeaf> leaves;
     std::vector<Node> internal;
     for (auto& nd : level) {
-      if (nd.s <= kKsBase) leaves.push_back({nd.mat, nd.s, nd.slot, nd.bound});
+      if (nd.s <= kKsBase) leaves.push_back({nd.mat, nd.rsum, nd.rmax, nd.s, nd.slot, nd.bound});
       else internal.push_back(nd);
     }
     if (!leaves.empty()) {
-      KsLeaf* d_leaves = nullptr;
-      hipError_t e = hipMalloc((void**)&d_leaves, leaves.size() * sizeof(KsLeaf));
-      if (e != hipSuccess) { rc = hip_fail(ctx, e, "leaf descriptors"); break; }
-      CAC_HIP(ctx, hipMemcpyAsync(d_leaves, leaves.data(), leaves.size() * sizeof(KsLeaf), hipMemcpyHostToDevice, s));
-      prof_begin(ctx, "ks_leaf", s);
-      hipLaunchKernelGGL(k_ks_leaf, dim3((unsigned)leaves.size()), dim3(64), 0, s, d_leaves, d_results);
-      prof_end(ctx, s);
-      CAC_LAUNCH_CHECK(ctx, "ks leaf");
-      CAC_HIP(ctx, hipStreamSynchronize(s));
-      CAC_HIP(ctx, hipFree(d_leaves));
+      int rc = run_leaves(ctx, leaves, buf, d_results, s);
+      if (rc) return rc;
     }
     if (internal.empty()) break;
-    // children: 2 per internal node
-    size_t bytes = 0;
-    uint32_t smax = 0;
+    // children sorted by block-size class (1024, 256, 64), two per internal node
+    std::vector<std::pair<int, uint32_t>> order;
+    for (uint32_t i = 0; i < internal.size(); ++i) order.push_back({-contract_block(internal[i].s), i});
+    std::stable_sort(order.begin(), order.end());
+    const size_t nchild = 2 * internal.size();
+    size_t arrays = 0, units = 0, max_units_child = 0;
+    uint32_t max_tgt = 0;
     for (auto& nd : internal) {
       const uint32_t t = ks_target(nd.s);
-      bytes += 2 * (carve_size((size_t)nd.s * nd.s * 8) + carve_size((size_t)t * t * 8));
-      smax = std::max(smax, nd.s);
+      arrays += 2 * child_bytes(nd.s, t);
+      units += 2 * child_pool_units(nd.s, t);
+      max_units_child = std::max<size_t>(max_units_child, (nd.s + kKsUnit - 1) / kKsUnit * (size_t)nd.s);
+      max_tgt = std::max(max_tgt, t);
     }
-    if (smax > kKsMaxLdsS) { rc = fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices"); break; }
-    const size_t nchild = 2 * internal.size();
-    bytes += carve_size(nchild * sizeof(KsChild)) + carve_size(nchild * 4);
-    void* arena = nullptr;
-    hipError_t e = hipMalloc(&arena, bytes);
-    if (e != hipSuccess) { rc = hip_fail(ctx, e, "Karger-Stein level buffers"); break; }
-    Carver cv(arena);
-    KsChild* d_children = cv.take<KsChild>(nchild);
-    uint32_t* d_kout = cv.take<uint32_t>(nchild);
     std::vector<KsChild> ch(nchild);
-    for (size_t i = 0; i < internal.size(); ++i) {
-      const Node& nd = internal[i];
-      const uint32_t t = ks_target(nd.s);
-      for (uint32_t c = 0; c < 2; ++c) {
-        KsChild& x = ch[2 * i + c];
-        x.parent = nd.mat;
-        x.work = cv.take<int64_t>((size_t)nd.s * nd.s);
-        x.out = cv.take<int64_t>((size_t)t * t);
-        x.s = nd.s;
-        x.tgt = t;
-        x.trial = nd.trial;
-        x.nid = 2 * nd.nid + c;
-        x.slot = nd.slot;
-        x.pad = 0;
-        x.bound = nd.bound;
-      }
-    }
-    CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
-    static std::once_flag lds_attr;
-    hipError_t attr_err = hipSuccess;
-    std::call_once(lds_attr, [&] {
-      attr_err = hipFuncSetAttribute((const void*)k_ks_contract, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)contract_lds_bytes(kKsMaxLdsS));
-    });
-    CAC_HIP(ctx, attr_err);
-    prof_begin(ctx, "ks_contract", s);
-    hipLaunchKernelGGL(k_ks_contract, dim3((unsigned)nchild), dim3(kKsBlock), contract_lds_bytes(smax), s, d_children,
-                       seed, d_kout, d_results);
-    prof_end(ctx, s);
-    CAC_LAUNCH_CHECK(ctx, "ks contract");
+    std::vector<uint32_t> item_prefix(nchild + 1);
     std::vector<uint32_t> kout(nchild);
-    CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
-    CAC_HIP(ctx, hipStreamSynchronize(s));
-    if (prev_arena) CAC_HIP(ctx, hipFree(prev_arena));
-    prev_arena = arena;
+    for (int attempt = 0;; ++attempt) {
+      const size_t pool_units = units + std::max<size_t>((size_t)(spare * (double)units), max_units_child);
+      const size_t bytes = c64(nchild * sizeof(KsChild)) + c64((nchild + 1) * 4) + c64(nchild * 4) + 64 + arrays +
+                           pool_units * kKsUnit * 8 + 1024;
+      void* base = nullptr;
+      int rc = ks_reserve(ctx, buf, bytes, &base);
+      if (rc) return rc;
+      Carve64 cv(base);
+      KsChild* d_children = cv.take<KsChild>(nchild);
+      uint32_t* d_prefix = cv.take<uint32_t>(nchild + 1);
+      uint32_t* d_kout = cv.take<uint32_t>(nchild);
+      unsigned long long* d_used = cv.take<unsigned long long>(1);
+      uint32_t items = 0;
+      size_t ci = 0;
+      for (auto& o : order) {
+        const Node& nd = internal[o.second];
+        const uint32_t t = ks_target(nd.s);
+        for (uint32_t k = 0; k < 2; ++k, ++ci) {
+          KsChild& x = ch[ci];
+          x.parent = nd.mat;
+          x.prsum = nd.rsum;
+          x.out = cv.take<int64_t>((size_t)t * t);
+          x.orsum = cv.take<int64_t>(t);
+          x.ormax = cv.take<int64_t>(t);
+          x.colmap = cv.take<uint32_t>(nd.s);
+          x.live = cv.take<uint32_t>(t);
+          x.s = nd.s;
+          x.tgt = t;
+          x.trial = nd.trial;
+          x.nid = 2 * nd.nid + k;
+          x.slot = nd.slot;
+          x.pad = 0;
+          x.bound = nd.bound;
+          item_prefix[ci] = items;
+          items += (t + compact_rows(t) - 1) / compact_rows(t);
+        }
+      }
+      item_prefix[nchild] = items;
+      cv.off = (cv.off + 511) & ~size_t(511);
+      KsPool pool;
+      pool.base = (int64_t*)(cv.base + cv.off);
+      pool.used_units = d_used;
+      pool.cap_units = pool_units;
+      CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
+      CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, d_used, 1u);
+      prof_begin(ctx, "ks_contract", s);
+      size_t c0 = 0;
+      while (c0 < nchild) {
+        const int B = contract_block(ch[c0].s);
+        size_t c1 = c0;
+        uint32_t smax = 0;
+        while (c1 < nchild && contract_block(ch[c1].s) == B) smax = std::max(smax, ch[c1++].s);
+        const uint32_t cnt = (uint32_t)(c1 - c0);
+        if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+        else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+        else if (B == 256) launch_contract<256>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+        else if (B == 128) launch_contract<128>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+        else launch_contract<64>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+        c0 = c1;
+      }
+      prof_end(ctx, s);
+      CAC_LAUNCH_CHECK(ctx, "ks contract");
+      prof_begin(ctx, "ks_compact", s);
+      const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
+      hipLaunchKernelGGL(k_ks_compact, dim3(items), dim3(256), lds, s, (const KsChild*)d_children,
+                         (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+      prof_end(ctx, s);
+      CAC_LAUNCH_CHECK(ctx, "ks compact");
+      CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
+      CAC_HIP(ctx, hipStreamSynchronize(s));
+      bool overflow = false;
+      for (uint32_t k : kout) overflow = overflow || k == kKsOverflow;
+      if (!overflow) break;
+      if (attempt >= 6) return fail(ctx, CAC_ENOMEM, "Karger-Stein row pool keeps overflowing");
+      spare = spare * 4 + 0.25;  // PR passes merged more rows than expected: rerun the level
+    }
     std::vector<Node> next;
     for (size_t i = 0; i < nchild; ++i) {
-      if (kout[i] == 0xFFFFFFFFu) rc = fail(ctx, CAC_EDEVICE, "Karger-Stein selection out of range");
-      if (kout[i] == 0 || kout[i] == 0xFFFFFFFFu) continue;  // disconnected child: its 0 is already recorded
-      if (kout[i] & 0x80000000u) continue;                     // certified leaf: contributes the bound
-      next.push_back({ch[i].out, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound});
+      if (kout[i] == kKsBad) return fail(ctx, CAC_EDEVICE, "Karger-Stein selection out of range");
+      if (kout[i] == 0) continue;  // disconnected child: its 0 is already recorded
+      next.push_back({ch[i].out, ch[i].orsum, ch[i].ormax, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound});
     }
     level.swap(next);
+    buf ^= 1;
   }
   CAC_HIP(ctx, hipStreamSynchronize(s));
-  if (prev_arena) CAC_HIP(ctx, hipFree(prev_arena));
-  return rc;
+  return CAC_OK;
 }
 
 // Validates a host matrix: symmetric, zero diagonal, non-negative, no overflow.
@@ -587,51 +836,52 @@ int ks_check_matrix(cac_ctx* ctx, const int64_t* adj, uint32_t n) {
 int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, uint32_t first_trial,
                  uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut, hipStream_t s) {
   if (ntrials == 0) return fail(ctx, CAC_EINVAL, "ntrials must be >= 1");
-  int64_t *d_bound = nullptr, *d_res = nullptr;
-  CAC_HIP(ctx, hipMalloc((void**)&d_bound, 8 * (size_t)(ntrials + 1)));
-  CAC_HIP(ctx, hipMalloc((void**)&d_res, 8 * (size_t)ntrials));
-  int rc = ks_min_degree(ctx, d_adj, n, d_bound, s);
-  if (rc) return rc;
+  if (n > kKsMaxS) return fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
+  int64_t* d_small = nullptr;  // [0] bound, [1..n] row sums, then the trial results
+  CAC_HIP(ctx, hipMalloc((void**)&d_small, 8 * ((size_t)n + ntrials + 2)));
+  int64_t* d_bound = d_small;
+  int64_t* d_rsum = d_small + 1;
+  int64_t* d_res = d_small + 1 + n;
+  int rc = ks_row_sums(ctx, d_adj, n, d_rsum, d_bound, s);
   int64_t bound = 0;
-  CAC_HIP(ctx, hipMemcpyAsync(&bound, d_bound, 8, hipMemcpyDeviceToHost, s));
-  CAC_HIP(ctx, hipStreamSynchronize(s));
-  std::vector<int64_t> hb(ntrials, bound);
-  CAC_HIP(ctx, hipMemcpyAsync(d_bound, hb.data(), 8 * (size_t)ntrials, hipMemcpyHostToDevice, s));
-  CAC_HIP(ctx, hipMemcpyAsync(d_res, hb.data(), 8 * (size_t)ntrials, hipMemcpyHostToDevice, s));
-  // trials run in batches sized to the free device memory
-  size_t free_b = 0, total_b = 0;
-  CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-  size_t per_trial = 0;
-  {
-    uint64_t nodes = 1;
-    for (uint32_t sz = n; sz > kKsBase; sz = ks_target(sz), nodes *= 2) {
-      const uint64_t t = ks_target(sz);
-      per_trial = std::max<size_t>(per_trial, (size_t)(nodes * 2 * (sz * sz + t * t) * 8 + nodes * sz * sz * 8));
-    }
+  if (rc == CAC_OK) {
+    hipError_t e = hipMemcpyAsync(&bound, d_bound, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "row sums");
   }
-  size_t batch = per_trial ? std::max<size_t>(1, (size_t)(0.5 * (double)free_b) / per_trial) : ntrials;
-  batch = std::min<size_t>(batch, ntrials);
-  rc = CAC_OK;
-  for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
-    std::vector<KsRoot> roots;
-    for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
-      roots.push_back({d_adj, n, first_trial + t, t, bound});
-    rc = ks_run(ctx, roots, seed, d_res, s);
+  if (rc == CAC_OK) {
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ntrials, 256, 64)), dim3(256), 0, s, d_res, ntrials, bound);
+    // trials run in batches whose two live levels fit beside the free memory
+    size_t free_b = 0, total_b = 0;
+    hipError_t e = hipMemGetInfo(&free_b, &total_b);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "memory info");
+    const size_t avail = free_b + ctx->ks_bytes[0] + ctx->ks_bytes[1];
+    const size_t per_trial = 2 * ks_level_bytes_per_root(n);
+    size_t batch = std::max<size_t>(1, (size_t)(0.6 * (double)avail) / per_trial);
+    batch = std::min<size_t>(batch, ntrials);
+    const size_t nb = (ntrials + batch - 1) / batch;
+    batch = (ntrials + nb - 1) / nb;  // even batches
+    for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
+      std::vector<KsRoot> roots;
+      for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
+        roots.push_back({d_adj, d_rsum, n, first_trial + t, t, bound});
+      rc = ks_run(ctx, roots, seed, d_res, s);
+    }
   }
   if (rc == CAC_OK) {
     std::vector<int64_t> res(ntrials);
-    CAC_HIP(ctx, hipMemcpyAsync(res.data(), d_res, 8 * (size_t)ntrials, hipMemcpyDeviceToHost, s));
-    CAC_HIP(ctx, hipStreamSynchronize(s));
+    hipError_t e = hipMemcpyAsync(res.data(), d_res, 8 * (size_t)ntrials, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "trial results");
     int64_t mn = INT64_MAX;
-    for (uint32_t t = 0; t < ntrials; ++t) {
+    for (uint32_t t = 0; t < ntrials && rc == CAC_OK; ++t) {
       if (raw_cuts) raw_cuts[t] = res[t];
       mn = std::min(mn, res[t]);
     }
-    if (min_cut) *min_cut = mn;
+    if (rc == CAC_OK && min_cut) *min_cut = mn;
   }
-  CAC_HIP(ctx, hipStreamSynchronize(s));
-  (void)hipFree(d_bound);
-  (void)hipFree(d_res);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d_small);
   return rc;
 }
 

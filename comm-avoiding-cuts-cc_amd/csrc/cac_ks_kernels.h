@@ -3,17 +3,20 @@
 // One workgroup contracts one child of the recursion tree (indexed_adjacency_matrix /
 // lazy_adjacency_matrix semantics, src/karger-stein/indexed_adjacency_matrix.hpp:137-191,
 // lazy_adjacency_matrix.hpp:120-216):
-//   - the parent matrix is shared read-only by both children; a child writes only
-//     the rows it merges into (copy on write, tracked per row in LDS), so no copy
-//     of the s x s parent is ever made;
-//   - capacities (weighted degrees) and representatives live in LDS;
-//   - each pick: row i proportional to capacity, column k proportional to row i,
-//     j = rep[k], merge max(i,j) into min(i,j): rep relabel, row add with loop
-//     zeroing, fused capacity reduction;
-//   - an optional PR pass (co_mincut.h:50-63) and the rank compaction into the
-//     child's t x t output with LDS int64 atomics.
-// Leaves (<= 128 vertices) are solved by one wavefront each: a lower-bound
-// certificate first, Stoer-Wagner in LDS when the certificate does not settle it.
+//   - the parent matrix and its row sums are shared read-only by both children; a
+//     child writes only the rows it merges into, into rows taken from a level-wide
+//     pool (copy on write, the row's pool offset kept in LDS);
+//   - capacities live in LDS with 64-entry block sums, so the row pick is a
+//     two-level search by one wavefront;
+//   - every thread owns the same contiguous chunk of every row for the whole
+//     kernel: the picked row stays in registers for the column pick and the merge,
+//     and a thread only ever reads back what it wrote itself;
+//   - a PR pass (co_mincut.h:50-63) may follow; the kernel ends by writing the
+//     column map and the list of live rows, and a separate many-workgroup kernel
+//     compacts the live rows into the child's t x t matrix with its row sums and
+//     row maxima.
+// Leaves (<= 128 vertices) get a lower-bound certificate from those row sums and
+// maxima; Stoer-Wagner runs only on the leaves it does not settle.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,23 +26,43 @@
 
 namespace cac {
 
-constexpr int kKsBlock = 256;
-constexpr int kKsWaves = kKsBlock / 64;
-constexpr uint32_t kKsBase = 128;       // base_case_size (co_mincut.h:24)
-constexpr uint32_t kKsMaxLdsS = 8192;   // LDS holds cap + rep + row flags for s <= this
+constexpr uint32_t kKsBase = 128;        // base_case_size (co_mincut.h:24)
+constexpr uint32_t kKsMaxS = 8192;       // LDS holds caps, reps and row slots for s <= this
+constexpr uint32_t kKsChunk = 8;         // row entries per thread (s <= 8 * block size)
+constexpr uint32_t kKsUnit = 64;         // pool allocation unit: 64 int64 (512 bytes)
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kLiveWork = 0x80000000u;  // live[] entry: row taken from the pool
+constexpr uint32_t kKsBad = 0xFFFFFFFFu;       // k_out: selection out of range
+constexpr uint32_t kKsOverflow = 0xFFFFFFFEu;  // k_out: row pool exhausted, rerun the level
 
 struct KsChild {
   const int64_t* parent;  // s x s, row stride s
-  int64_t* work;          // s x s rows written on first merge
-  int64_t* out;           // compacted k x k (k <= tgt)
+  const int64_t* prsum;   // parent row sums [s]
+  int64_t* out;           // k x k compacted (k <= tgt), row stride k
+  int64_t* orsum;         // [tgt] row sums of out
+  int64_t* ormax;         // [tgt] row maxima of out
+  uint32_t* colmap;       // [s] column -> output index
+  uint32_t* live;         // [tgt] output row -> parent row, or kLiveWork | pool unit
   uint32_t s, tgt, trial, nid;
-  uint32_t slot;          // result slot of the trial
-  uint32_t pad;
+  uint32_t slot, pad;
   int64_t bound;
 };
 
+struct KsPool {
+  int64_t* base;                    // merged rows, kKsUnit-aligned
+  unsigned long long* used_units;   // device bump counter
+  unsigned long long cap_units;
+};
+
+// Compaction work item: rows [r0, r0 + nr) of child c (rows >= k are skipped).
+struct KsCompactItem {
+  uint32_t child, r0, nr, pad;
+};
+
 struct KsLeaf {
-  const int64_t* mat;  // k x k, row stride k
+  const int64_t* mat;   // k x k, row stride k
+  const int64_t* rsum;  // [k] row sums, or null
+  const int64_t* rmax;  // [k] row maxima, or null
   uint32_t k, slot;
   int64_t bound;
 };
@@ -55,115 +78,100 @@ __device__ inline uint32_t wave_min_u32(uint32_t x) {
   for (int d = 32; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, 64));
   return x;
 }
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
 
-// Block sum; every thread gets the total.  lds: >= kKsWaves u64.
+// Block sum over B threads; every thread gets the total.  lds: >= B/64 u64.
+template <int B>
 __device__ inline uint64_t block_sum_u64(uint64_t x, uint64_t* lds) {
   x = wave_sum_u64(x);
+  if (B == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
   uint64_t t = 0;
 #pragma unroll
-  for (int w = 0; w < kKsWaves; ++w) t += lds[w];
+  for (int w = 0; w < B / 64; ++w) t += lds[w];
   __syncthreads();
   return t;
 }
+template <int B>
 __device__ inline uint32_t block_min_u32(uint32_t x, uint32_t* lds) {
   x = wave_min_u32(x);
+  if (B == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
   uint32_t t = 0xFFFFFFFFu;
 #pragma unroll
-  for (int w = 0; w < kKsWaves; ++w) t = min(t, lds[w]);
+  for (int w = 0; w < B / 64; ++w) t = min(t, lds[w]);
   __syncthreads();
   return t;
 }
 
-// First index i in [0, s) whose inclusive prefix of get(.) reaches r (1 <= r <= total).
-// Wave w owns a contiguous segment; segment totals locate the wave, which then
-// scans its segment 64 elements at a time.  lds_u64: >= kKsWaves; lds_idx: 1 u32.
-template <class Get>
-__device__ inline uint32_t block_select(Get get, uint32_t s, uint64_t r, uint64_t* lds_u64, uint32_t* lds_idx) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) *lds_idx = 0xFFFFFFFFu;  // stays when r exceeds the total
-  const uint32_t seg = ((s + kKsWaves - 1) / kKsWaves + 63) & ~63u;
-  const uint32_t lo = min(s, (uint32_t)wave * seg), hi = min(s, lo + seg);
-  uint64_t part = 0;
-  for (uint32_t x = lo + lane; x < hi; x += 64) part += (uint64_t)get(x);
-  part = wave_sum_u64(part);
-  if (lane == 0) lds_u64[wave] = part;
-  __syncthreads();
-  uint64_t before = 0;
-  int owner = -1;
-  for (int w = 0; w < kKsWaves; ++w) {
-    uint64_t v = lds_u64[w];
-    if (owner < 0 && before + v >= r) owner = w;
-    else if (owner < 0) before += v;
+// First index i in [0, s) whose inclusive prefix of cap reaches r (1 <= r <= total),
+// by one wavefront: 64-entry block sums first (lane l owns blocks 2l, 2l + 1),
+// then the 64 entries of the chosen block.  kNoSlot when r exceeds the total.
+__device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint64_t* cbs, uint32_t s, uint32_t nb,
+                                                 uint64_t r) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t b0 = 2 * lane, b1 = b0 + 1;
+  const uint64_t c0 = b0 < nb ? cbs[b0] : 0, c1 = b1 < nb ? cbs[b1] : 0;
+  const uint64_t incl = wave_incl_scan_u64(c0 + c1, lane);
+  const uint64_t hit = __ballot(incl >= r);
+  if (!hit) return kNoSlot;
+  const int owner = __ffsll((long long)hit) - 1;
+  const uint64_t before = __shfl(incl - c0 - c1, owner, 64);
+  const uint64_t oc0 = __shfl(c0, owner, 64);
+  uint32_t blk = 2 * owner;
+  uint64_t rr = r - before;
+  if (rr > oc0) {
+    rr -= oc0;
+    blk += 1;
   }
-  if (wave == owner) {
-    uint64_t carry = before;
-    uint32_t found = 0xFFFFFFFFu;
-    for (uint32_t base = lo; base < hi; base += 64) {
-      uint32_t x = base + lane;
-      uint64_t v = x < hi ? (uint64_t)get(x) : 0;
-      uint64_t inc = v;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        uint64_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-      }
-      uint64_t hit = __ballot(x < hi && carry + inc >= r);
-      if (hit) {
-        found = base + (uint32_t)(__ffsll((long long)hit) - 1);
-        break;
-      }
-      carry += __shfl(inc, 63, 64);
-    }
-    if (lane == 0) *lds_idx = found;
-  }
-  __syncthreads();
-  uint32_t res = *lds_idx;
-  __syncthreads();
-  return res;
+  const uint32_t x = blk * 64 + lane;
+  const uint64_t v = x < s ? (uint64_t)cap[x] : 0;
+  const uint64_t inc2 = wave_incl_scan_u64(v, lane);
+  const uint64_t hit2 = __ballot(x < s && inc2 >= rr);
+  if (!hit2) return kNoSlot;
+  return blk * 64 + (uint32_t)(__ffsll((long long)hit2) - 1);
 }
 
-// Same contract as block_select for a row in global memory: thread t owns the
-// contiguous chunk [t*c, (t+1)*c), c = ceil(s / kKsBlock) <= kKsMaxChunk, and keeps
-// it in registers, so one round of loads, one block scan and an in-register scan
-// by the owning thread replace the serial wave walk.  lds_u64: >= kKsWaves + 1.
-constexpr uint32_t kKsMaxChunk = kKsMaxLdsS / kKsBlock;
-__device__ inline uint32_t block_select_row(const int64_t* row, uint32_t s, uint64_t r, uint64_t* lds_u64,
-                                            uint32_t* lds_idx) {
+// First index of a row whose inclusive prefix reaches r, with the row held in
+// registers: thread t owns entries [t*cc, t*cc + cc).  One block scan of the
+// chunk sums, then the owning thread walks its chunk.  kNoSlot if none.
+template <int B>
+__device__ inline uint32_t block_select_chunk(const int64_t (&v)[kKsChunk], uint32_t cc, uint32_t s, uint64_t r,
+                                              uint64_t* lds_u64, uint32_t* lds_idx) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t c = (s + kKsBlock - 1) / kKsBlock;
-  const uint32_t lo = threadIdx.x * c;
-  if (threadIdx.x == 0) *lds_idx = 0xFFFFFFFFu;
-  uint64_t v[kKsMaxChunk];
+  const uint32_t lo = threadIdx.x * cc;
+  if (threadIdx.x == 0) *lds_idx = kNoSlot;
   uint64_t sum = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kKsMaxChunk; ++k) {
-    const uint32_t x = lo + k;
-    v[k] = (k < c && x < s) ? (uint64_t)row[x] : 0ull;
-    sum += v[k];
+  for (uint32_t k = 0; k < kKsChunk; ++k)
+    if (k < cc && lo + k < s) sum += (uint64_t)v[k];
+  const uint64_t inc = wave_incl_scan_u64(sum, lane);
+  if (B > 64) {
+    if (lane == 63) lds_u64[wave] = inc;
   }
-  // block exclusive scan of the chunk sums
-  uint64_t inc = sum;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint64_t y = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += y;
-  }
-  if (lane == 63) lds_u64[wave] = inc;
   __syncthreads();
   uint64_t before = 0;
-  for (int w = 0; w < wave; ++w) before += lds_u64[w];
+  if (B > 64)
+    for (int w = 0; w < wave; ++w) before += lds_u64[w];
   const uint64_t ex = before + inc - sum;
   if (sum != 0 && ex < r && r <= ex + sum) {
     uint64_t acc = ex;
-    uint32_t found = 0xFFFFFFFFu;
+    uint32_t found = kNoSlot;
 #pragma unroll
-    for (uint32_t k = 0; k < kKsMaxChunk; ++k) {
-      acc += v[k];
-      if (found == 0xFFFFFFFFu && v[k] != 0 && acc >= r) found = lo + k;
+    for (uint32_t k = 0; k < kKsChunk; ++k) {
+      if (k < cc && lo + k < s) {
+        acc += (uint64_t)v[k];
+        if (found == kNoSlot && v[k] != 0 && acc >= r) found = lo + k;
+      }
     }
     *lds_idx = found;
   }

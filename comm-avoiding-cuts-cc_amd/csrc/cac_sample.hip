@@ -46,6 +46,7 @@ struct TrialScratch {
   uint64_t* pre;          // [m]
   uint2* draws;           // [n]
   int64_t* mat;           // [t x t]
+  int64_t* rsum;          // [t] row sums of mat (may be null)
   uint32_t* nv_out;       // [1]
 };
 
@@ -244,12 +245,13 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
     atomicAdd((unsigned long long*)&S.mat[(uint64_t)b * k + a], (unsigned long long)x.w);
   }
   block_sync_global();
-  // bound = min row sum: one wave per row
+  // row sums (the Karger-Stein capacities) and bound = min row sum: one wave per row
   uint64_t best = ~0ull;
   for (uint32_t r = threadIdx.x >> 6; r < k; r += kSmpWaves) {
     uint64_t acc = 0;
     for (uint32_t x = threadIdx.x & 63; x < k; x += 64) acc += (uint64_t)S.mat[(uint64_t)r * k + x];
     acc = wsum64(acc);
+    if (S.rsum && (threadIdx.x & 63) == 0) S.rsum[r] = (int64_t)acc;
     best = acc < best ? acc : best;
   }
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = best;
@@ -296,12 +298,12 @@ uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p) {
 // Device bytes one sampling job needs beside the Karger-Stein levels.
 size_t job_sample_bytes(const SqrtJob& j) {
   return carve_size((size_t)j.n * 4) * 2 + carve_size(std::max<uint64_t>(j.m, 1) * 8) + carve_size((size_t)j.n * 8) +
-         carve_size((size_t)j.t * j.t * 8) + 256;
+         carve_size((size_t)j.t * j.t * 8) + carve_size((size_t)j.t * 8) + 256;
 }
 // Upper estimate of the Karger-Stein level arenas of one root of size t: each
 // level holds about t^2 entries in copy-on-write work rows plus the contracted
 // children, and two levels are alive at once.
-size_t job_ks_bytes(const SqrtJob& j) { return (size_t)j.t * j.t * 8 * 8 + 4096; }
+size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t); }
 
 // Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
 // one sampling launch, one densify launch and one breadth-first Karger-Stein
@@ -350,6 +352,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       x.pre = cv.take<uint64_t>(std::max<uint64_t>(j.m, 1));
       x.draws = cv.take<uint2>(j.n);
       x.mat = cv.take<int64_t>((size_t)j.t * j.t);
+      x.rsum = cv.take<int64_t>(j.t);
       x.nv_out = d_nv + i;
     }
     e = hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
@@ -376,7 +379,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     if (rc == CAC_OK) {
       std::vector<KsRoot> roots;
       for (size_t i = 0; i < nb; ++i)
-        if (nvs[i] >= 2 && nvs[i] <= scr[i].t) roots.push_back({scr[i].mat, nvs[i], scr[i].trial, scr[i].slot, hb[i]});
+        if (nvs[i] >= 2 && nvs[i] <= scr[i].t)
+          roots.push_back({scr[i].mat, scr[i].rsum, nvs[i], scr[i].trial, scr[i].slot, hb[i]});
       rc = ks_run(ctx, roots, seed, d_results, s);
     }
     (void)hipStreamSynchronize(s);
@@ -422,6 +426,7 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   h.nv_out = cv.take<uint32_t>(1);
   int64_t* d_misc = cv.take<int64_t>(2);
   h.mat = nullptr;
+  h.rsum = nullptr;
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
   CAC_LAUNCH_CHECK(ctx, "sample trial");

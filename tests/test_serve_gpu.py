@@ -49,11 +49,27 @@ def test_served_results_equal_alone(ctx):
             raw = ctx.sqrt_cut_trials(e, n, t, 5, 0, T)
             assert res[i]["t"] == t and res[i]["trials"] == T
             assert res[i]["cut"] == raw.min(), i
-    assert res["batch_size"].max() > 1  # launch groups formed
     assert np.all(res["admit_s"] >= res["submit_s"]) and np.all(res["done_s"] >= res["admit_s"])
+
+
+def test_burst_forms_groups_that_overlap(ctx):
+    """A burst arriving at once: queries waiting behind a running group are
+    admitted together as launch groups, and groups on the two workers overlap."""
+    rng = np.random.default_rng(7)
+    qs = _stream(rng, 24)
+    _say("serve burst of 24 queries")
+    with cac.Server(0, workers=2, hbm_budget=8 << 30, max_batch=8) as srv:
+        recs = [cac.Server.query(k, e, n, F32_95, 5, lab) for (k, n, e, lab) in qs]
+        res = srv.replay(recs, np.zeros(len(qs)))
+    assert np.all(res["status"] == 0)
+    for i, (k, n, e, lab) in enumerate(qs):
+        if k == cac.QUERY_CC:
+            want, nc = ctx.cc(np.stack([e["u"], e["v"]], 1), n)
+            assert res[i]["ncomp"] == nc and np.array_equal(lab, want), i
+    assert res["batch_size"].max() > 1  # launch groups formed
     order = np.argsort(res["admit_s"])
     overlap = any(res[order[j + 1]]["admit_s"] < res[order[j]]["done_s"] for j in range(len(order) - 1))
-    assert overlap  # later queries admitted while earlier ones ran
+    assert overlap  # a group was admitted while another ran
 
 
 def test_bad_query_fails_alone(ctx):
