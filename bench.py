@@ -157,7 +157,7 @@ def bench_c3(ctx, cac, trials: int):
     n = 8192
     d_adj = ctx.alloc(n * n * 8)
     ctx.gen_complete_dev(n, 0, 42, d_adj.ptr)
-    ctx.ks_dense_dev(d_adj.ptr, n, 0, 1000, 2)  # warm-up (other trial ids)
+    ctx.ks_dense_dev(d_adj.ptr, n, 0, 1 << 20, trials)  # warm-up: same size, other trial ids
     ctx.synchronize()
     t0 = time.perf_counter()
     raw, cut = ctx.ks_dense_dev(d_adj.ptr, n, 0, 0, trials)
@@ -194,7 +194,7 @@ def bench_c4(ctx, cac, count: int, stride: int):
     buf.upload(edges)
     m_total = len(edges)
     del edges
-    ctx.mincut_batch_dev(buf.ptr, descs[:32].copy(), F32_95, 0)  # warm-up
+    ctx.mincut_batch_dev(buf.ptr, descs, F32_95, 1)  # warm-up at full size (another seed)
     t0 = time.perf_counter()
     cuts, trials = ctx.mincut_batch_dev(buf.ptr, descs, F32_95, 0)
     dt = time.perf_counter() - t0

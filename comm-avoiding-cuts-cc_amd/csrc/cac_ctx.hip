@@ -81,6 +81,23 @@ __global__ void __launch_bounds__(256) k_copy(uint4* __restrict__ dst, const uin
   if (blockIdx.x == 0 && threadIdx.x < tail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
+int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes) {
+  if (bytes <= *cap && *buf) return CAC_OK;
+  if (*buf) CAC_HIP(ctx, hipFree(*buf));
+  *buf = nullptr;
+  *cap = 0;
+  CAC_HIP(ctx, hipMalloc(buf, bytes));
+  *cap = bytes;
+  return CAC_OK;
+}
+
+int device_memory_available(cac_ctx* ctx, size_t* bytes) {
+  size_t free_b = 0, total_b = 0;
+  CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+  *bytes = free_b + ctx->ks_bytes[0] + ctx->ks_bytes[1] + ctx->smp_bytes + ctx->ks_retry_bytes;
+  return CAC_OK;
+}
+
 }  // namespace cac
 
 extern "C" {
@@ -116,6 +133,8 @@ int cac_ctx_destroy(cac_ctx* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   for (int b = 0; b < 2; ++b)
     if (ctx->ks_buf[b]) (void)hipFree(ctx->ks_buf[b]);
+  if (ctx->smp_buf) (void)hipFree(ctx->smp_buf);
+  if (ctx->ks_retry_buf) (void)hipFree(ctx->ks_retry_buf);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

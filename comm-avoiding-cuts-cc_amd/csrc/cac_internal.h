@@ -30,6 +30,11 @@ struct cac_ctx {
   // in one, children in the other).
   void* ks_buf[2] = {nullptr, nullptr};
   size_t ks_bytes[2] = {0, 0};
+  void* ks_retry_buf = nullptr;  // children redone with a full-size row pool
+  size_t ks_retry_bytes = 0;
+  // Grow-only buffer for the per-trial scratch of the sampling contraction.
+  void* smp_buf = nullptr;
+  size_t smp_bytes = 0;
 };
 
 namespace cac {
@@ -43,6 +48,12 @@ int bind(cac_ctx* ctx);
 
 // Ensures the scratch arena holds at least 'bytes'; returns its base.
 int scratch_reserve(cac_ctx* ctx, size_t bytes, void** base);
+
+// Grow-only device buffer (*buf, *cap): reallocated only when 'bytes' exceeds it.
+int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes);
+
+// Free device memory plus what the context's grow-only buffers already hold.
+int device_memory_available(cac_ctx* ctx, size_t* bytes);
 
 inline hipStream_t pick_stream(cac_ctx* ctx, void* s) {
   return s ? (hipStream_t)s : ctx->stream;

@@ -272,9 +272,10 @@ __device__ inline void load_chunk(const int64_t* row, uint32_t lo, uint32_t cc, 
 
 // Merge row a into row b (a dies, b keeps its index).  The calling threads hold
 // their chunk of row `held` (a or b) in v; the other row is loaded here.  The
-// merged row goes to b's pool row (taken now if b has none).  Returns the
-// contracted weight (cap[a] + cap[b] - cap_new) / 2 in w; false when the pool is
-// exhausted.  Called by the whole block.
+// merged row goes to b's pool row; if b has none it takes a's (a dies, and every
+// thread rewrites exactly the entries it read), and only when neither has one a
+// new pool row.  Returns the contracted weight (cap[a] + cap[b] - cap_new) / 2
+// in w; false when the pool is exhausted.  Called by the whole block.
 template <int B>
 __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, uint32_t a, uint32_t b, uint32_t held,
                          const int64_t (&v)[kKsChunk], uint32_t cc, int64_t& w) {
@@ -284,7 +285,11 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
     L.misc[2] = L.wslot[other];
     uint32_t db = L.wslot[b];
     uint32_t overflow = 0;
-    if (db == kNoSlot) {
+    if (db == kNoSlot && L.wslot[a] != kNoSlot) {
+      db = L.wslot[a];
+      L.wslot[b] = db;
+      L.wslot[a] = kNoSlot;
+    } else if (db == kNoSlot) {
       const unsigned long long units = (s + kKsUnit - 1) / kKsUnit;
       const unsigned long long off = atomicAdd(pool.used_units, units);
       if (off + units > pool.cap_units) {
@@ -594,15 +599,9 @@ inline uint32_t ks_target(uint32_t s) { return ks_target_host(s); }
 
 // Grow-only level buffer `which` of the context.
 int ks_reserve(cac_ctx* ctx, int which, size_t bytes, void** out) {
-  if (bytes > ctx->ks_bytes[which]) {
-    if (ctx->ks_buf[which]) CAC_HIP(ctx, hipFree(ctx->ks_buf[which]));
-    ctx->ks_buf[which] = nullptr;
-    ctx->ks_bytes[which] = 0;
-    CAC_HIP(ctx, hipMalloc(&ctx->ks_buf[which], bytes));
-    ctx->ks_bytes[which] = bytes;
-  }
+  int rc = grow_buffer(ctx, &ctx->ks_buf[which], &ctx->ks_bytes[which], bytes);
   *out = ctx->ks_buf[which];
-  return CAC_OK;
+  return rc;
 }
 
 // Device bytes of the per-child arrays of one child (out, sums, maxima, map, list).
@@ -644,6 +643,51 @@ void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, K
                      uint32_t* d_kout, int64_t* d_results, hipStream_t s) {
   hipLaunchKernelGGL(k_ks_contract<B>, dim3(count), dim3(B), contract_lds_bytes(smax, B), s, d_children, pool, seed,
                      d_kout, d_results);
+}
+
+// One level's launches for the children ch (already in d_children's order, by
+// block-size class): contraction per class, then compaction; waits and returns
+// the children's k_out.
+int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_children, uint32_t* d_prefix,
+                   uint32_t* d_kout, unsigned long long* d_used, KsPool pool, uint32_t max_tgt, uint32_t seed,
+                   int64_t* d_results, std::vector<uint32_t>& kout, hipStream_t s) {
+  const size_t nchild = ch.size();
+  std::vector<uint32_t> item_prefix(nchild + 1);
+  uint32_t items = 0;
+  for (size_t i = 0; i < nchild; ++i) {
+    item_prefix[i] = items;
+    items += (ch[i].tgt + compact_rows(ch[i].tgt) - 1) / compact_rows(ch[i].tgt);
+  }
+  item_prefix[nchild] = items;
+  CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
+  CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, d_used, 1u);
+  prof_begin(ctx, "ks_contract", s);
+  size_t c0 = 0;
+  while (c0 < nchild) {
+    const int B = contract_block(ch[c0].s);
+    size_t c1 = c0;
+    uint32_t smax = 0;
+    while (c1 < nchild && contract_block(ch[c1].s) == B) smax = std::max(smax, ch[c1++].s);
+    const uint32_t cnt = (uint32_t)(c1 - c0);
+    if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    else if (B == 256) launch_contract<256>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    else if (B == 128) launch_contract<128>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    else launch_contract<64>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    c0 = c1;
+  }
+  prof_end(ctx, s);
+  CAC_LAUNCH_CHECK(ctx, "ks contract");
+  prof_begin(ctx, "ks_compact", s);
+  const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
+  hipLaunchKernelGGL(k_ks_compact, dim3(items), dim3(256), lds, s, (const KsChild*)d_children, (const uint32_t*)d_kout,
+                     (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+  prof_end(ctx, s);
+  CAC_LAUNCH_CHECK(ctx, "ks compact");
+  CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  return CAC_OK;
 }
 
 }  // namespace
@@ -697,7 +741,6 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, 1u, r.slot, r.bound});
   }
   int buf = 0;  // level buffer for the children of this level; the parents are in the other one
-  double spare = 0.125;
   while (!level.empty()) {
     std::vector<KsLeaf> leaves;
     std::vector<Node> internal;
@@ -727,82 +770,77 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     std::vector<KsChild> ch(nchild);
     std::vector<uint32_t> item_prefix(nchild + 1);
     std::vector<uint32_t> kout(nchild);
-    for (int attempt = 0;; ++attempt) {
-      const size_t pool_units = units + std::max<size_t>((size_t)(spare * (double)units), max_units_child);
-      const size_t bytes = c64(nchild * sizeof(KsChild)) + c64((nchild + 1) * 4) + c64(nchild * 4) + 64 + arrays +
-                           pool_units * kKsUnit * 8 + 1024;
-      void* base = nullptr;
-      int rc = ks_reserve(ctx, buf, bytes, &base);
+    // first pass: pool rows for the picks with 1/8 spare; children whose PR pass
+    // needs more come back as kKsOverflow and are redone below
+    const size_t pool_units = units + std::max<size_t>(units / 8, max_units_child);
+    if (pool_units >= kLiveWork) return fail(ctx, CAC_ENOMEM, "Karger-Stein level too large for one pass");
+    const size_t bytes = c64(nchild * sizeof(KsChild)) + c64((nchild + 1) * 4) + c64(nchild * 4) + 64 + arrays +
+                         pool_units * kKsUnit * 8 + 1024;
+    void* base = nullptr;
+    int rc = ks_reserve(ctx, buf, bytes, &base);
+    if (rc) return rc;
+    Carve64 cv(base);
+    KsChild* d_children = cv.take<KsChild>(nchild);
+    uint32_t* d_prefix = cv.take<uint32_t>(nchild + 1);
+    uint32_t* d_kout = cv.take<uint32_t>(nchild);
+    unsigned long long* d_used = cv.take<unsigned long long>(1);
+    size_t ci = 0;
+    for (auto& o : order) {
+      const Node& nd = internal[o.second];
+      const uint32_t t = ks_target(nd.s);
+      for (uint32_t k = 0; k < 2; ++k, ++ci) {
+        KsChild& x = ch[ci];
+        x.parent = nd.mat;
+        x.prsum = nd.rsum;
+        x.out = cv.take<int64_t>((size_t)t * t);
+        x.orsum = cv.take<int64_t>(t);
+        x.ormax = cv.take<int64_t>(t);
+        x.colmap = cv.take<uint32_t>(nd.s);
+        x.live = cv.take<uint32_t>(t);
+        x.s = nd.s;
+        x.tgt = t;
+        x.trial = nd.trial;
+        x.nid = 2 * nd.nid + k;
+        x.slot = nd.slot;
+        x.pad = 0;
+        x.bound = nd.bound;
+      }
+    }
+    cv.off = (cv.off + 511) & ~size_t(511);
+    KsPool pool{(int64_t*)(cv.base + cv.off), d_used, pool_units};
+    rc = contract_level(ctx, ch, d_children, d_prefix, d_kout, d_used, pool, max_tgt, seed, d_results, kout, s);
+    if (rc) return rc;
+    // children that ran out of pool rows: again, with s pool rows each (a child
+    // merges at most s - 2 times, so this cannot overflow)
+    std::vector<uint32_t> over;
+    for (uint32_t i = 0; i < nchild; ++i)
+      if (kout[i] == kKsOverflow) over.push_back(i);
+    if (!over.empty()) {
+      std::vector<KsChild> rch;
+      size_t runits = 0;
+      for (uint32_t i : over) {
+        rch.push_back(ch[i]);
+        runits += (size_t)ch[i].s * ((ch[i].s + kKsUnit - 1) / kKsUnit);
+      }
+      const size_t no = rch.size();
+      if (runits >= kLiveWork) return fail(ctx, CAC_ENOMEM, "Karger-Stein level too large for one pass");
+      const size_t rbytes = c64(no * sizeof(KsChild)) + c64((no + 1) * 4) + c64(no * 4) + 64 + runits * kKsUnit * 8 + 1024;
+      rc = grow_buffer(ctx, &ctx->ks_retry_buf, &ctx->ks_retry_bytes, rbytes);
       if (rc) return rc;
-      Carve64 cv(base);
-      KsChild* d_children = cv.take<KsChild>(nchild);
-      uint32_t* d_prefix = cv.take<uint32_t>(nchild + 1);
-      uint32_t* d_kout = cv.take<uint32_t>(nchild);
-      unsigned long long* d_used = cv.take<unsigned long long>(1);
-      uint32_t items = 0;
-      size_t ci = 0;
-      for (auto& o : order) {
-        const Node& nd = internal[o.second];
-        const uint32_t t = ks_target(nd.s);
-        for (uint32_t k = 0; k < 2; ++k, ++ci) {
-          KsChild& x = ch[ci];
-          x.parent = nd.mat;
-          x.prsum = nd.rsum;
-          x.out = cv.take<int64_t>((size_t)t * t);
-          x.orsum = cv.take<int64_t>(t);
-          x.ormax = cv.take<int64_t>(t);
-          x.colmap = cv.take<uint32_t>(nd.s);
-          x.live = cv.take<uint32_t>(t);
-          x.s = nd.s;
-          x.tgt = t;
-          x.trial = nd.trial;
-          x.nid = 2 * nd.nid + k;
-          x.slot = nd.slot;
-          x.pad = 0;
-          x.bound = nd.bound;
-          item_prefix[ci] = items;
-          items += (t + compact_rows(t) - 1) / compact_rows(t);
-        }
+      Carve64 rv(ctx->ks_retry_buf);
+      KsChild* d_rch = rv.take<KsChild>(no);
+      uint32_t* d_rprefix = rv.take<uint32_t>(no + 1);
+      uint32_t* d_rkout = rv.take<uint32_t>(no);
+      unsigned long long* d_rused = rv.take<unsigned long long>(1);
+      rv.off = (rv.off + 511) & ~size_t(511);
+      KsPool rpool{(int64_t*)(rv.base + rv.off), d_rused, runits};
+      std::vector<uint32_t> rkout(no);
+      rc = contract_level(ctx, rch, d_rch, d_rprefix, d_rkout, d_rused, rpool, max_tgt, seed, d_results, rkout, s);
+      if (rc) return rc;
+      for (size_t k = 0; k < no; ++k) {
+        if (rkout[k] == kKsOverflow) return fail(ctx, CAC_EDEVICE, "Karger-Stein row pool overflow");
+        kout[over[k]] = rkout[k];
       }
-      item_prefix[nchild] = items;
-      cv.off = (cv.off + 511) & ~size_t(511);
-      KsPool pool;
-      pool.base = (int64_t*)(cv.base + cv.off);
-      pool.used_units = d_used;
-      pool.cap_units = pool_units;
-      CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
-      CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, d_used, 1u);
-      prof_begin(ctx, "ks_contract", s);
-      size_t c0 = 0;
-      while (c0 < nchild) {
-        const int B = contract_block(ch[c0].s);
-        size_t c1 = c0;
-        uint32_t smax = 0;
-        while (c1 < nchild && contract_block(ch[c1].s) == B) smax = std::max(smax, ch[c1++].s);
-        const uint32_t cnt = (uint32_t)(c1 - c0);
-        if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-        else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-        else if (B == 256) launch_contract<256>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-        else if (B == 128) launch_contract<128>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-        else launch_contract<64>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-        c0 = c1;
-      }
-      prof_end(ctx, s);
-      CAC_LAUNCH_CHECK(ctx, "ks contract");
-      prof_begin(ctx, "ks_compact", s);
-      const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
-      hipLaunchKernelGGL(k_ks_compact, dim3(items), dim3(256), lds, s, (const KsChild*)d_children,
-                         (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
-      prof_end(ctx, s);
-      CAC_LAUNCH_CHECK(ctx, "ks compact");
-      CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
-      CAC_HIP(ctx, hipStreamSynchronize(s));
-      bool overflow = false;
-      for (uint32_t k : kout) overflow = overflow || k == kKsOverflow;
-      if (!overflow) break;
-      if (attempt >= 6) return fail(ctx, CAC_ENOMEM, "Karger-Stein row pool keeps overflowing");
-      spare = spare * 4 + 0.25;  // PR passes merged more rows than expected: rerun the level
     }
     std::vector<Node> next;
     for (size_t i = 0; i < nchild; ++i) {
@@ -852,10 +890,8 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
   if (rc == CAC_OK) {
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ntrials, 256, 64)), dim3(256), 0, s, d_res, ntrials, bound);
     // trials run in batches whose two live levels fit beside the free memory
-    size_t free_b = 0, total_b = 0;
-    hipError_t e = hipMemGetInfo(&free_b, &total_b);
-    if (e != hipSuccess) rc = hip_fail(ctx, e, "memory info");
-    const size_t avail = free_b + ctx->ks_bytes[0] + ctx->ks_bytes[1];
+    size_t avail = 0;
+    rc = device_memory_available(ctx, &avail);
     const size_t per_trial = 2 * ks_level_bytes_per_root(n);
     size_t batch = std::max<size_t>(1, (size_t)(0.6 * (double)avail) / per_trial);
     batch = std::min<size_t>(batch, ntrials);

@@ -313,9 +313,10 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
                    size_t budget) {
   if (jobs.empty()) return CAC_OK;
   if (budget == 0) {
-    size_t free_b = 0, total_b = 0;
-    CAC_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
-    budget = std::max<size_t>((size_t)(0.4 * (double)free_b), (size_t)1 << 28);
+    size_t avail = 0;
+    int rc0 = device_memory_available(ctx, &avail);
+    if (rc0) return rc0;
+    budget = std::max<size_t>((size_t)(0.45 * (double)avail), (size_t)1 << 28);
   }
   int rc = CAC_OK;
   size_t b0 = 0;
@@ -330,10 +331,10 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       ++nb;
     }
     const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) + 1024;
-    void* arena = nullptr;
-    hipError_t e = hipMalloc(&arena, bytes);
-    if (e != hipSuccess) return hip_fail(ctx, e, "sqrt-cut job scratch");
-    Carver cv(arena);
+    rc = grow_buffer(ctx, &ctx->smp_buf, &ctx->smp_bytes, bytes);
+    if (rc) return rc;
+    Carver cv(ctx->smp_buf);
+    hipError_t e = hipSuccess;
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
     uint32_t* d_nv = cv.take<uint32_t>(nb);
     int64_t* d_bounds = cv.take<int64_t>(nb);
@@ -384,7 +385,6 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       rc = ks_run(ctx, roots, seed, d_results, s);
     }
     (void)hipStreamSynchronize(s);
-    (void)hipFree(arena);
     b0 += nb;
   }
   return rc;
