@@ -69,6 +69,10 @@ SIGNATURES = {
     "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
     "cac_ks_dense": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64]),
     "cac_ks_dense_dev": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64, _vp]),
+    "cac_ks_sparse": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _pi64, _pi64]),
+    "cac_karger_stein": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
+    "cac_race_key": (_u64, [_u64, _u64]),
+    "cac_sparse_contract": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_ks_trials": (_u32, [_u32, ctypes.c_double]),
     "cac_intermediate_size": (_u32, [_u32, _u64]),
     "cac_sqrt_trials": (_u32, [_u32, _u64, ctypes.c_double]),
@@ -245,6 +249,35 @@ class Context:
                                               raw.ctypes.data_as(_pi64), ctypes.byref(mn), stream))
         return raw, int(mn.value)
 
+    # ---- sparse Karger-Stein ----------------------------------------------
+    def ks_sparse(self, edges, n: int, seed: int, first_trial: int, ntrials: int):
+        """Sparse recursion on an edge list.  Returns (raw cut per trial, min cut)."""
+        e = to_edges(edges)
+        raw = np.empty(ntrials, dtype=np.int64)
+        mn = ctypes.c_int64()
+        self._check(self.lib.cac_ks_sparse(self.handle, _ptr(e), len(e), n, seed, first_trial, ntrials,
+                                           raw.ctypes.data_as(_pi64), ctypes.byref(mn)))
+        return raw, int(mn.value)
+
+    def sparse_contract(self, edges, n: int, tgt: int, seed: int, trial: int, nid: int) -> np.ndarray:
+        """One sparse contraction; returns the child edges as (m', 3) uint64 [u, v, w]."""
+        e = to_edges(edges)
+        out = np.empty(max(len(e), 1), dtype=EDGE)
+        mo = ctypes.c_uint64()
+        self._check(self.lib.cac_sparse_contract(self.handle, _ptr(e), len(e), n, tgt, seed, trial, nid, _ptr(out),
+                                                 len(out), ctypes.byref(mo)))
+        c = out[: mo.value]
+        return np.stack([c["u"].astype(np.uint64), c["v"].astype(np.uint64), c["w"]], axis=1)
+
+    def karger_stein(self, edges, n: int, p: float, seed: int):
+        """SequentialKargerSteinCut semantics: returns (cut, trials)."""
+        e = to_edges(edges)
+        cut = ctypes.c_int64()
+        T = ctypes.c_uint32()
+        self._check(self.lib.cac_karger_stein(self.handle, _ptr(e), len(e), n, p, seed, ctypes.byref(cut),
+                                              ctypes.byref(T)))
+        return int(cut.value), int(T.value)
+
     # ---- sampling contraction and sqrt-cut min cut ------------------------
     def sample_contract(self, edges, n: int, t: int, seed: int, trial: int):
         """Returns (vmap, n_out, merged (k, 3) uint64 [u, v, w])."""
@@ -394,6 +427,11 @@ def serve_mincut_plan(n: int, m: int, p: float):
     if rc != CAC_OK:
         raise CacError(rc, "cac_serve_mincut_plan")
     return int(t.value), int(T.value)
+
+
+def race_key(x: int, w: int) -> int:
+    """The sparse contraction's race key of a 64-bit word x and weight w."""
+    return int(load_library().cac_race_key(x, w))
 
 
 def to_edges(edges) -> np.ndarray:

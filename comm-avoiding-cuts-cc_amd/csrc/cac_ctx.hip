@@ -135,6 +135,9 @@ int cac_ctx_destroy(cac_ctx* ctx) {
     if (ctx->ks_buf[b]) (void)hipFree(ctx->ks_buf[b]);
   if (ctx->smp_buf) (void)hipFree(ctx->smp_buf);
   if (ctx->ks_retry_buf) (void)hipFree(ctx->ks_retry_buf);
+  for (void* b : ctx->sp_bufs)
+    if (b) (void)hipFree(b);
+  if (ctx->sp_dense_buf) (void)hipFree(ctx->sp_dense_buf);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

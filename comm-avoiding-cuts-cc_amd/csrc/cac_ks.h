@@ -16,7 +16,8 @@ struct KsRoot {
   const int64_t* mat;
   const int64_t* rsum;
   uint32_t n, trial, slot;
-  int64_t bound;  // the trial's upper bound (minimum weighted degree)
+  int64_t bound;     // the trial's upper bound (minimum weighted degree)
+  uint32_t nid = 1;  // recursion node id of the matrix (1: the trial's root)
 };
 
 // Row sums of a dense device matrix into d_rsum and their minimum into *d_min.
@@ -50,6 +51,10 @@ size_t job_ks_bytes(const SqrtJob& j);
 // Sqrt-cut trials on a device edge list: raw cut of trial_ids[i] to d_results[i].
 int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
                  const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s);
+// Sparse Karger-Stein trials on a checked device edge list (u < v, 1 <= w < 2^40):
+// raw cut of trial_ids[i] to d_results[i], which must already hold the bound.
+int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
+                  const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s);
 int ks_check_matrix(cac_ctx* ctx, const int64_t* adj, uint32_t n);
 
 // One sampling contraction (cac_sample_contract): canonical labels and the merged

@@ -738,7 +738,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
   std::vector<Node> level;
   for (const KsRoot& r : roots) {
     if (r.n > kKsMaxS) return fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
-    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, 1u, r.slot, r.bound});
+    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, r.nid, r.slot, r.bound});
   }
   int buf = 0;  // level buffer for the children of this level; the parents are in the other one
   while (!level.empty()) {

@@ -1,0 +1,772 @@
+// cac_sparse.hip -- sparse Karger-Stein on gfx950 and the karger_stein driver.
+//
+// Replaces the sparse recursion of comincut: contract_graph / sparse_contract_edges
+// (src/karger-stein/co_mincut.h:74-155), recursive_contraction on sparse graphs
+// (:176-223, :280-293), the dense/sparse dispatch of minimum_cut_trials_start
+// (:396-414) and SequentialKargerSteinCut::compute (src/SequentialKargerSteinCut.cpp:6-27).
+//
+// A node is an edge list (u < v, w >= 1, parallel edges allowed) over n vertices.
+// Contracting it to t' vertices is an exponential race: edge e gets the integer
+// key race_key(x_e, w_e), x_e from Philox(ctr = {trial, node id, e_lo, e_hi},
+// key = {seed, 'SPRS'}), and edges are united in (key, index) order until t'
+// components remain.  One 1024-thread workgroup per child finds that prefix by
+// halving the candidate key range around a sampled median, as the reference
+// halves around nth_element; a trial union-find kept equal to the identity
+// between probes makes each probe cost only its candidate edges.  The child is
+// the node's edges in their order, relabelled by the rank of each component's
+// minimum id, loops dropped and parallel edges kept.  A node with n <= 2048, or
+// with m >= n^2/32 and n <= 8192, is densified and continues in the dense engine
+// under its own node id.  Every node uses the trial's bound, the minimum
+// weighted degree of the input.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cac_common.h"
+#include "cac_internal.h"
+#include "cac_ks.h"
+#include "cac_ks_kernels.h"
+
+namespace cac {
+namespace {
+
+constexpr uint32_t kDomSparse = 0x53505253u;  // 'SPRS'
+constexpr int kSpBlock = 1024;
+constexpr int kSpWaves = kSpBlock / 64;
+constexpr uint32_t kSpSort = 1024;  // LDS sort size: pivot samples, final candidates
+constexpr uint32_t kSparseDenseMax = 2048;
+constexpr unsigned long long kSpDisconnected = ~0ull;
+
+}  // namespace
+
+// floor(2^20 * L / w), L = -log2(U / 2^53) in Q32.32, U = (x >> 11) + 1.  The
+// fraction of log2 comes from bit-by-bit squaring of the mantissa (each square
+// truncated to its top 64 bits), so host and device agree exactly.
+__host__ __device__ inline uint64_t race_key(uint64_t x, uint64_t w) {
+  const uint64_t U = (x >> 11) + 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const int e = 63 - __clzll((long long)U);
+#else
+  const int e = 63 - __builtin_clzll(U);
+#endif
+  uint64_t y = U << (63 - e);  // mantissa in [1, 2), Q1.63
+  uint64_t frac = 0;
+  for (int i = 0; i < 32; ++i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t hi = __umul64hi(y, y), lo = y * y;
+#else
+    const unsigned __int128 sq = (unsigned __int128)y * y;
+    const uint64_t hi = (uint64_t)(sq >> 64), lo = (uint64_t)sq;
+#endif
+    frac <<= 1;
+    if (hi >> 63) {
+      frac |= 1;
+      y = hi;
+    } else {
+      y = (hi << 1) | (lo >> 63);
+    }
+  }
+  const uint64_t L = ((uint64_t)53 << 32) - (((uint64_t)e << 32) | frac);
+  return (L << 20) / (w ? w : 1);
+}
+
+namespace {
+
+struct SpChild {
+  const cac_edge* edges;  // parent edges [m], u < v
+  uint64_t m;
+  uint32_t n, tgt, trial, nid;  // nid: this child's node id (keys come from it)
+  uint32_t slot, pad;
+  cac_edge* out;              // [m] child edges
+  uint64_t* keys;             // [m]
+  uint32_t* list[2];          // [m] candidate edge indices, ping-pong
+  uint32_t* parc;             // [n] committed forest
+  uint32_t* part;             // [n] trial forest, identity between probes; ranks at the end
+  unsigned long long* m_out;  // child edge count, kSpDisconnected when disconnected
+};
+
+// Root with path halving.  Links always point to smaller ids, so even a stale
+// view of the forest ends at a root.
+__device__ inline uint32_t sp_root(uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    const uint32_t g = par[p];
+    if (g != p) par[x] = g;
+    x = p;
+    p = g;
+  }
+  return x;
+}
+
+// Links the sets of a and b (larger root under the smaller); true when this
+// call joined two sets.
+__device__ inline bool sp_union(uint32_t* par, uint32_t a, uint32_t b) {
+  uint32_t ra = sp_root(par, a), rb = sp_root(par, b);
+  while (ra != rb) {
+    const uint32_t lo = min(ra, rb), hi = max(ra, rb);
+    const uint32_t old = atomicCAS(par + hi, hi, lo);
+    if (old == hi) return true;
+    ra = sp_root(par, old);
+    rb = sp_root(par, lo);
+  }
+  return false;
+}
+
+__device__ inline uint64_t sp_sum(uint64_t x, uint64_t* red) { return block_sum_u64<kSpBlock>(x, red); }
+
+__device__ inline uint64_t sp_min(uint64_t x, uint64_t* red) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = min(x, (uint64_t)__shfl_xor(x, d, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint64_t t = ~0ull;
+  for (int w = 0; w < kSpWaves; ++w) t = min(t, red[w]);
+  __syncthreads();
+  return t;
+}
+
+__device__ inline uint64_t sp_max(uint64_t x, uint64_t* red) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x = max(x, (uint64_t)__shfl_xor(x, d, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint64_t t = 0;
+  for (int w = 0; w < kSpWaves; ++w) t = max(t, red[w]);
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan of per-thread counts in thread order; *total gets the sum.
+__device__ inline uint64_t sp_exscan(uint64_t x, uint64_t* red, uint64_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t inc = wave_incl_scan_u64(x, lane);
+  if (lane == 63) red[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0, t = 0;
+  for (int w = 0; w < kSpWaves; ++w) {
+    if (w < wave) before += red[w];
+    t += red[w];
+  }
+  __syncthreads();
+  *total = t;
+  return before + inc - x;
+}
+
+// Bitonic sort of kSpSort (key, idx) pairs in LDS, ascending by key then idx.
+__device__ inline void sp_sort(uint64_t* key, uint32_t* idx) {
+  const uint32_t i = threadIdx.x;
+  for (uint32_t k = 2; k <= kSpSort; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t p = i ^ j;
+      if (p > i) {
+        const uint64_t ka = key[i], kb = key[p];
+        const uint32_t ia = idx[i], ib = idx[p];
+        const bool greater = ka > kb || (ka == kb && ia > ib);
+        const bool up = (i & k) == 0;
+        if (greater == up) {
+          key[i] = kb;
+          key[p] = ka;
+          idx[i] = ib;
+          idx[p] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Components removed by uniting, on top of the committed forest, the candidate
+// edges list[pos] with sel(pos, key) true.  The trial forest is restored to the
+// identity afterwards: every entry a probe writes is the committed root of an
+// endpoint of one of its edges, and those are reset.
+template <class Sel>
+__device__ uint32_t sp_probe(const SpChild& c, const uint32_t* list, uint64_t ns, Sel sel, uint64_t* red) {
+  uint64_t merged = 0;
+  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
+    const uint32_t e = list[pos];
+    if (!sel(pos, c.keys[e])) continue;
+    const cac_edge x = c.edges[e];
+    const uint32_t a = sp_root(c.parc, x.u), b = sp_root(c.parc, x.v);
+    if (a != b && sp_union(c.part, a, b)) ++merged;
+  }
+  block_sync_global();
+  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
+    const uint32_t e = list[pos];
+    if (!sel(pos, c.keys[e])) continue;
+    const cac_edge x = c.edges[e];
+    const uint32_t a = sp_root(c.parc, x.u), b = sp_root(c.parc, x.v);
+    c.part[a] = a;
+    c.part[b] = b;
+  }
+  const uint64_t total = sp_sum(merged, red);
+  block_sync_global();
+  return (uint32_t)total;
+}
+
+// Unites the selected candidate edges into the committed forest.
+template <class Sel>
+__device__ void sp_commit(const SpChild& c, const uint32_t* list, uint64_t ns, Sel sel) {
+  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
+    const uint32_t e = list[pos];
+    if (!sel(pos, c.keys[e])) continue;
+    const cac_edge x = c.edges[e];
+    sp_union(c.parc, x.u, x.v);
+  }
+  block_sync_global();
+}
+
+struct SelBelow {  // key <= tau
+  uint64_t tau;
+  __device__ bool operator()(uint64_t, uint64_t k) const { return k <= tau; }
+};
+struct SelPrefix {  // the first p candidates
+  uint64_t p;
+  __device__ bool operator()(uint64_t pos, uint64_t) const { return pos < p; }
+};
+
+__global__ void __launch_bounds__(kSpBlock) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
+                                                              int64_t* __restrict__ results) {
+  __shared__ uint64_t red[kSpWaves];
+  __shared__ uint64_t skey[kSpSort];
+  __shared__ uint32_t sidx[kSpSort];
+  const SpChild c = children[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+
+  // 1. race keys, identity forests, every edge a candidate
+  uint64_t kmin = ~0ull, kmax = 0;
+  for (uint64_t e = tid; e < c.m; e += kSpBlock) {
+    const u32x4 r = philox4x32(c.trial, c.nid, (uint32_t)e, (uint32_t)(e >> 32), seed, kDomSparse);
+    const uint64_t k = race_key(lo64(r), c.edges[e].w);
+    c.keys[e] = k;
+    c.list[0][e] = (uint32_t)e;
+    kmin = min(kmin, k);
+    kmax = max(kmax, k);
+  }
+  for (uint32_t v = tid; v < c.n; v += kSpBlock) {
+    c.parc[v] = v;
+    c.part[v] = v;
+  }
+  kmin = sp_min(kmin, red);
+  kmax = sp_max(kmax, red);
+  block_sync_global();
+
+  if (c.n > c.tgt) {  // else nothing is contracted
+    // 2. halving: the committed forest holds the edges with key <= lo, the
+    //    candidates are the edges with key in (lo, hi]; components > tgt always
+    uint32_t* cur = c.list[0];
+    uint32_t* nxt = c.list[1];
+    uint64_t ns = c.m;
+    uint32_t comps = c.n;
+    while (ns > kSpSort && kmin < kmax) {
+      skey[tid] = c.keys[cur[(uint64_t)tid * ns / kSpSort]];
+      sidx[tid] = tid;
+      __syncthreads();
+      sp_sort(skey, sidx);
+      const uint64_t tau = min(max(skey[kSpSort / 2 - 1], kmin), kmax - 1);
+      __syncthreads();
+      const SelBelow below{tau};
+      const uint32_t merged = sp_probe(c, cur, ns, below, red);
+      const bool commit = comps - merged > c.tgt;
+      if (commit) {
+        sp_commit(c, cur, ns, below);
+        comps -= merged;
+      }
+      // keep the half that still holds the boundary, in list (= index) order
+      const uint64_t chunk = (ns + kSpBlock - 1) / kSpBlock;
+      const uint64_t p0 = min(ns, (uint64_t)tid * chunk), p1 = min(ns, p0 + chunk);
+      uint64_t cnt = 0;
+      for (uint64_t pos = p0; pos < p1; ++pos) cnt += (c.keys[cur[pos]] <= tau) != commit;
+      uint64_t total = 0;
+      uint64_t off = sp_exscan(cnt, red, &total);
+      uint64_t nmin = ~0ull, nmax = 0;
+      for (uint64_t pos = p0; pos < p1; ++pos) {
+        const uint32_t e = cur[pos];
+        const uint64_t k = c.keys[e];
+        if ((k <= tau) == commit) continue;
+        nxt[off++] = e;
+        nmin = min(nmin, k);
+        nmax = max(nmax, k);
+      }
+      kmin = sp_min(nmin, red);
+      kmax = sp_max(nmax, red);
+      ns = total;
+      uint32_t* t = cur;
+      cur = nxt;
+      nxt = t;
+      block_sync_global();
+    }
+
+    // 3. candidates in (key, index) order: sorted in LDS when few, else they all
+    //    share one key and the list is already in index order
+    if (ns <= kSpSort) {
+      const bool have = tid < ns;
+      const uint32_t e = have ? cur[tid] : 0xFFFFFFFFu;
+      skey[tid] = have ? c.keys[e] : ~0ull;
+      sidx[tid] = e;
+      __syncthreads();
+      sp_sort(skey, sidx);
+      if (have) cur[tid] = sidx[tid];
+      block_sync_global();
+    }
+    // shortest prefix of the candidates that leaves tgt components
+    if (comps - sp_probe(c, cur, ns, SelPrefix{ns}, red) > c.tgt) {
+      if (tid == 0) {
+        *c.m_out = kSpDisconnected;
+        atomicMin((long long*)&results[c.slot], 0ll);
+      }
+      return;
+    }
+    uint64_t lo = 0, hi = ns;  // comps(lo) > tgt >= comps(hi)
+    while (hi - lo > 1) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (comps - sp_probe(c, cur, ns, SelPrefix{mid}, red) <= c.tgt) hi = mid;
+      else lo = mid;
+    }
+    sp_commit(c, cur, ns, SelPrefix{hi});
+  }
+
+  // 4. rank of each component's minimum id (the root), in id order
+  const uint64_t vc = ((uint64_t)c.n + kSpBlock - 1) / kSpBlock;
+  const uint32_t v0 = (uint32_t)min((uint64_t)c.n, tid * vc), v1 = (uint32_t)min((uint64_t)c.n, v0 + vc);
+  uint64_t cnt = 0;
+  for (uint32_t v = v0; v < v1; ++v) cnt += sp_root(c.parc, v) == v;
+  uint64_t total = 0;
+  uint64_t off = sp_exscan(cnt, red, &total);
+  for (uint32_t v = v0; v < v1; ++v)
+    if (sp_root(c.parc, v) == v) c.part[v] = (uint32_t)off++;
+  block_sync_global();
+
+  // 5. the child: edges in parent order, relabelled, loops dropped
+  const uint64_t ec = (c.m + kSpBlock - 1) / kSpBlock;
+  const uint64_t e0 = min(c.m, (uint64_t)tid * ec), e1 = min(c.m, e0 + ec);
+  cnt = 0;
+  for (uint64_t e = e0; e < e1; ++e) cnt += sp_root(c.parc, c.edges[e].u) != sp_root(c.parc, c.edges[e].v);
+  off = sp_exscan(cnt, red, &total);
+  for (uint64_t e = e0; e < e1; ++e) {
+    const cac_edge x = c.edges[e];
+    const uint32_t ra = sp_root(c.parc, x.u), rb = sp_root(c.parc, x.v);
+    if (ra == rb) continue;
+    const uint32_t a = c.part[ra], b = c.part[rb];
+    c.out[off++] = cac_edge{min(a, b), max(a, b), x.w};
+  }
+  if (tid == 0) *c.m_out = total;
+}
+
+// Handover to the dense engine: n x n matrix and row sums from an edge list.
+struct SpDense {
+  const cac_edge* edges;
+  uint64_t m;
+  uint32_t n, pad;
+  int64_t* mat;
+  int64_t* rsum;
+};
+
+__global__ void __launch_bounds__(256) k_sp_zero(const SpDense* __restrict__ nodes) {
+  const SpDense d = nodes[blockIdx.y];
+  const uint64_t nn = (uint64_t)d.n * d.n;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * 256) d.mat[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_sp_scatter(const SpDense* __restrict__ nodes) {
+  const SpDense d = nodes[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < d.m; i += (uint64_t)gridDim.x * 256) {
+    const cac_edge x = d.edges[i];
+    atomicAdd((unsigned long long*)&d.mat[(uint64_t)x.u * d.n + x.v], (unsigned long long)x.w);
+    atomicAdd((unsigned long long*)&d.mat[(uint64_t)x.v * d.n + x.u], (unsigned long long)x.w);
+  }
+}
+
+// One wavefront per row.
+__global__ void __launch_bounds__(256) k_sp_rowsum(const SpDense* __restrict__ nodes) {
+  const SpDense d = nodes[blockIdx.y];
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < d.n; r += gridDim.x * 4) {
+    uint64_t acc = 0;
+    for (uint32_t x = lane; x < d.n; x += 64) acc += (uint64_t)d.mat[(uint64_t)r * d.n + x];
+    acc = wave_sum_u64(acc);
+    if (lane == 0) d.rsum[r] = (int64_t)acc;
+  }
+}
+
+// Edge check (ids < n, u != v, 1 <= w < 2^40), u < v order, weighted degrees,
+// and the total weight as sums of its high (w >> 20) and low 20-bit parts,
+// which cannot wrap for fewer than 2^32 edges.  flags: bit 0 bad id or loop,
+// bit 1 weight out of range.
+__global__ void __launch_bounds__(256) k_sp_prep(cac_edge* e, uint64_t m, uint32_t n, unsigned long long* deg,
+                                                 uint32_t* flags, unsigned long long* wsum) {
+  uint32_t f = 0;
+  unsigned long long whi = 0, wlo = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256) {
+    cac_edge x = e[i];
+    if (x.u >= n || x.v >= n || x.u == x.v) {
+      f |= 1;
+      continue;
+    }
+    if (x.w == 0 || x.w >= (1ull << 40)) {
+      f |= 2;
+      continue;
+    }
+    if (x.u > x.v) {
+      const uint32_t t = x.u;
+      x.u = x.v;
+      x.v = t;
+      e[i] = x;
+    }
+    atomicAdd(&deg[x.u], (unsigned long long)x.w);
+    atomicAdd(&deg[x.v], (unsigned long long)x.w);
+    whi += x.w >> 20;
+    wlo += x.w & 0xFFFFF;
+  }
+  if (f) atomicOr(flags, f);
+  if (whi) atomicAdd(&wsum[0], whi);
+  if (wlo) atomicAdd(&wsum[1], wlo);
+}
+
+__global__ void __launch_bounds__(256) k_sp_min(const unsigned long long* deg, uint32_t n, unsigned long long* out) {
+  __shared__ unsigned long long w[4];
+  unsigned long long b = ~0ull;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) b = min(b, deg[i]);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) b = min(b, (unsigned long long)__shfl_xor(b, d, 64));
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMin(out, min(min(w[0], w[1]), min(w[2], w[3])));
+}
+
+__global__ void k_sp_fill(unsigned long long* p, uint64_t count, unsigned long long v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+inline bool hands_over(uint32_t n, uint64_t m) {
+  return n <= kSparseDenseMax || (m >= (uint64_t)n * n / 32 && n <= kKsMaxS);
+}
+
+struct SpNode {
+  const cac_edge* edges;
+  uint64_t m;
+  uint32_t n, trial, nid, slot;
+};
+
+size_t child_bytes(const SpNode& nd) {
+  return carve_size(nd.m * sizeof(cac_edge)) + carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) +
+         2 * carve_size((size_t)nd.n * 4) + sizeof(SpChild) + 8;
+}
+size_t dense_bytes(const SpNode& nd) {
+  return carve_size((size_t)nd.n * nd.n * 8) + carve_size((size_t)nd.n * 8) + sizeof(SpDense);
+}
+
+struct SpRun {
+  cac_ctx* ctx;
+  uint32_t seed;
+  int64_t bound;
+  int64_t* d_results;
+  hipStream_t s;
+  size_t depth_budget;
+
+  // Densifies nodes in batches that fit the free memory (matrices plus the
+  // dense engine's level buffers) and runs the dense engine on them.
+  int dense(const std::vector<SpNode>& nodes) {
+    size_t i0 = 0;
+    while (i0 < nodes.size()) {
+      size_t avail = 0;
+      int rc = device_memory_available(ctx, &avail);
+      if (rc) return rc;
+      avail += ctx->sp_dense_bytes;
+      const size_t budget = (size_t)(0.6 * (double)avail);
+      size_t i1 = i0, bytes = 256, mats = 256;
+      uint32_t nmax = 0;
+      while (i1 < nodes.size() && i1 - i0 < 4096) {
+        const size_t b = dense_bytes(nodes[i1]) + 2 * ks_level_bytes_per_root(nodes[i1].n);
+        if (i1 > i0 && bytes + b > budget) break;
+        bytes += b;
+        mats += dense_bytes(nodes[i1]);
+        nmax = std::max(nmax, nodes[i1].n);
+        ++i1;
+      }
+      const size_t count = i1 - i0;
+      rc = grow_buffer(ctx, &ctx->sp_dense_buf, &ctx->sp_dense_bytes, mats);
+      if (rc) return rc;
+      Carver cv(ctx->sp_dense_buf);
+      SpDense* d_nodes = cv.take<SpDense>(count);
+      std::vector<SpDense> hd(count);
+      std::vector<KsRoot> roots;
+      uint64_t mmax = 0;
+      for (size_t i = 0; i < count; ++i) {
+        const SpNode& nd = nodes[i0 + i];
+        hd[i] = SpDense{nd.edges, nd.m, nd.n, 0, cv.take<int64_t>((size_t)nd.n * nd.n), cv.take<int64_t>(nd.n)};
+        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, bound};
+        r.nid = nd.nid;
+        roots.push_back(r);
+        mmax = std::max(mmax, nd.m);
+      }
+      CAC_HIP(ctx, hipMemcpyAsync(d_nodes, hd.data(), count * sizeof(SpDense), hipMemcpyHostToDevice, s));
+      const unsigned gz = grid_for((uint64_t)nmax * nmax, 256 * 16, 1024);
+      const unsigned gs = grid_for(mmax, 256 * 4, 1024);
+      const unsigned gr = grid_for(nmax, 4, 1024);
+      prof_begin(ctx, "sparse_densify", s);
+      hipLaunchKernelGGL(k_sp_zero, dim3(gz, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_scatter, dim3(gs, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_rowsum, dim3(gr, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
+      prof_end(ctx, s);
+      CAC_LAUNCH_CHECK(ctx, "sparse densify");
+      rc = ks_run(ctx, roots, seed, d_results, s);
+      if (rc) return rc;
+      i0 = i1;
+    }
+    return CAC_OK;
+  }
+
+  // One depth of the recursion: hands over what is small or dense enough, and
+  // contracts the rest in chunks whose children fit this depth's buffer, each
+  // chunk's subtree finished before the next chunk reuses the buffer.
+  int level(const std::vector<SpNode>& nodes, size_t depth) {
+    std::vector<SpNode> dn, sp;
+    for (const SpNode& nd : nodes) (hands_over(nd.n, nd.m) ? dn : sp).push_back(nd);
+    if (!dn.empty()) {
+      int rc = dense(dn);
+      if (rc) return rc;
+    }
+    if (sp.empty()) return CAC_OK;
+    if (ctx->sp_bufs.size() <= depth) {
+      ctx->sp_bufs.resize(depth + 1, nullptr);
+      ctx->sp_bytes.resize(depth + 1, 0);
+    }
+    size_t i0 = 0;
+    while (i0 < sp.size()) {
+      size_t i1 = i0, bytes = 1024;
+      while (i1 < sp.size() && i1 - i0 < 8192) {
+        const size_t b = 2 * child_bytes(sp[i1]);
+        if (i1 > i0 && bytes + b > depth_budget) break;
+        bytes += b;
+        ++i1;
+      }
+      const size_t count = i1 - i0, nchild = 2 * count;
+      int rc = grow_buffer(ctx, &ctx->sp_bufs[depth], &ctx->sp_bytes[depth], bytes);
+      if (rc) return rc;
+      Carver cv(ctx->sp_bufs[depth]);
+      SpChild* d_children = cv.take<SpChild>(nchild);
+      unsigned long long* d_mout = cv.take<unsigned long long>(nchild);
+      std::vector<SpChild> ch(nchild);
+      for (size_t i = 0; i < count; ++i) {
+        const SpNode& nd = sp[i0 + i];
+        for (uint32_t k = 0; k < 2; ++k) {
+          SpChild& x = ch[2 * i + k];
+          x.edges = nd.edges;
+          x.m = nd.m;
+          x.n = nd.n;
+          x.tgt = ks_target_host(nd.n);
+          x.trial = nd.trial;
+          x.nid = 2 * nd.nid + k;
+          x.slot = nd.slot;
+          x.pad = 0;
+          x.out = cv.take<cac_edge>(nd.m);
+          x.keys = cv.take<uint64_t>(nd.m);
+          x.list[0] = cv.take<uint32_t>(nd.m);
+          x.list[1] = cv.take<uint32_t>(nd.m);
+          x.parc = cv.take<uint32_t>(nd.n);
+          x.part = cv.take<uint32_t>(nd.n);
+          x.m_out = d_mout + 2 * i + k;
+        }
+      }
+      CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(SpChild), hipMemcpyHostToDevice, s));
+      prof_begin(ctx, "sparse_contract", s);
+      hipLaunchKernelGGL(k_sparse_contract, dim3((unsigned)nchild), dim3(kSpBlock), 0, s, (const SpChild*)d_children,
+                         seed, d_results);
+      prof_end(ctx, s);
+      CAC_LAUNCH_CHECK(ctx, "sparse contract");
+      std::vector<unsigned long long> mout(nchild);
+      CAC_HIP(ctx, hipMemcpyAsync(mout.data(), d_mout, nchild * 8, hipMemcpyDeviceToHost, s));
+      CAC_HIP(ctx, hipStreamSynchronize(s));
+      std::vector<SpNode> next;
+      for (size_t i = 0; i < nchild; ++i) {
+        if (mout[i] == kSpDisconnected) continue;  // its cut of 0 is already recorded
+        next.push_back({ch[i].out, mout[i], ch[i].tgt, ch[i].trial, ch[i].nid, ch[i].slot});
+      }
+      rc = level(next, depth + 1);
+      if (rc) return rc;
+      i0 = i1;
+    }
+    return CAC_OK;
+  }
+};
+
+// Depth of the sparse recursion in the worst case (no edge ever lost).
+size_t sparse_depth(uint32_t n, uint64_t m) {
+  size_t d = 0;
+  for (uint32_t s = n; !hands_over(s, m); s = ks_target_host(s)) ++d;
+  return d;
+}
+
+}  // namespace
+
+int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
+                  const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s) {
+  size_t avail = 0;
+  int rc = device_memory_available(ctx, &avail);
+  if (rc) return rc;
+  for (size_t i = 0; i < ctx->sp_bufs.size(); ++i) avail += ctx->sp_bytes[i];
+  avail += ctx->sp_dense_bytes;
+  const size_t depth = sparse_depth(n, m);
+  SpRun run{ctx, seed, bound, d_results, s, (size_t)(0.45 * (double)avail) / (depth + 1)};
+  std::vector<SpNode> roots;
+  for (size_t i = 0; i < trial_ids.size(); ++i) roots.push_back({d_edges, m, n, trial_ids[i], 1u, (uint32_t)i});
+  rc = run.level(roots, 0);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  return CAC_OK;
+}
+
+}  // namespace cac
+
+extern "C" {
+
+uint64_t cac_race_key(uint64_t x, uint64_t w) { return cac::race_key(x, w); }
+
+int cac_ks_sparse(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t seed, uint32_t first_trial,
+                  uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!edges || m == 0 || n < 2 || ntrials == 0 || !min_cut)
+    return cac::fail(ctx, CAC_EINVAL, "need n >= 2, at least one edge and one trial");
+  if (m >= 0xFFFFFFFFull) return cac::fail(ctx, CAC_EINVAL, "sparse Karger-Stein supports fewer than 2^32 - 1 edges");
+  hipStream_t s = ctx->stream;
+  cac_edge* d_e = nullptr;
+  unsigned long long* d_aux = nullptr;  // degrees [n], minimum, flags, weight sum (high, low parts)
+  int64_t* d_res = nullptr;
+  hipError_t e = hipMalloc((void**)&d_e, m * sizeof(cac_edge));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_aux, 8 * ((size_t)n + 4));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_res, 8 * (size_t)ntrials);
+  if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "sparse Karger-Stein buffers");
+  uint32_t flags = 0;
+  unsigned long long bound = 0, wsum[2] = {0, 0};
+  if (rc == CAC_OK) {
+    unsigned long long* d_min = d_aux + n;
+    uint32_t* d_flags = (uint32_t*)(d_aux + n + 1);
+    unsigned long long* d_wsum = d_aux + n + 2;
+    e = hipMemcpyAsync(d_e, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(cac::k_sp_fill, dim3(cac::grid_for((uint64_t)n + 4, 256, 1024)), dim3(256), 0, s, d_aux,
+                       (uint64_t)n + 4, 0ull);
+    hipLaunchKernelGGL(cac::k_sp_fill, dim3(1), dim3(64), 0, s, d_min, (uint64_t)1, ~0ull);
+    hipLaunchKernelGGL(cac::k_sp_prep, dim3(cac::grid_for(m, 256, 4096)), dim3(256), 0, s, d_e, m, n, d_aux,
+                       d_flags, d_wsum);
+    hipLaunchKernelGGL(cac::k_sp_min, dim3(cac::grid_for(n, 256, 1024)), dim3(256), 0, s,
+                       (const unsigned long long*)d_aux, n, d_min);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&bound, d_min, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(wsum, d_wsum, 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "sparse Karger-Stein edge check");
+  }
+  if (rc == CAC_OK && (flags & 1)) rc = cac::fail(ctx, CAC_EINVAL, "edge endpoint >= n, or a loop");
+  if (rc == CAC_OK && (flags & 2)) rc = cac::fail(ctx, CAC_ERANGE, "edge weight must be in [1, 2^40)");
+  if (rc == CAC_OK && ((unsigned __int128)wsum[0] << 20) + wsum[1] >= ((unsigned __int128)1 << 61))
+    rc = cac::fail(ctx, CAC_ERANGE, "total edge weight must stay below 2^61");
+  if (rc == CAC_OK) {
+    hipLaunchKernelGGL(cac::k_sp_fill, dim3(cac::grid_for(ntrials, 256, 64)), dim3(256), 0, s,
+                       (unsigned long long*)d_res, (uint64_t)ntrials, bound);
+    std::vector<uint32_t> ids(ntrials);
+    for (uint32_t i = 0; i < ntrials; ++i) ids[i] = first_trial + i;
+    rc = cac::ks_sparse_run(ctx, d_e, m, n, seed, ids, (int64_t)bound, d_res, s);
+  }
+  if (rc == CAC_OK) {
+    std::vector<int64_t> res(ntrials);
+    e = hipMemcpyAsync(res.data(), d_res, 8 * (size_t)ntrials, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "sparse Karger-Stein results");
+    int64_t mn = INT64_MAX;
+    for (uint32_t t = 0; t < ntrials && rc == CAC_OK; ++t) {
+      if (raw_cuts) raw_cuts[t] = res[t];
+      mn = std::min(mn, res[t]);
+    }
+    if (rc == CAC_OK) *min_cut = mn;
+  }
+  (void)hipStreamSynchronize(s);
+  if (d_res) (void)hipFree(d_res);
+  if (d_aux) (void)hipFree(d_aux);
+  if (d_e) (void)hipFree(d_e);
+  return rc;
+}
+
+int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t tgt, uint32_t seed,
+                        uint32_t trial, uint32_t nid, cac_edge* out, uint64_t out_cap, uint64_t* m_out) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!edges || !m_out || (m > 0 && !out) || n < 1 || tgt < 1)
+    return cac::fail(ctx, CAC_EINVAL, "need edges, n >= 1, tgt >= 1 and an output");
+  if (m >= 0xFFFFFFFFull) return cac::fail(ctx, CAC_EINVAL, "sparse contraction supports fewer than 2^32 - 1 edges");
+  for (uint64_t i = 0; i < m; ++i) {
+    const cac_edge& x = edges[i];
+    if (x.u >= n || x.v >= n || x.u >= x.v) return cac::fail(ctx, CAC_EINVAL, "edges must have u < v < n");
+    if (x.w == 0 || x.w >= (1ull << 40)) return cac::fail(ctx, CAC_ERANGE, "edge weight must be in [1, 2^40)");
+  }
+  hipStream_t s = ctx->stream;
+  const size_t mm = m ? m : 1;
+  const size_t bytes = cac::carve_size(sizeof(cac::SpChild)) + cac::carve_size(mm * sizeof(cac_edge)) * 2 +
+                       cac::carve_size(mm * 8) + 2 * cac::carve_size(mm * 4) + 2 * cac::carve_size((size_t)n * 4) +
+                       cac::carve_size(16) + 256;
+  void* base = nullptr;
+  rc = cac::scratch_reserve(ctx, bytes, &base);
+  if (rc) return rc;
+  cac::Carver cv(base);
+  cac::SpChild c{};
+  cac::SpChild* d_child = cv.take<cac::SpChild>(1);
+  cac_edge* d_in = cv.take<cac_edge>(mm);
+  c.edges = d_in;
+  c.m = m;
+  c.n = n;
+  c.tgt = tgt;
+  c.trial = trial;
+  c.nid = nid;
+  c.out = cv.take<cac_edge>(mm);
+  c.keys = cv.take<uint64_t>(mm);
+  c.list[0] = cv.take<uint32_t>(mm);
+  c.list[1] = cv.take<uint32_t>(mm);
+  c.parc = cv.take<uint32_t>(n);
+  c.part = cv.take<uint32_t>(n);
+  unsigned long long* d_mout = cv.take<unsigned long long>(1);
+  int64_t* d_res = cv.take<int64_t>(1);
+  c.m_out = d_mout;
+  if (m) CAC_HIP(ctx, hipMemcpyAsync(d_in, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice, s));
+  CAC_HIP(ctx, hipMemcpyAsync(d_child, &c, sizeof(c), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(cac::k_sparse_contract, dim3(1), dim3(cac::kSpBlock), 0, s, (const cac::SpChild*)d_child, seed,
+                     d_res);
+  CAC_LAUNCH_CHECK(ctx, "sparse contract");
+  unsigned long long mo = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&mo, d_mout, 8, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  if (mo == cac::kSpDisconnected) return cac::fail(ctx, CAC_ENOTCONN, "more than tgt components");
+  *m_out = mo;
+  if (mo > out_cap) return cac::fail(ctx, CAC_ERANGE, "output capacity too small");
+  if (mo) CAC_HIP(ctx, hipMemcpy(out, c.out, mo * sizeof(cac_edge), hipMemcpyDeviceToHost));
+  return CAC_OK;
+}
+
+int cac_karger_stein(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed,
+                     int64_t* cut, uint32_t* trials) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!edges || m == 0 || n < 2 || !cut) return cac::fail(ctx, CAC_EINVAL, "need n >= 2 and at least one edge");
+  if (!(p >= 0.0 && p < 1.0)) return cac::fail(ctx, CAC_EINVAL, "success probability must be in [0, 1)");
+  const uint32_t T = cac::ks_trials_host(n, p);
+  if (trials) *trials = T;
+  if (m >= (uint64_t)n * n / 32) {  // the reference's dense switch (co_mincut.h:401)
+    if (n > cac::kKsMaxS) return cac::fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
+    std::vector<int64_t> A((size_t)n * n, 0);
+    unsigned __int128 total = 0;
+    for (uint64_t i = 0; i < m; ++i) {
+      const cac_edge& x = edges[i];
+      if (x.u >= n || x.v >= n || x.u == x.v) return cac::fail(ctx, CAC_EINVAL, "edge endpoint >= n, or a loop");
+      if (x.w == 0 || x.w >= (1ull << 40)) return cac::fail(ctx, CAC_ERANGE, "edge weight must be in [1, 2^40)");
+      total += x.w;
+      if (total >= ((unsigned __int128)1 << 61)) return cac::fail(ctx, CAC_ERANGE, "total edge weight must stay below 2^61");
+      A[(size_t)x.u * n + x.v] += (int64_t)x.w;
+      A[(size_t)x.v * n + x.u] += (int64_t)x.w;
+    }
+    return cac_ks_dense(ctx, A.data(), n, seed, 0, T, nullptr, cut);
+  }
+  return cac_ks_sparse(ctx, edges, m, n, seed, 0, T, nullptr, cut);
+}
+
+}  // extern "C"

@@ -32,7 +32,7 @@ extern "C" {
 #define CAC_EINVAL (-1)   /* bad argument: null pointer, vertex id >= n, size out of range */
 #define CAC_ENOMEM (-2)   /* device or host allocation failed */
 #define CAC_EDEVICE (-3)  /* HIP runtime error */
-#define CAC_ENOTCONN (-4) /* reserved (a disconnected graph is reported as cut 0) */
+#define CAC_ENOTCONN (-4) /* more components than a contraction target (a min cut of a disconnected graph is 0) */
 #define CAC_ERANGE (-5)   /* a weight sum would overflow int64 */
 #define CAC_EIO (-6)      /* edge-list file cannot be opened or parsed */
 
@@ -114,6 +114,45 @@ int cac_ks_dense(cac_ctx* ctx, const int64_t* adj, uint32_t n, uint32_t seed, ui
 /* Device variant: d_adj in device memory (not validated); results on the host. */
 int cac_ks_dense_dev(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, uint32_t first_trial,
                      uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut, void* stream);
+
+/* ---- sparse Karger-Stein ------------------------------------------------------
+ * Replaces comincut::minimum_cut(sparse_graph<T>*, double, int) on a graph that
+ * stays sparse: contract_graph / sparse_contract_edges (src/karger-stein/
+ * co_mincut.h:74-155) inside recursive_contraction (:176-223, :280-293).
+ * edges: u != v, ids < n, 1 <= w < 2^40, total weight < 2^61, either order,
+ * parallel edges allowed; fewer than 2^32 - 1 edges.  A node of n vertices is contracted to
+ * t' = max(ceil(n / sqrt 2) + 1, 128) vertices by uniting its edges in
+ * (race key, index) order, the race key of edge e of the node with id nid in
+ * trial t being cac_race_key(x, w), x = the low 64 bits of
+ * Philox-4x32-10(ctr = {t, nid, e_lo, e_hi}, key = {seed, 'SPRS'}); the children
+ * of node nid are 2 nid and 2 nid + 1 (the root is 1).  A child keeps the
+ * node's edges in order, relabelled by the rank of each component's minimum
+ * vertex id, loops dropped, parallel edges not merged.  A node with n <= 2048,
+ * or with m >= n^2 / 32 and n <= 8192, continues in the dense engine
+ * (cac_ks_dense rules) under its own node id.  The bound of every trial is the
+ * minimum weighted degree; a disconnected graph has cut 0.  raw_cuts (optional,
+ * ntrials entries) and *min_cut as for cac_ks_dense. */
+int cac_ks_sparse(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t seed, uint32_t first_trial,
+                  uint32_t ntrials, int64_t* raw_cuts, int64_t* min_cut);
+/* One sparse contraction, the step above for node nid of trial `trial`: the
+ * child edge list of (edges, m, n) contracted to tgt vertices (tgt >= n: no
+ * contraction, loops none).  Edges must have u < v < n and 1 <= w < 2^40.
+ * *m_out = child edges; CAC_ERANGE if out_cap is smaller, CAC_ENOTCONN if all
+ * edges leave more than tgt components.  Replaces sparse contract_graph
+ * (co_mincut.h:131-155) without its PR pass. */
+int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, uint32_t tgt, uint32_t seed,
+                        uint32_t trial, uint32_t nid, cac_edge* out, uint64_t out_cap, uint64_t* m_out);
+/* Replaces SequentialKargerSteinCut::compute (src/SequentialKargerSteinCut.cpp:6-27)
+ * with comincut::minimum_cut_trials_start's dense switch (co_mincut.h:396-414):
+ * T = cac_ks_trials(n, p) trials (ids 0..T-1; p = 0 gives one trial); the dense
+ * engine when m >= n^2 / 32 (n <= 8192), else cac_ks_sparse.  *cut = minimum,
+ * *trials = T. */
+int cac_karger_stein(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed,
+                     int64_t* cut, uint32_t* trials);
+/* The race key of the sparse contraction: floor(2^20 * L / w) with
+ * L = -log2(((x >> 11) + 1) / 2^53) in Q32.32 fixed point, the fraction of the
+ * logarithm by 32 steps of mantissa squaring.  Exposed for tests. */
+uint64_t cac_race_key(uint64_t x, uint64_t w);
 
 /* ---- sizes and trial counts ------------------------------------------------
  * Same floating point as the reference: comincut::number_of_trials

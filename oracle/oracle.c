@@ -620,3 +620,140 @@ void orc_approx_cut(uint32_t n, uint64_t m, const uint64_t* edges, double p, uin
     free(lab);
     free(sub);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Sparse Karger-Stein (co_mincut.h:74-155, recursion :176-223, 280-293)     */
+/* ------------------------------------------------------------------------- */
+#define DOM_SPARSE 0x53505253u /* 'SPRS' */
+#define SPARSE_DENSE_MAX 2048u
+
+/* Exponential-race key of an edge of weight w from a 64-bit word x:
+ * floor(2^20 * L / w), L = -log2(U / 2^53) in Q32.32, U = (x >> 11) + 1.
+ * log2 by the classical bit-by-bit squaring of the mantissa (32 steps, each
+ * squaring truncated to 64 bits), so CPU and GPU agree exactly.  Ordering edges
+ * by (key, index) is, up to the fixed point, ordering by Exp(w) variables:
+ * the exponential race of co_mincut.h:141-144. */
+uint64_t orc_race_key(uint64_t x, uint64_t w) {
+    uint64_t U = (x >> 11) + 1;
+    int e = 63 - __builtin_clzll(U);
+    uint64_t y = U << (63 - e); /* mantissa in [1, 2), Q1.63 */
+    uint64_t frac = 0;
+    for (int i = 0; i < 32; ++i) {
+        __uint128_t sq = (__uint128_t)y * y;
+        frac <<= 1;
+        if ((uint64_t)(sq >> 127)) {
+            frac |= 1;
+            y = (uint64_t)(sq >> 64);
+        } else {
+            y = (uint64_t)(sq >> 63);
+        }
+    }
+    uint64_t log2u = ((uint64_t)e << 32) | frac;
+    uint64_t L = ((uint64_t)53 << 32) - log2u;
+    return (L << 20) / (w ? w : 1);
+}
+
+typedef struct {
+    uint64_t key;
+    uint64_t idx;
+} race_item;
+
+static int cmp_race(const void* a, const void* b) {
+    const race_item* x = (const race_item*)a;
+    const race_item* y = (const race_item*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx ? 1 : 0);
+}
+
+/* Contract the sparse graph (n, m, E = u,v,w triples, u < v) to tgt vertices:
+ * edges in (key, index) order are united until tgt components remain; the
+ * child keeps every edge in its original order relabelled by the rank of its
+ * components' minimum ids, loops dropped, parallel edges not merged.  Returns
+ * the child edge list (caller frees) and *m_out, or NULL when even all edges
+ * leave more than tgt components (disconnected: cut 0). */
+uint64_t* orc_sparse_contract(uint32_t n, uint64_t m, const uint64_t* E, uint32_t tgt, uint32_t seed, uint32_t trial,
+                              uint32_t nid, uint64_t* m_out) {
+    race_item* it = (race_item*)malloc(sizeof(race_item) * (m ? m : 1));
+    uint32_t key[2] = {seed, DOM_SPARSE};
+    for (uint64_t e = 0; e < m; ++e) {
+        uint32_t ctr[4] = {trial, nid, (uint32_t)e, (uint32_t)(e >> 32)}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        it[e].key = orc_race_key(((uint64_t)o[1] << 32) | o[0], E[3 * e + 2]);
+        it[e].idx = e;
+    }
+    qsort(it, m, sizeof(race_item), cmp_race);
+    uint32_t* par = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (uint32_t v = 0; v < n; ++v) par[v] = v;
+    uint32_t comps = n;
+    for (uint64_t k = 0; k < m && comps > tgt; ++k) {
+        uint64_t e = it[k].idx;
+        uint32_t a = root_min(par, (uint32_t)E[3 * e]), b = root_min(par, (uint32_t)E[3 * e + 1]);
+        if (a == b) continue;
+        if (a < b) par[b] = a;
+        else par[a] = b;
+        --comps;
+    }
+    free(it);
+    if (comps > tgt) {
+        free(par);
+        return NULL;
+    }
+    uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t r = 0;
+    for (uint32_t v = 0; v < n; ++v)
+        if (root_min(par, v) == v) rank[v] = r++;
+    uint64_t* C = (uint64_t*)malloc(sizeof(uint64_t) * 3 * (m ? m : 1));
+    uint64_t mc = 0;
+    for (uint64_t e = 0; e < m; ++e) {
+        uint32_t a = rank[root_min(par, (uint32_t)E[3 * e])], b = rank[root_min(par, (uint32_t)E[3 * e + 1])];
+        if (a == b) continue;
+        C[3 * mc] = a < b ? a : b;
+        C[3 * mc + 1] = a < b ? b : a;
+        C[3 * mc + 2] = E[3 * e + 2];
+        ++mc;
+    }
+    free(par);
+    free(rank);
+    *m_out = mc;
+    return C;
+}
+
+static int64_t sparse_node(uint32_t n, uint64_t m, const uint64_t* E, uint32_t seed, uint32_t trial, uint32_t nid,
+                           int64_t bound) {
+    if (n <= SPARSE_DENSE_MAX || (m >= (uint64_t)n * n / 32 && n <= 8192)) {
+        int64_t* A = (int64_t*)calloc((size_t)n * n, sizeof(int64_t));
+        for (uint64_t e = 0; e < m; ++e) {
+            uint32_t a = (uint32_t)E[3 * e], b = (uint32_t)E[3 * e + 1];
+            A[(size_t)a * n + b] += (int64_t)E[3 * e + 2];
+            A[(size_t)b * n + a] += (int64_t)E[3 * e + 2];
+        }
+        int64_t v = ks_node(n, A, seed, trial, nid, bound);
+        free(A);
+        return v;
+    }
+    uint32_t tgt = orc_ks_target(n);
+    int64_t best = INT64_MAX;
+    for (uint32_t c = 0; c < 2; ++c) {
+        uint32_t cid = 2 * nid + c;
+        uint64_t mc = 0;
+        uint64_t* C = orc_sparse_contract(n, m, E, tgt, seed, trial, cid, &mc);
+        int64_t v = C ? sparse_node(tgt, mc, C, seed, trial, cid, bound) : 0;
+        free(C);
+        if (v < best) best = v;
+    }
+    return best;
+}
+
+int64_t orc_ks_sparse_trial(uint32_t n, uint64_t m, const uint64_t* E, uint32_t seed, uint32_t trial) {
+    if (n < 2) return 0;
+    int64_t* deg = (int64_t*)calloc(n, sizeof(int64_t));
+    for (uint64_t e = 0; e < m; ++e) {
+        deg[E[3 * e]] += (int64_t)E[3 * e + 2];
+        deg[E[3 * e + 1]] += (int64_t)E[3 * e + 2];
+    }
+    int64_t bound = INT64_MAX;
+    for (uint32_t v = 0; v < n; ++v) bound = deg[v] < bound ? deg[v] : bound;
+    free(deg);
+    int64_t v = sparse_node(n, m, E, seed, trial, 1, bound);
+    return v < bound ? v : bound;
+}

@@ -125,6 +125,24 @@ uint64_t orc_keep_threshold(uint32_t i, uint64_t w);
 void orc_approx_cut(uint32_t n, uint64_t m, const uint64_t* edges, double p, uint32_t seed, uint64_t* approx,
                     uint32_t* trials);
 
+/* ---- sparse Karger-Stein ---------------------------------------------------
+ * Restates the sparse recursion of comincut (co_mincut.h:74-155 contraction,
+ * :176-223 recursion, :280-293 sparse dispatch) with these rules of this build:
+ *   - the exponential race uses integer keys orc_race_key(x, w) from
+ *     Philox(ctr={trial, nid, e_lo, e_hi}, key={seed,'SPRS'}) word pair 0-1,
+ *     edges taken in (key, index) order until t' components remain;
+ *   - the child keeps the edges in their order, relabelled by component rank,
+ *     loops dropped, parallel edges not merged;
+ *   - a node with n <= 2048, or with m >= n^2/32 and n <= 8192, is densified and continues as the
+ *     dense recursion (orc_ks_trial's nodes) under the same node id;
+ *   - every node uses the trial's bound, the minimum weighted degree.
+ * orc_ks_sparse_trial returns min(bound, every leaf) of one trial.
+ * edges: m x (u, v, w), u < v. */
+uint64_t orc_race_key(uint64_t x, uint64_t w);
+uint64_t* orc_sparse_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t tgt, uint32_t seed,
+                              uint32_t trial, uint32_t nid, uint64_t* m_out);
+int64_t orc_ks_sparse_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t seed, uint32_t trial);
+
 #ifdef __cplusplus
 }
 #endif

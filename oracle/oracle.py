@@ -42,6 +42,9 @@ SIGS = {
     "orc_ks_first_pick": (None, [_u32, _vp, _u32, _u32, _u32, _vp, _vp]),
     "orc_keep_threshold": (_u64, [_u32, _u64]),
     "orc_approx_cut": (None, [_u32, _u64, _vp, ctypes.c_double, _u32, _vp, _vp]),
+    "orc_race_key": (_u64, [_u64, _u64]),
+    "orc_sparse_contract": (_vp, [_u32, _u64, _vp, _u32, _u32, _u32, _u32, _vp]),
+    "orc_ks_sparse_trial": (_i64, [_u32, _u64, _vp, _u32, _u32]),
 }
 
 _lib = None
@@ -179,6 +182,28 @@ def densify(n: int, edges, labels):
 def sqrt_trial(n: int, edges, t: int, seed: int, trial: int) -> int:
     e = _edges3(edges)
     return int(lib().orc_sqrt_trial(n, e.shape[0], _p(e), t, seed, trial))
+
+
+def race_key(x: int, w: int) -> int:
+    return int(lib().orc_race_key(x, w))
+
+
+def sparse_contract(n: int, edges, tgt: int, seed: int, trial: int, nid: int):
+    """Child edge list ((m', 3) uint64) of one sparse contraction, or None if disconnected."""
+    e = _edges3(edges)
+    mc = ctypes.c_uint64()
+    ptr = lib().orc_sparse_contract(n, e.shape[0], _p(e), tgt, seed, trial, nid, ctypes.byref(mc))
+    if not ptr:
+        return None
+    buf = (ctypes.c_uint64 * (3 * max(mc.value, 1))).from_address(ptr)
+    out = np.frombuffer(buf, dtype=np.uint64, count=3 * mc.value).reshape(-1, 3).copy()
+    ctypes.CDLL(None).free(ctypes.c_void_p(ptr))
+    return out
+
+
+def ks_sparse_trial(n: int, edges, seed: int, trial: int) -> int:
+    e = _edges3(edges)
+    return int(lib().orc_ks_sparse_trial(n, e.shape[0], _p(e), seed, trial))
 
 
 def keep_threshold(i: int, w: int) -> int:
