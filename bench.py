@@ -15,7 +15,9 @@ the metric, each on its own workload:
   c4  4096 planted-cut graphs, n log-uniform in [256, 65536], batched in one call
       vs the same kernels one graph per call;
   c5  served Poisson stream, 70% CC / 30% min cut, G(n, d/n), n log-uniform in
-      [10^3, 10^6], d uniform in [8, 64], weights U[1, 100].
+      [10^3, 10^6], d uniform in [8, 64], weights U[1, 100];
+  ks  karger_stein (sparse Karger-Stein, p = 0.90) on Watts-Strogatz graphs of
+      the reference's ws_32k_16 / ws_96k_16 shape, beside its published times.
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle's single-thread union-find on the same edge list).
@@ -51,8 +53,8 @@ def parse():
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC byte-counter scaling)")
-    ap.add_argument("--extras", default="c1,c3,c4,c5",
-                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5) or 'none'")
+    ap.add_argument("--extras", default="c1,c3,c4,c5,ks",
+                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks) or 'none'")
     ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
     ap.add_argument("--c4-graphs", type=int, default=4096)
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
@@ -179,6 +181,45 @@ def bench_c3(ctx, cac, trials: int):
             "roofline": {"bound": "hbm", "algorithmic_bytes_per_trial": bytes_per_trial,
                          "achieved_gbs": bytes_per_trial * trials / dt / 1e9, "peak": HBM_PEAK_GBS,
                          "frac": bytes_per_trial * trials / dt / 1e9 / HBM_PEAK_GBS}}
+
+
+# The reference's karger_stein (p = 0.90, one core) on its Watts-Strogatz inputs:
+# mean seconds over its runs, evaluation/data/e1/baselines.csv (ws_32k_16:
+# n = 32000, m = 512000; ws_96k_16: n = 96000, m = 1536000).
+PUBLISHED_KS_WS_S = {32000: 59.59, 96000: 224.9}
+
+
+def bench_ks(ctx, cac, sizes, cpu_budget_s: float):
+    """karger_stein (sparse recursion, p = 0.90) on Watts-Strogatz graphs shaped
+    like the reference's ws_{32k,96k}_16 inputs: ring lattice with 16 neighbours
+    a side, 30% rewired, weight 100."""
+    out = {}
+    for n in sizes:
+        e = cac.gen_ws(n, 16, 0)
+        ctx.karger_stein(e, n, 0.90, 1)  # warm-up: same graph, another seed (buffers reach full size)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        cut, T = ctx.karger_stein(e, n, 0.90, 0)
+        dt = time.perf_counter() - t0
+        deg = np.bincount(e["u"], e["w"], n) + np.bincount(e["v"], e["w"], n)
+        r = {"workload": f"karger_stein p=0.90 seed 0, Watts-Strogatz n={n}, 16 ring neighbours a side, "
+                         f"beta 0.3, weight 100, m={len(e)}",
+             "seconds": dt, "trials": T, "trials_per_s": T / dt, "cut": int(cut), "min_degree": int(deg.min()),
+             "published_reference_seconds": PUBLISHED_KS_WS_S.get(n),
+             "speedup_vs_published": PUBLISHED_KS_WS_S[n] / dt if n in PUBLISHED_KS_WS_S else None}
+        if cpu_budget_s > 0 and n == min(sizes):
+            import oracle  # the reference's own library (oracle/_ref): baseline only
+            if oracle.ref_available():
+                e3 = np.stack([e["u"], e["v"], e["w"]], axis=1).astype(np.uint64)
+                t0 = time.perf_counter()
+                oracle.ref_ks_mincut(n, e3, 0.0, 0)  # p = 0: exactly one trial
+                per_trial = time.perf_counter() - t0
+                r["cpu_baseline"] = {"value": 1.0 / per_trial, "unit": "trials/s", "cores": 1, "kind": "reference",
+                                     "sample": f"one trial of the reference's comincut::minimum_cut (p=0) on the "
+                                               f"same graph, {per_trial:.1f} s; x{T} trials = "
+                                               f"{per_trial * T:.0f} s"}
+        out[f"ws_{n // 1000}k_16"] = r
+    return out
 
 
 def bench_c4(ctx, cac, count: int, stride: int):
@@ -377,7 +418,8 @@ def main():
     for key, fn in (("c1", lambda: bench_c1(ctx, cac, min(args.cpu_baseline_seconds, 10.0))),
                     ("c3", lambda: bench_c3(ctx, cac, args.c3_trials)),
                     ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride)),
-                    ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate))):
+                    ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate)),
+                    ("ks", lambda: bench_ks(ctx, cac, (32000, 96000), min(args.cpu_baseline_seconds, 20.0)))):
         if key in wanted:
             t0 = time.perf_counter()
             extras[key] = fn()

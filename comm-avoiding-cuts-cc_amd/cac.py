@@ -95,6 +95,7 @@ SIGNATURES = {
     "cac_server_stats": (_i32, [_vp, _pu64, _pu64]),
     "cac_server_destroy": (_i32, [_vp]),
     "cac_gen_gnp": (_i32, [_u32, ctypes.c_double, _u32, _u32, _u32, _vp, _u64, _pu64]),
+    "cac_gen_ws": (_i32, [_u32, _u32, ctypes.c_double, _u32, _u64, _vp, _u64, _pu64]),
     "cac_gen_planted_sizes": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _pu32, _pu64]),
     "cac_gen_planted": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _u32, _vp, _vp, _u64, _pu64,
                                _pu32]),
@@ -499,6 +500,20 @@ def gen_planted(first: int, count: int, seed: int, nmin: int = 256, nmax: int = 
     if rc != CAC_OK:
         raise CacError(rc, "cac_gen_planted")
     return descs[:count], edges[: m.value], planted[:count]
+
+
+def gen_ws(n: int, d: int, seed: int, beta: float = 0.3, weight: int = 100) -> np.ndarray:
+    """Watts-Strogatz graph (see cac_gen_ws) as EDGE records, n * d edges."""
+    lib = load_library()
+    m = ctypes.c_uint64()
+    rc = lib.cac_gen_ws(n, d, beta, seed, weight, None, 0, ctypes.byref(m))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_ws")
+    out = np.empty(max(m.value, 1), dtype=EDGE)
+    rc = lib.cac_gen_ws(n, d, beta, seed, weight, _ptr(out), len(out), ctypes.byref(m))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_ws")
+    return out[: m.value]
 
 
 def gen_gnp(n: int, avg_degree: float, seed: int, stream: int, wmax: int = 100) -> np.ndarray:

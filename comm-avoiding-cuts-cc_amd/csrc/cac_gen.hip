@@ -5,6 +5,7 @@
 // graph is produced directly in HBM from a counter-based stream so a benchmark
 // input of 2^28 edges costs milliseconds instead of a 5 GB text parse.  Sorting
 // uses rocPRIM's radix sort: the generator is set-up work, not the measured path.
+#include <cmath>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -219,4 +220,27 @@ extern "C" int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_
     }
   *m_out = m;
   return (out && m > cap) ? CAC_ERANGE : CAC_OK;
+}
+
+extern "C" int cac_gen_ws(uint32_t n, uint32_t d, double beta, uint32_t seed, uint64_t weight, cac_edge* out,
+                          uint64_t cap, uint64_t* m_out) {
+  // ring lattice u -- u + j (j = 1..d), each edge rewired with probability beta
+  // to (u, x), x uniform over the other n - 1 vertices; draws from
+  // Philox(ctr = {u, j, 0, 0}, key = {seed, 'SWRL'}).  Host-side (input generation).
+  constexpr uint32_t kDomWs = 0x5357524Cu;  // 'SWRL'
+  if (!m_out || d == 0 || n <= 2 * (uint64_t)d || !(beta >= 0.0 && beta <= 1.0) || weight == 0) return CAC_EINVAL;
+  const uint64_t m = (uint64_t)n * d;
+  *m_out = m;
+  if (!out) return CAC_OK;
+  if (m > cap) return CAC_ERANGE;
+  const uint64_t thr = (uint64_t)std::llround(beta * 4294967296.0);
+  uint64_t i = 0;
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t j = 1; j <= d; ++j) {
+      const cac::u32x4 x = cac::philox4x32(u, j, 0u, 0u, seed, kDomWs);
+      uint32_t v = (uint32_t)(((uint64_t)u + j) % n);
+      if ((uint64_t)x.x < thr) v = (uint32_t)(((uint64_t)u + 1 + (((uint64_t)(n - 1) * x.y) >> 32)) % n);
+      out[i++] = cac_edge{u < v ? u : v, u < v ? v : u, weight};
+    }
+  return CAC_OK;
 }

@@ -337,6 +337,18 @@ int cac_gen_er(uint32_t n, uint32_t threshold, uint32_t seed, uint32_t wmax, cac
  * Host-side, single-threaded, reentrant. */
 int cac_gen_gnp(uint32_t n, double avg_degree, uint32_t seed, uint32_t stream, uint32_t wmax, cac_edge* out,
                 uint64_t cap, uint64_t* m_out);
+/* Watts-Strogatz small-world graph, the shape of the reference's sw_generator
+ * (src/executables/sw_generator.cpp: Boost small_world_iterator with k = 2 d
+ * neighbours, rewiring probability 0.3, every weight 100), from this library's
+ * own draws: vertex u links to u + j mod n for j = 1..d, and that edge is
+ * rewired with probability beta to (u, x), x uniform over the other n - 1
+ * vertices; edge (u, j) uses Philox-4x32-10(ctr = {u, j, 0, 0},
+ * key = {seed, 'SWRL'}): rewired iff word 0 < round(beta 2^32), x = u + 1 +
+ * floor((n - 1) word1 / 2^32) mod n.  Writes m = n d edges (min, max, weight)
+ * in (u, j) order; parallel edges possible.  out = NULL only reports *m_out.
+ * Needs n > 2 d.  Host-side. */
+int cac_gen_ws(uint32_t n, uint32_t d, double beta, uint32_t seed, uint64_t weight, cac_edge* out, uint64_t cap,
+               uint64_t* m_out);
 int cac_gen_planted_sizes(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
                           double avg_degree, uint32_t cmax, uint32_t* n_out, uint64_t* edge_bound);
 int cac_gen_planted(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax, double avg_degree,

@@ -1,4 +1,4 @@
-"""The square_root / parallel_cc / approx_cut command lines: same argv and CSV
+"""The square_root / parallel_cc / approx_cut / karger_stein command lines: same argv and CSV
 shape as the reference executables (src/executables/*.cpp).  Argument and file
 errors are checked on the CPU (they fail before any GPU call); the fixture runs
 need a GPU."""
@@ -22,7 +22,8 @@ def run(*args, check=False):
     return r
 
 
-@pytest.mark.parametrize("tool,nargs", [("square_root", 0), ("parallel_cc", 0), ("approx_cut", 0)])
+@pytest.mark.parametrize("tool,nargs", [("square_root", 0), ("parallel_cc", 0), ("approx_cut", 0),
+                                       ("karger_stein", 0)])
 def test_usage(tool, nargs):
     r = run(BIN / tool)
     assert r.returncode == 1 and r.stdout.startswith("Usage:")
@@ -41,6 +42,10 @@ def test_missing_and_malformed_files(tmp_path):
     r = run(BIN / "parallel_cc", short, "0")
     assert r.returncode == 1 and "fewer edge lines" in r.stderr
     r = run(BIN / "square_root", "zz", GOLD / "cycle_4.graph", "0")
+    assert r.returncode == 1
+    r = run(BIN / "karger_stein", tmp_path / "nope.graph", "0")
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = run(BIN / "karger_stein", GOLD / "cycle_4.graph", "x")
     assert r.returncode == 1
 
 
@@ -92,3 +97,29 @@ def test_approx_cut_cli_matches_oracle(orc):
 def test_click_mode():
     out = run(BIN / "square_root", "0.95", "CLICK", "60", "1", check=True).stdout.strip().split(",")
     assert out[0] == "CLICK" and int(out[3]) == 60 and int(out[4]) == 60 * 59 // 2 and int(out[9]) >= 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,cut", [("cycle_4.graph", 200), ("cycle_6.graph", 10000), ("cycle_10.graph", 10),
+                                      ("cycle_20.graph", 100), ("complete_4.graph", 85)])
+def test_karger_stein_fixtures(name, cut):
+    out = run(BIN / "karger_stein", GOLD / name, "5", check=True).stdout.strip().split(",")
+    assert len(out) == 9 and out[0].endswith(name) and out[1] == "5" and out[2] == "1"
+    assert out[6] == "?" and out[7] == "co-karger-stein" and int(out[8]) == cut
+    float(out[5])
+
+
+@pytest.mark.gpu
+def test_karger_stein_sparse_ws_matches_oracle(tmp_path, orc):
+    # Watts-Strogatz, n = 3000, 8 ring neighbours a side: the sparse recursion
+    import cac
+
+    n = 3000
+    e = cac.gen_ws(n, 8, 4)
+    E = np.stack([e["u"], e["v"], e["w"]], 1).astype(np.uint64)
+    f = tmp_path / "ws3000.in"
+    write_graph(f, n, E)
+    out = run(BIN / "karger_stein", f, "6", check=True).stdout.strip().split(",")
+    T = orc.ks_trials(n, 0.90)
+    assert int(out[3]) == n and int(out[4]) == len(E)
+    assert int(out[8]) == min(orc.ks_sparse_trial(n, E, 6, t) for t in range(T))

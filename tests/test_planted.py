@@ -88,3 +88,22 @@ def test_gnp_generator():
     counts = [len(cac.gen_gnp(400, 8.0, 11, s)) for s in range(40)]
     mean = 400 * 8.0 / 2
     assert abs(np.mean(counts) - mean) < 4 * np.sqrt(mean / 40)
+
+
+def test_ws_generator():
+    n, d = 2000, 8
+    e = cac.gen_ws(n, d, 5)
+    u, v = e["u"].astype(np.int64), e["v"].astype(np.int64)
+    assert len(e) == n * d and np.all(u < v) and np.all(v < n) and np.all(e["w"] == 100)
+    assert np.array_equal(e, cac.gen_ws(n, d, 5))
+    # beta = 0: the ring lattice, edge (u, j) joins u and u + j mod n
+    r = cac.gen_ws(n, d, 5, beta=0.0)
+    src = np.repeat(np.arange(n), d)
+    dst = (src + np.tile(np.arange(1, d + 1), n)) % n
+    assert np.array_equal(r["u"], np.minimum(src, dst)) and np.array_equal(r["v"], np.maximum(src, dst))
+    # beta = 0.3: about 30% of the lattice edges rewired to uniform endpoints
+    moved = (e["u"] != r["u"]) | (e["v"] != r["v"])
+    N = n * d
+    assert abs(moved.mean() - 0.3 * (1 - 1 / (n - 1))) < 4.5 * np.sqrt(0.21 / N)
+    k, _ = connected_components(coo_matrix((np.ones(N), (u, v)), shape=(n, n)), directed=False)
+    assert k == 1
