@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
     ap.add_argument("--c5-queries", type=int, default=200)
     ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
+    ap.add_argument("--ks-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the ks leg")
     return ap.parse_args()
 
 
@@ -419,7 +420,8 @@ def main():
                     ("c3", lambda: bench_c3(ctx, cac, args.c3_trials)),
                     ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride)),
                     ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate)),
-                    ("ks", lambda: bench_ks(ctx, cac, (32000, 96000), min(args.cpu_baseline_seconds, 20.0)))):
+                    ("ks", lambda: bench_ks(ctx, cac, [int(x) for x in args.ks_sizes.split(",")],
+                                                    min(args.cpu_baseline_seconds, 20.0)))):
         if key in wanted:
             t0 = time.perf_counter()
             extras[key] = fn()

@@ -86,6 +86,15 @@ int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes) {
   if (*buf) CAC_HIP(ctx, hipFree(*buf));
   *buf = nullptr;
   *cap = 0;
+  // 1/8 headroom, so a later call a little larger than this one does not
+  // reallocate (a large hipFree + hipMalloc costs far more than the memory)
+  const size_t want = bytes + bytes / 8;
+  if (hipMalloc(buf, want) == hipSuccess) {
+    *cap = want;
+    return CAC_OK;
+  }
+  (void)hipGetLastError();
+  *buf = nullptr;
   CAC_HIP(ctx, hipMalloc(buf, bytes));
   *cap = bytes;
   return CAC_OK;
@@ -138,6 +147,7 @@ int cac_ctx_destroy(cac_ctx* ctx) {
   for (void* b : ctx->sp_bufs)
     if (b) (void)hipFree(b);
   if (ctx->sp_dense_buf) (void)hipFree(ctx->sp_dense_buf);
+  if (ctx->sp_tmp) (void)hipFree(ctx->sp_tmp);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -36,10 +36,13 @@ struct cac_ctx {
   void* smp_buf = nullptr;
   size_t smp_bytes = 0;
   // Sparse Karger-Stein: one grow-only buffer per recursion depth (the children
-  // of a chunk of nodes stay alive while their subtree runs), and the dense
-  // matrices handed over to the dense engine.
+  // of a chunk of nodes stay alive while their subtree runs), the scratch of
+  // one contraction launch, and the dense matrices handed over to the dense
+  // engine.
   std::vector<void*> sp_bufs;
   std::vector<size_t> sp_bytes;
+  void* sp_tmp = nullptr;  // contraction scratch of one launch
+  size_t sp_tmp_bytes = 0;
   void* sp_dense_buf = nullptr;
   size_t sp_dense_bytes = 0;
 };

@@ -10,14 +10,15 @@
 // key race_key(x_e, w_e), x_e from Philox(ctr = {trial, node id, e_lo, e_hi},
 // key = {seed, 'SPRS'}), and edges are united in (key, index) order until t'
 // components remain.  One 1024-thread workgroup per child finds that prefix by
-// halving the candidate key range around a sampled median, as the reference
-// halves around nth_element; a trial union-find kept equal to the identity
-// between probes makes each probe cost only its candidate edges.  The child is
-// the node's edges in their order, relabelled by the rank of each component's
-// minimum id, loops dropped and parallel edges kept.  A node with n <= 2048, or
-// with m >= n^2/32 and n <= 8192, is densified and continues in the dense engine
-// under its own node id.  Every node uses the trial's bound, the minimum
-// weighted degree of the input.
+// narrowing a candidate key range, as the reference halves around nth_element:
+// the first cut aims at the quantile that should hold twice the unions still
+// needed, later cuts at the median, and a trial union-find kept equal to the
+// identity between probes makes each probe cost only its own edges.  The child
+// is the node's edges in their order, relabelled by the rank of each
+// component's minimum id, loops dropped and parallel edges kept.  A node with
+// n <= 2048, or with m >= n^2/32 and n <= 8192, is densified and continues in
+// the dense engine under its own node id.  Every node uses the trial's bound,
+// the minimum weighted degree of the input.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -40,8 +41,9 @@ constexpr unsigned long long kSpDisconnected = ~0ull;
 }  // namespace
 
 // floor(2^20 * L / w), L = -log2(U / 2^53) in Q32.32, U = (x >> 11) + 1.  The
-// fraction of log2 comes from bit-by-bit squaring of the mantissa (each square
-// truncated to its top 64 bits), so host and device agree exactly.
+// fraction of log2 comes from bit-by-bit squaring of the top 32 mantissa bits
+// (each square truncated to 32 bits, |error| < 2^-29), so host and device agree
+// exactly.
 __host__ __device__ inline uint64_t race_key(uint64_t x, uint64_t w) {
   const uint64_t U = (x >> 11) + 1;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -49,25 +51,17 @@ __host__ __device__ inline uint64_t race_key(uint64_t x, uint64_t w) {
 #else
   const int e = 63 - __builtin_clzll(U);
 #endif
-  uint64_t y = U << (63 - e);  // mantissa in [1, 2), Q1.63
-  uint64_t frac = 0;
+  uint32_t y = (uint32_t)((U << (63 - e)) >> 32);  // mantissa in [1, 2), Q1.31
+  uint32_t frac = 0;
+#pragma unroll 8
   for (int i = 0; i < 32; ++i) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const uint64_t hi = __umul64hi(y, y), lo = y * y;
-#else
-    const unsigned __int128 sq = (unsigned __int128)y * y;
-    const uint64_t hi = (uint64_t)(sq >> 64), lo = (uint64_t)sq;
-#endif
-    frac <<= 1;
-    if (hi >> 63) {
-      frac |= 1;
-      y = hi;
-    } else {
-      y = (hi << 1) | (lo >> 63);
-    }
+    const uint64_t sq = (uint64_t)y * y;  // Q2.62
+    const bool top = (sq >> 63) != 0;
+    frac = (frac << 1) | (top ? 1u : 0u);
+    y = top ? (uint32_t)(sq >> 32) : (uint32_t)(sq >> 31);
   }
   const uint64_t L = ((uint64_t)53 << 32) - (((uint64_t)e << 32) | frac);
-  return (L << 20) / (w ? w : 1);
+  return w == 1 ? (L << 20) : (L << 20) / (w ? w : 1);
 }
 
 namespace {
@@ -77,11 +71,11 @@ struct SpChild {
   uint64_t m;
   uint32_t n, tgt, trial, nid;  // nid: this child's node id (keys come from it)
   uint32_t slot, pad;
-  cac_edge* out;              // [m] child edges
-  uint64_t* keys;             // [m]
-  uint32_t* list[2];          // [m] candidate edge indices, ping-pong
-  uint32_t* parc;             // [n] committed forest
-  uint32_t* part;             // [n] trial forest, identity between probes; ranks at the end
+  cac_edge* out;              // [m] child edges (kept for the next depth)
+  uint64_t* keys;             // [m] scratch: race keys
+  uint32_t* list[2];          // [m] scratch: candidate edge indices, ping-pong
+  uint32_t* parc;             // [n] scratch: committed forest, then component ranks
+  uint32_t* part;             // [n] scratch: trial forest (identity between probes), then labels
   unsigned long long* m_out;  // child edge count, kSpDisconnected when disconnected
 };
 
@@ -152,6 +146,25 @@ __device__ inline uint64_t sp_exscan(uint64_t x, uint64_t* red, uint64_t* total)
   return before + inc - x;
 }
 
+// Position of this thread's item among the block's kept items of one
+// 1024-item tile, and the tile's kept count (order-preserving compaction).
+__device__ inline uint32_t sp_tile_rank(bool keep, uint32_t* wcnt, uint32_t* tile_total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t bal = __ballot(keep);
+  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(bal);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int w = 0; w < kSpWaves; ++w) {
+    const uint32_t c = wcnt[w];
+    off += w < wave ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  *tile_total = tot;
+  return off + below;
+}
+
 // Bitonic sort of kSpSort (key, idx) pairs in LDS, ascending by key then idx.
 __device__ inline void sp_sort(uint64_t* key, uint32_t* idx) {
   const uint32_t i = threadIdx.x;
@@ -175,25 +188,54 @@ __device__ inline void sp_sort(uint64_t* key, uint32_t* idx) {
   }
 }
 
-// Components removed by uniting, on top of the committed forest, the candidate
-// edges list[pos] with sel(pos, key) true.  The trial forest is restored to the
-// identity afterwards: every entry a probe writes is the committed root of an
-// endpoint of one of its edges, and those are reset.
-template <class Sel>
-__device__ uint32_t sp_probe(const SpChild& c, const uint32_t* list, uint64_t ns, Sel sel, uint64_t* red) {
+// Candidate edge at position pos: list[pos], or pos itself while the candidates
+// are still every edge (list == nullptr).
+__device__ inline uint32_t sp_at(const uint32_t* list, uint64_t pos) { return list ? list[pos] : (uint32_t)pos; }
+
+// Stable compaction of the candidates whose key satisfies keep into dst; returns
+// the count, and the smallest and largest kept key in *kmin, *kmax.
+template <class Keep>
+__device__ uint64_t sp_filter(const SpChild& c, const uint32_t* src, uint64_t ns, Keep keep, uint32_t* dst,
+                              uint64_t* kmin, uint64_t* kmax, uint32_t* wcnt, uint64_t* red) {
+  uint64_t base = 0, lo = ~0ull, hi = 0;
+  for (uint64_t t0 = 0; t0 < ns; t0 += kSpBlock) {
+    const uint64_t pos = t0 + threadIdx.x;
+    uint32_t e = 0;
+    bool k = false;
+    if (pos < ns) {
+      e = sp_at(src, pos);
+      const uint64_t key = c.keys[e];
+      k = keep(key);
+      if (k) {
+        lo = min(lo, key);
+        hi = max(hi, key);
+      }
+    }
+    uint32_t tot = 0;
+    const uint32_t r = sp_tile_rank(k, wcnt, &tot);
+    if (k) dst[base + r] = e;
+    base += tot;
+  }
+  *kmin = sp_min(lo, red);
+  *kmax = sp_max(hi, red);
+  block_sync_global();
+  return base;
+}
+
+// Components removed by uniting, on top of the committed forest, the first p
+// candidates.  The trial forest is restored to the identity afterwards: every
+// entry a probe writes is the committed root of an endpoint of one of its
+// edges, and those are reset.
+__device__ uint32_t sp_probe(const SpChild& c, const uint32_t* list, uint64_t p, uint64_t* red) {
   uint64_t merged = 0;
-  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
-    const uint32_t e = list[pos];
-    if (!sel(pos, c.keys[e])) continue;
-    const cac_edge x = c.edges[e];
+  for (uint64_t pos = threadIdx.x; pos < p; pos += kSpBlock) {
+    const cac_edge x = c.edges[sp_at(list, pos)];
     const uint32_t a = sp_root(c.parc, x.u), b = sp_root(c.parc, x.v);
     if (a != b && sp_union(c.part, a, b)) ++merged;
   }
   block_sync_global();
-  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
-    const uint32_t e = list[pos];
-    if (!sel(pos, c.keys[e])) continue;
-    const cac_edge x = c.edges[e];
+  for (uint64_t pos = threadIdx.x; pos < p; pos += kSpBlock) {
+    const cac_edge x = c.edges[sp_at(list, pos)];
     const uint32_t a = sp_root(c.parc, x.u), b = sp_root(c.parc, x.v);
     c.part[a] = a;
     c.part[b] = b;
@@ -203,42 +245,39 @@ __device__ uint32_t sp_probe(const SpChild& c, const uint32_t* list, uint64_t ns
   return (uint32_t)total;
 }
 
-// Unites the selected candidate edges into the committed forest.
-template <class Sel>
-__device__ void sp_commit(const SpChild& c, const uint32_t* list, uint64_t ns, Sel sel) {
-  for (uint64_t pos = threadIdx.x; pos < ns; pos += kSpBlock) {
-    const uint32_t e = list[pos];
-    if (!sel(pos, c.keys[e])) continue;
-    const cac_edge x = c.edges[e];
+// Unites the first p candidates into the committed forest.
+__device__ void sp_commit(const SpChild& c, const uint32_t* list, uint64_t p) {
+  for (uint64_t pos = threadIdx.x; pos < p; pos += kSpBlock) {
+    const cac_edge x = c.edges[sp_at(list, pos)];
     sp_union(c.parc, x.u, x.v);
   }
   block_sync_global();
 }
 
-struct SelBelow {  // key <= tau
+struct KeepBelow {  // key <= tau
   uint64_t tau;
-  __device__ bool operator()(uint64_t, uint64_t k) const { return k <= tau; }
+  __device__ bool operator()(uint64_t k) const { return k <= tau; }
 };
-struct SelPrefix {  // the first p candidates
-  uint64_t p;
-  __device__ bool operator()(uint64_t pos, uint64_t) const { return pos < p; }
+struct KeepAbove {  // key > tau
+  uint64_t tau;
+  __device__ bool operator()(uint64_t k) const { return k > tau; }
 };
 
-__global__ void __launch_bounds__(kSpBlock) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
+__global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
                                                               int64_t* __restrict__ results) {
   __shared__ uint64_t red[kSpWaves];
+  __shared__ uint32_t wcnt[kSpWaves];
   __shared__ uint64_t skey[kSpSort];
   __shared__ uint32_t sidx[kSpSort];
   const SpChild c = children[blockIdx.x];
   const uint32_t tid = threadIdx.x;
 
-  // 1. race keys, identity forests, every edge a candidate
+  // 1. race keys, identity forests
   uint64_t kmin = ~0ull, kmax = 0;
   for (uint64_t e = tid; e < c.m; e += kSpBlock) {
     const u32x4 r = philox4x32(c.trial, c.nid, (uint32_t)e, (uint32_t)(e >> 32), seed, kDomSparse);
     const uint64_t k = race_key(lo64(r), c.edges[e].w);
     c.keys[e] = k;
-    c.list[0][e] = (uint32_t)e;
     kmin = min(kmin, k);
     kmax = max(kmax, k);
   }
@@ -251,65 +290,60 @@ __global__ void __launch_bounds__(kSpBlock) k_sparse_contract(const SpChild* __r
   block_sync_global();
 
   if (c.n > c.tgt) {  // else nothing is contracted
-    // 2. halving: the committed forest holds the edges with key <= lo, the
-    //    candidates are the edges with key in (lo, hi]; components > tgt always
-    uint32_t* cur = c.list[0];
-    uint32_t* nxt = c.list[1];
+    // 2. narrowing: the committed forest holds the edges with key <= lo, the
+    //    candidates are the edges with key in (lo, hi]; components > tgt always.
+    //    cur == nullptr: the candidates are every edge, in index order.
+    const uint32_t* cur = nullptr;
+    int nxt_i = 0;
     uint64_t ns = c.m;
     uint32_t comps = c.n;
+    uint64_t rho = 2;  // the next cut aims at rho x (unions still needed) edges
     while (ns > kSpSort && kmin < kmax) {
-      skey[tid] = c.keys[cur[(uint64_t)tid * ns / kSpSort]];
+      skey[tid] = c.keys[sp_at(cur, (uint64_t)tid * ns / kSpSort)];
       sidx[tid] = tid;
       __syncthreads();
       sp_sort(skey, sidx);
-      const uint64_t tau = min(max(skey[kSpSort / 2 - 1], kmin), kmax - 1);
+      const uint64_t want = (rho * (comps - c.tgt) * kSpSort + ns - 1) / ns;  // samples below the cut
+      const uint32_t k = (uint32_t)max<uint64_t>(1, min<uint64_t>(want, kSpSort / 2)) - 1;
+      const uint64_t tau = min(max(skey[k], kmin), kmax - 1);
       __syncthreads();
-      const SelBelow below{tau};
-      const uint32_t merged = sp_probe(c, cur, ns, below, red);
-      const bool commit = comps - merged > c.tgt;
-      if (commit) {
-        sp_commit(c, cur, ns, below);
+      uint32_t* nxt = nxt_i ? c.list[1] : c.list[0];
+      uint64_t xmin, xmax;
+      const uint64_t nx = sp_filter(c, cur, ns, KeepBelow{tau}, nxt, &xmin, &xmax, wcnt, red);
+      const uint32_t merged = sp_probe(c, nxt, nx, red);
+      if (comps - merged > c.tgt) {  // not enough: keep these unions, continue above tau
+        sp_commit(c, nxt, nx);
         comps -= merged;
+        uint64_t rmin, rmax;
+        ns = sp_filter(c, cur, ns, KeepAbove{tau}, nxt, &rmin, &rmax, wcnt, red);
+        kmin = rmin;
+        kmax = rmax;
+        rho = min<uint64_t>(rho * 2, 1u << 20);
+      } else {
+        ns = nx;
+        kmin = xmin;
+        kmax = xmax;
       }
-      // keep the half that still holds the boundary, in list (= index) order
-      const uint64_t chunk = (ns + kSpBlock - 1) / kSpBlock;
-      const uint64_t p0 = min(ns, (uint64_t)tid * chunk), p1 = min(ns, p0 + chunk);
-      uint64_t cnt = 0;
-      for (uint64_t pos = p0; pos < p1; ++pos) cnt += (c.keys[cur[pos]] <= tau) != commit;
-      uint64_t total = 0;
-      uint64_t off = sp_exscan(cnt, red, &total);
-      uint64_t nmin = ~0ull, nmax = 0;
-      for (uint64_t pos = p0; pos < p1; ++pos) {
-        const uint32_t e = cur[pos];
-        const uint64_t k = c.keys[e];
-        if ((k <= tau) == commit) continue;
-        nxt[off++] = e;
-        nmin = min(nmin, k);
-        nmax = max(nmax, k);
-      }
-      kmin = sp_min(nmin, red);
-      kmax = sp_max(nmax, red);
-      ns = total;
-      uint32_t* t = cur;
       cur = nxt;
-      nxt = t;
-      block_sync_global();
+      nxt_i ^= 1;
     }
 
     // 3. candidates in (key, index) order: sorted in LDS when few, else they all
-    //    share one key and the list is already in index order
+    //    share one key and are already in index order
     if (ns <= kSpSort) {
       const bool have = tid < ns;
-      const uint32_t e = have ? cur[tid] : 0xFFFFFFFFu;
+      const uint32_t e = have ? sp_at(cur, tid) : 0xFFFFFFFFu;
       skey[tid] = have ? c.keys[e] : ~0ull;
       sidx[tid] = e;
       __syncthreads();
       sp_sort(skey, sidx);
-      if (have) cur[tid] = sidx[tid];
+      uint32_t* dst = nxt_i ? c.list[1] : c.list[0];
+      if (have) dst[tid] = sidx[tid];
+      cur = dst;
       block_sync_global();
     }
     // shortest prefix of the candidates that leaves tgt components
-    if (comps - sp_probe(c, cur, ns, SelPrefix{ns}, red) > c.tgt) {
+    if (comps - sp_probe(c, cur, ns, red) > c.tgt) {
       if (tid == 0) {
         *c.m_out = kSpDisconnected;
         atomicMin((long long*)&results[c.slot], 0ll);
@@ -319,37 +353,47 @@ __global__ void __launch_bounds__(kSpBlock) k_sparse_contract(const SpChild* __r
     uint64_t lo = 0, hi = ns;  // comps(lo) > tgt >= comps(hi)
     while (hi - lo > 1) {
       const uint64_t mid = lo + (hi - lo) / 2;
-      if (comps - sp_probe(c, cur, ns, SelPrefix{mid}, red) <= c.tgt) hi = mid;
+      if (comps - sp_probe(c, cur, mid, red) <= c.tgt) hi = mid;
       else lo = mid;
     }
-    sp_commit(c, cur, ns, SelPrefix{hi});
+    sp_commit(c, cur, hi);
   }
 
-  // 4. rank of each component's minimum id (the root), in id order
+  // 4. labels: part[v] = root, parc[root] = rank of the root (minimum id) in id
+  //    order, then part[v] = rank of v's root
+  for (uint32_t v = tid; v < c.n; v += kSpBlock) c.part[v] = sp_root(c.parc, v);
+  block_sync_global();
   const uint64_t vc = ((uint64_t)c.n + kSpBlock - 1) / kSpBlock;
   const uint32_t v0 = (uint32_t)min((uint64_t)c.n, tid * vc), v1 = (uint32_t)min((uint64_t)c.n, v0 + vc);
   uint64_t cnt = 0;
-  for (uint32_t v = v0; v < v1; ++v) cnt += sp_root(c.parc, v) == v;
+  for (uint32_t v = v0; v < v1; ++v) cnt += c.part[v] == v;
   uint64_t total = 0;
   uint64_t off = sp_exscan(cnt, red, &total);
   for (uint32_t v = v0; v < v1; ++v)
-    if (sp_root(c.parc, v) == v) c.part[v] = (uint32_t)off++;
+    if (c.part[v] == v) c.parc[v] = (uint32_t)off++;
+  block_sync_global();
+  for (uint32_t v = tid; v < c.n; v += kSpBlock) c.part[v] = c.parc[c.part[v]];
   block_sync_global();
 
   // 5. the child: edges in parent order, relabelled, loops dropped
-  const uint64_t ec = (c.m + kSpBlock - 1) / kSpBlock;
-  const uint64_t e0 = min(c.m, (uint64_t)tid * ec), e1 = min(c.m, e0 + ec);
-  cnt = 0;
-  for (uint64_t e = e0; e < e1; ++e) cnt += sp_root(c.parc, c.edges[e].u) != sp_root(c.parc, c.edges[e].v);
-  off = sp_exscan(cnt, red, &total);
-  for (uint64_t e = e0; e < e1; ++e) {
-    const cac_edge x = c.edges[e];
-    const uint32_t ra = sp_root(c.parc, x.u), rb = sp_root(c.parc, x.v);
-    if (ra == rb) continue;
-    const uint32_t a = c.part[ra], b = c.part[rb];
-    c.out[off++] = cac_edge{min(a, b), max(a, b), x.w};
+  uint64_t base = 0;
+  for (uint64_t t0 = 0; t0 < c.m; t0 += kSpBlock) {
+    const uint64_t e = t0 + tid;
+    cac_edge x{0, 0, 0};
+    bool keep = false;
+    if (e < c.m) {
+      x = c.edges[e];
+      const uint32_t a = c.part[x.u], b = c.part[x.v];
+      keep = a != b;
+      x.u = min(a, b);
+      x.v = max(a, b);
+    }
+    uint32_t tot = 0;
+    const uint32_t r = sp_tile_rank(keep, wcnt, &tot);
+    if (keep) c.out[base + r] = x;
+    base += tot;
   }
-  if (tid == 0) *c.m_out = total;
+  if (tid == 0) *c.m_out = base;
 }
 
 // Handover to the dense engine: n x n matrix and row sums from an edge list.
@@ -361,10 +405,15 @@ struct SpDense {
   int64_t* rsum;
 };
 
+// Zeroes every handed-over matrix with 16-byte stores (n x n int64 is a whole
+// number of 16-byte words when n is even; an odd tail word is stored alone).
 __global__ void __launch_bounds__(256) k_sp_zero(const SpDense* __restrict__ nodes) {
   const SpDense d = nodes[blockIdx.y];
   const uint64_t nn = (uint64_t)d.n * d.n;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nn; i += (uint64_t)gridDim.x * 256) d.mat[i] = 0;
+  ulonglong2* m2 = (ulonglong2*)d.mat;
+  const ulonglong2 z{0ull, 0ull};
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nn / 2; i += (uint64_t)gridDim.x * 256) m2[i] = z;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (nn & 1)) d.mat[nn - 1] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_sp_scatter(const SpDense* __restrict__ nodes) {
@@ -376,13 +425,14 @@ __global__ void __launch_bounds__(256) k_sp_scatter(const SpDense* __restrict__ 
   }
 }
 
-// One wavefront per row.
+// Row sums (the weighted degrees), one wavefront per row.
 __global__ void __launch_bounds__(256) k_sp_rowsum(const SpDense* __restrict__ nodes) {
   const SpDense d = nodes[blockIdx.y];
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < d.n; r += gridDim.x * 4) {
+    const int64_t* row = d.mat + (uint64_t)r * d.n;
     uint64_t acc = 0;
-    for (uint32_t x = lane; x < d.n; x += 64) acc += (uint64_t)d.mat[(uint64_t)r * d.n + x];
+    for (uint32_t x = lane; x < d.n; x += 64) acc += (uint64_t)row[x];
     acc = wave_sum_u64(acc);
     if (lane == 0) d.rsum[r] = (int64_t)acc;
   }
@@ -448,9 +498,11 @@ struct SpNode {
   uint32_t n, trial, nid, slot;
 };
 
-size_t child_bytes(const SpNode& nd) {
-  return carve_size(nd.m * sizeof(cac_edge)) + carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) +
-         2 * carve_size((size_t)nd.n * 4) + sizeof(SpChild) + 8;
+// Device bytes of one child: what stays alive for the next depth (its edge
+// list, descriptor, count) and the scratch it needs only while it contracts.
+size_t child_keep_bytes(const SpNode& nd) { return carve_size(nd.m * sizeof(cac_edge)) + sizeof(SpChild) + 8; }
+size_t child_tmp_bytes(const SpNode& nd) {
+  return carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) + 2 * carve_size((size_t)nd.n * 4);
 }
 size_t dense_bytes(const SpNode& nd) {
   return carve_size((size_t)nd.n * nd.n * 8) + carve_size((size_t)nd.n * 8) + sizeof(SpDense);
@@ -462,7 +514,8 @@ struct SpRun {
   int64_t bound;
   int64_t* d_results;
   hipStream_t s;
-  size_t depth_budget;
+  size_t depth_budget;  // bytes of children kept alive per depth
+  size_t tmp_budget;    // bytes of contraction scratch per launch
 
   // Densifies nodes in batches that fit the free memory (matrices plus the
   // dense engine's level buffers) and runs the dense engine on them.
@@ -501,7 +554,7 @@ struct SpRun {
         mmax = std::max(mmax, nd.m);
       }
       CAC_HIP(ctx, hipMemcpyAsync(d_nodes, hd.data(), count * sizeof(SpDense), hipMemcpyHostToDevice, s));
-      const unsigned gz = grid_for((uint64_t)nmax * nmax, 256 * 16, 1024);
+      const unsigned gz = grid_for((uint64_t)nmax * nmax / 2, 256 * 16, 1024);
       const unsigned gs = grid_for(mmax, 256 * 4, 1024);
       const unsigned gr = grid_for(nmax, 4, 1024);
       prof_begin(ctx, "sparse_densify", s);
@@ -534,17 +587,21 @@ struct SpRun {
     }
     size_t i0 = 0;
     while (i0 < sp.size()) {
-      size_t i1 = i0, bytes = 1024;
+      size_t i1 = i0, bytes = 1024, tmp = 1024;
       while (i1 < sp.size() && i1 - i0 < 8192) {
-        const size_t b = 2 * child_bytes(sp[i1]);
-        if (i1 > i0 && bytes + b > depth_budget) break;
+        const size_t b = 2 * child_keep_bytes(sp[i1]), t = 2 * child_tmp_bytes(sp[i1]);
+        if (i1 > i0 && (bytes + b > depth_budget || tmp + t > tmp_budget)) break;
         bytes += b;
+        tmp += t;
         ++i1;
       }
       const size_t count = i1 - i0, nchild = 2 * count;
       int rc = grow_buffer(ctx, &ctx->sp_bufs[depth], &ctx->sp_bytes[depth], bytes);
       if (rc) return rc;
+      rc = grow_buffer(ctx, &ctx->sp_tmp, &ctx->sp_tmp_bytes, tmp);
+      if (rc) return rc;
       Carver cv(ctx->sp_bufs[depth]);
+      Carver ct(ctx->sp_tmp);
       SpChild* d_children = cv.take<SpChild>(nchild);
       unsigned long long* d_mout = cv.take<unsigned long long>(nchild);
       std::vector<SpChild> ch(nchild);
@@ -561,11 +618,11 @@ struct SpRun {
           x.slot = nd.slot;
           x.pad = 0;
           x.out = cv.take<cac_edge>(nd.m);
-          x.keys = cv.take<uint64_t>(nd.m);
-          x.list[0] = cv.take<uint32_t>(nd.m);
-          x.list[1] = cv.take<uint32_t>(nd.m);
-          x.parc = cv.take<uint32_t>(nd.n);
-          x.part = cv.take<uint32_t>(nd.n);
+          x.keys = ct.take<uint64_t>(nd.m);
+          x.list[0] = ct.take<uint32_t>(nd.m);
+          x.list[1] = ct.take<uint32_t>(nd.m);
+          x.parc = ct.take<uint32_t>(nd.n);
+          x.part = ct.take<uint32_t>(nd.n);
           x.m_out = d_mout + 2 * i + k;
         }
       }
@@ -591,10 +648,10 @@ struct SpRun {
   }
 };
 
-// Depth of the sparse recursion in the worst case (no edge ever lost).
-size_t sparse_depth(uint32_t n, uint64_t m) {
+// Depth of the sparse recursion at most: until the node size alone hands over.
+size_t sparse_depth(uint32_t n) {
   size_t d = 0;
-  for (uint32_t s = n; !hands_over(s, m); s = ks_target_host(s)) ++d;
+  for (uint32_t s = n; s > kSparseDenseMax; s = ks_target_host(s)) ++d;
   return d;
 }
 
@@ -606,9 +663,10 @@ int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n,
   int rc = device_memory_available(ctx, &avail);
   if (rc) return rc;
   for (size_t i = 0; i < ctx->sp_bufs.size(); ++i) avail += ctx->sp_bytes[i];
-  avail += ctx->sp_dense_bytes;
-  const size_t depth = sparse_depth(n, m);
-  SpRun run{ctx, seed, bound, d_results, s, (size_t)(0.45 * (double)avail) / (depth + 1)};
+  avail += ctx->sp_dense_bytes + ctx->sp_tmp_bytes;
+  const size_t depth = sparse_depth(n);
+  SpRun run{ctx, seed, bound, d_results, s, (size_t)(0.4 * (double)avail) / (depth + 1),
+            (size_t)(0.15 * (double)avail)};
   std::vector<SpNode> roots;
   for (size_t i = 0; i < trial_ids.size(); ++i) roots.push_back({d_edges, m, n, trial_ids[i], 1u, (uint32_t)i});
   rc = run.level(roots, 0);

@@ -629,23 +629,23 @@ void orc_approx_cut(uint32_t n, uint64_t m, const uint64_t* edges, double p, uin
 
 /* Exponential-race key of an edge of weight w from a 64-bit word x:
  * floor(2^20 * L / w), L = -log2(U / 2^53) in Q32.32, U = (x >> 11) + 1.
- * log2 by the classical bit-by-bit squaring of the mantissa (32 steps, each
- * squaring truncated to 64 bits), so CPU and GPU agree exactly.  Ordering edges
- * by (key, index) is, up to the fixed point, ordering by Exp(w) variables:
- * the exponential race of co_mincut.h:141-144. */
+ * log2 by the classical bit-by-bit squaring of the top 32 mantissa bits (32
+ * steps, each square truncated to 32 bits; |error| < 2^-29), so CPU and GPU
+ * agree exactly.  Ordering edges by (key, index) is, up to the fixed point,
+ * ordering by Exp(w) variables: the exponential race of co_mincut.h:141-144. */
 uint64_t orc_race_key(uint64_t x, uint64_t w) {
     uint64_t U = (x >> 11) + 1;
     int e = 63 - __builtin_clzll(U);
-    uint64_t y = U << (63 - e); /* mantissa in [1, 2), Q1.63 */
+    uint32_t y = (uint32_t)((U << (63 - e)) >> 32); /* mantissa in [1, 2), Q1.31 */
     uint64_t frac = 0;
     for (int i = 0; i < 32; ++i) {
-        __uint128_t sq = (__uint128_t)y * y;
+        uint64_t sq = (uint64_t)y * y; /* Q2.62 */
         frac <<= 1;
-        if ((uint64_t)(sq >> 127)) {
+        if (sq >> 63) {
             frac |= 1;
-            y = (uint64_t)(sq >> 64);
+            y = (uint32_t)(sq >> 32);
         } else {
-            y = (uint64_t)(sq >> 63);
+            y = (uint32_t)(sq >> 31);
         }
     }
     uint64_t log2u = ((uint64_t)e << 32) | frac;
