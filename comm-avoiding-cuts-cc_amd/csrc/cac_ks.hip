@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
 // wslot[s] u32 | red[B/64] u64 | misc[8] u32.
 __host__ __device__ inline size_t contract_lds_bytes(uint32_t s, int B) {
   const size_t nb = (s + 63) / 64;
-  return 8 * (size_t)s + 8 * nb + 8 * (size_t)s + 8 * (size_t)(B / 64) + 32;
+  return 8 * (size_t)s + 8 * nb + 8 * (size_t)s + 8 * (size_t)(kKsChunk + 1) * (B / 64) + 32;
 }
 
 struct KsLds {
@@ -265,9 +265,14 @@ __device__ inline const int64_t* ks_row(const KsChild& c, const KsPool& pool, ui
   return slot == kNoSlot ? c.parent + (size_t)r * c.s : pool.base + (size_t)slot * kKsUnit;
 }
 
-__device__ inline void load_chunk(const int64_t* row, uint32_t lo, uint32_t cc, uint32_t s, int64_t (&v)[kKsChunk]) {
+// This thread's entries x = k * B + t of a row (0 past s).
+template <int B>
+__device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int64_t (&v)[kKsChunk]) {
 #pragma unroll
-  for (uint32_t k = 0; k < kKsChunk; ++k) v[k] = (k < cc && lo + k < s) ? row[lo + k] : 0;
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    const uint32_t x = k * B + threadIdx.x;
+    v[k] = (k < cc && x < s) ? row[x] : 0;
+  }
 }
 
 // Merge row a into row b (a dies, b keeps its index).  The calling threads hold
@@ -310,13 +315,12 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
   __syncthreads();
   const int64_t* ro = ks_row(c, pool, so, other);
   int64_t* dst = pool.base + (size_t)db * kKsUnit;
-  const uint32_t lo = threadIdx.x * cc;
   int64_t u[kKsChunk];
-  load_chunk(ro, lo, cc, s, u);
+  load_row<B>(ro, cc, s, u);
   uint64_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
-    const uint32_t x = lo + k;
+    const uint32_t x = k * B + threadIdx.x;
     if (k < cc && x < s) {
       int64_t t = v[k] + u[k];
       if (L.rep[x] == b) t = 0;  // loop of the merged vertex
@@ -351,7 +355,7 @@ __global__ void __launch_bounds__(B) k_ks_contract(const KsChild* __restrict__ c
   L.rep = (uint32_t*)(smem + 8 * (size_t)s + 8 * nb);
   L.wslot = L.rep + s;
   L.red = (uint64_t*)(smem + 16 * (size_t)s + 8 * nb);
-  L.misc = (uint32_t*)(L.red + B / 64);
+  L.misc = (uint32_t*)(L.red + (kKsChunk + 1) * (B / 64));
 
   for (uint32_t x = threadIdx.x; x < s; x += B) {
     L.cap[x] = c.prsum[x];
@@ -389,8 +393,8 @@ __global__ void __launch_bounds__(B) k_ks_contract(const KsChild* __restrict__ c
       status = 2;
       break;
     }
-    load_chunk(ks_row(c, pool, L.wslot[i], i), lo, cc, s, v);
-    const uint32_t k = block_select_chunk<B>(v, cc, s, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1);
+    load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
+    const uint32_t k = block_select_row<B>(v, cc, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1);
     if (k >= s) {
       status = 2;
       break;
@@ -410,11 +414,11 @@ __global__ void __launch_bounds__(B) k_ks_contract(const KsChild* __restrict__ c
   if (status == 0 && pr1 > 0 && c.bound > 0) {
     for (uint32_t i = 0; i < s && nv > 2; ++i) {
       if (L.rep[i] != i || L.cap[i] < c.bound) continue;
-      load_chunk(ks_row(c, pool, L.wslot[i], i), lo, cc, s, v);
+      load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
       uint32_t first = kNoSlot;
 #pragma unroll
       for (uint32_t k = 0; k < kKsChunk; ++k)
-        if (first == kNoSlot && k < cc && lo + k < s && v[k] >= c.bound) first = lo + k;
+        if (first == kNoSlot && k < cc && k * B + threadIdx.x < s && v[k] >= c.bound) first = k * B + threadIdx.x;
       first = block_min_u32<B>(first, (uint32_t*)L.red);
       if (first == kNoSlot) continue;
       int64_t w = 0;

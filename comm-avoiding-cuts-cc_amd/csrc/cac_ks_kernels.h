@@ -8,9 +8,11 @@
 //     pool (copy on write, the row's pool offset kept in LDS);
 //   - capacities live in LDS with 64-entry block sums, so the row pick is a
 //     two-level search by one wavefront;
-//   - every thread owns the same contiguous chunk of every row for the whole
-//     kernel: the picked row stays in registers for the column pick and the merge,
-//     and a thread only ever reads back what it wrote itself;
+//   - every thread owns the same entries of every row for the whole kernel,
+//     interleaved (thread t of B owns x = k B + t, k < 8), so each load or store
+//     instruction of a wavefront covers 512 contiguous bytes of a row; the picked
+//     row stays in registers for the column pick and the merge, and a thread
+//     only ever reads back what it wrote itself;
 //   - a PR pass (co_mincut.h:50-63) may follow; the kernel ends by writing the
 //     column map and the list of live rows, and a separate many-workgroup kernel
 //     compacts the live rows into the child's t x t matrix with its row sums and
@@ -28,7 +30,7 @@ namespace cac {
 
 constexpr uint32_t kKsBase = 128;        // base_case_size (co_mincut.h:24)
 constexpr uint32_t kKsMaxS = 8192;       // LDS holds caps, reps and row slots for s <= this
-constexpr uint32_t kKsChunk = 8;         // row entries per thread (s <= 8 * block size)
+constexpr uint32_t kKsChunk = 8;         // row entries per thread (s <= 8 * block size), x = k * B + t
 constexpr uint32_t kKsUnit = 64;         // pool allocation unit: 64 int64 (512 bytes)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kLiveWork = 0x80000000u;  // live[] entry: row taken from the pool
@@ -141,40 +143,50 @@ __device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint6
   return blk * 64 + (uint32_t)(__ffsll((long long)hit2) - 1);
 }
 
-// First index of a row whose inclusive prefix reaches r, with the row held in
-// registers: thread t owns entries [t*cc, t*cc + cc).  One block scan of the
-// chunk sums, then the owning thread walks its chunk.  kNoSlot if none.
+// First index (natural column order) of a row whose inclusive prefix reaches r
+// (1 <= r), with the row held in registers interleaved: thread t owns entries
+// x = k * B + t, k < cc, and holds 0 for x >= s.  Natural order is segment k
+// by segment k, so: the eight segment totals (wave sums, one barrier), the
+// segment where the prefix reaches r, then one block scan inside it.  lds:
+// (kKsChunk + 1) * B/64 u64.  kNoSlot if the row total is below r.
 template <int B>
-__device__ inline uint32_t block_select_chunk(const int64_t (&v)[kKsChunk], uint32_t cc, uint32_t s, uint64_t r,
-                                              uint64_t* lds_u64, uint32_t* lds_idx) {
+__device__ inline uint32_t block_select_row(const int64_t (&v)[kKsChunk], uint32_t cc, uint64_t r, uint64_t* lds,
+                                            uint32_t* lds_idx) {
+  constexpr int W = B / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t lo = threadIdx.x * cc;
   if (threadIdx.x == 0) *lds_idx = kNoSlot;
-  uint64_t sum = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kKsChunk; ++k)
-    if (k < cc && lo + k < s) sum += (uint64_t)v[k];
-  const uint64_t inc = wave_incl_scan_u64(sum, lane);
-  if (B > 64) {
-    if (lane == 63) lds_u64[wave] = inc;
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    if (k < cc) {
+      const uint64_t t = wave_sum_u64((uint64_t)v[k]);
+      if (lane == 0) lds[k * W + wave] = t;
+    }
   }
   __syncthreads();
-  uint64_t before = 0;
-  if (B > 64)
-    for (int w = 0; w < wave; ++w) before += lds_u64[w];
-  const uint64_t ex = before + inc - sum;
-  if (sum != 0 && ex < r && r <= ex + sum) {
-    uint64_t acc = ex;
-    uint32_t found = kNoSlot;
+  uint64_t before = 0, val = 0;
+  uint32_t ks = kNoSlot;
 #pragma unroll
-    for (uint32_t k = 0; k < kKsChunk; ++k) {
-      if (k < cc && lo + k < s) {
-        acc += (uint64_t)v[k];
-        if (found == kNoSlot && v[k] != 0 && acc >= r) found = lo + k;
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    if (k < cc && ks == kNoSlot) {
+      uint64_t tot = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) tot += lds[k * W + w];
+      if (before + tot >= r) {
+        ks = k;
+        val = (uint64_t)v[k];
+      } else {
+        before += tot;
       }
     }
-    *lds_idx = found;
   }
+  if (ks == kNoSlot) return kNoSlot;  // the same for every thread
+  const uint64_t inc = wave_incl_scan_u64(val, lane);
+  if (lane == 63) lds[kKsChunk * W + wave] = inc;
+  __syncthreads();
+  uint64_t wb = 0;
+  for (int w = 0; w < wave; ++w) wb += lds[kKsChunk * W + w];
+  const uint64_t ex = before + wb + inc - val;
+  if (val != 0 && ex < r && r <= ex + val) *lds_idx = ks * B + threadIdx.x;
   __syncthreads();
   const uint32_t res = *lds_idx;
   __syncthreads();
