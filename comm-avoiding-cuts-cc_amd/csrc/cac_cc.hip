@@ -27,6 +27,10 @@
 // Steps 2-5 run when m is large; small inputs go straight to a plain hook.
 // Any edge order is correct; the run sampling pays off when equal u are
 // adjacent (the order of the edge-list files and of the R-MAT generator).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include "cac_common.h"
 #include "cac_internal.h"
 #include "cac_scan.h"
@@ -309,6 +313,95 @@ __global__ void __launch_bounds__(kCcBlock) k_check_ids(const cac_uedge* e, uint
   if (b) atomicOr(bad, 1u);
 }
 
+// ---- connected components on a symmetric CSR (Afforest) ---------------------
+// Sutton, Ben-Nun, Barak, "Optimizing Parallel Graph Connectivity Computation via
+// Subgraph Sampling" (IPDPS 2018): link every vertex with its first kAfRounds
+// neighbours, compress, find the most frequent root by sampling, then link only
+// the remaining neighbours of vertices outside that component.  Every edge is in
+// both adjacency lists, so an edge from a vertex of the big component to one
+// outside it is still linked from the outside end.
+constexpr uint32_t kAfRounds = 2;
+constexpr uint32_t kAfHeavy = 64;  // a vertex with more remaining neighbours goes to a workgroup
+
+// Vertices [v_lo, v_hi) link with their first kAfRounds neighbours (lists are
+// sorted by id, so these tend to be the low-id hubs of skewed graphs).  Both
+// rounds share one pass: with ~128 bytes of adjacency per vertex, reading a
+// first neighbour already touches nearly every line of the column array.
+__global__ void __launch_bounds__(kCcBlock) k_af_round(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ cols, uint32_t v_lo, uint32_t v_hi,
+                                                       uint32_t* par) {
+  static_assert(kAfRounds == 2, "k_af_round links two neighbours");
+  for (uint64_t i = v_lo + (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < v_hi; i += (uint64_t)gridDim.x * kCcBlock) {
+    const uint32_t u = (uint32_t)i, b = off[u], e = off[u + 1];
+    const uint32_t v0 = b < e ? cols[b] : u, v1 = b + 1 < e ? cols[b + 1] : u;
+    if (v0 != u) uf_link(par, u, v0);
+    if (v1 != u) uf_link(par, u, v1);
+  }
+}
+
+// Vertices whose settled root is not the sampled root link their neighbours
+// from index kAfRounds on; long lists are queued for k_af_heavy.
+__global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ cols, uint32_t n, uint32_t* par,
+                                                       const uint32_t* __restrict__ root, uint32_t* heavy,
+                                                       uint32_t* heavy_count) {
+  const uint32_t R = root[0];
+  for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
+    const uint32_t u = (uint32_t)i;
+    const uint32_t b = off[u] + kAfRounds, e = off[u + 1];
+    if (b >= e || ld_par_fresh(par, u) == R) continue;
+    if (e - b > kAfHeavy) {
+      heavy[atomicAdd(heavy_count, 1u)] = u;
+      continue;
+    }
+    for (uint32_t k = b; k < e; ++k) uf_link(par, u, cols[k]);
+  }
+}
+
+// One workgroup per queued vertex (grid-stride over the queue).
+__global__ void __launch_bounds__(kCcBlock) k_af_heavy(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ cols, uint32_t* par,
+                                                       const uint32_t* __restrict__ heavy,
+                                                       const uint32_t* __restrict__ heavy_count) {
+  const uint32_t count = heavy_count[0];
+  for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const uint32_t u = heavy[q];
+    const uint32_t b = off[u] + kAfRounds, e = off[u + 1];
+    for (uint32_t k = b + threadIdx.x; k < e; k += kCcBlock) uf_link(par, u, cols[k]);
+  }
+}
+
+// ---- symmetric CSR from an edge list ------------------------------------------
+// Directed entries (src << bits) | dst of both orientations (loops dropped),
+// radix-sorted, give every adjacency list sorted by neighbour id.
+__global__ void __launch_bounds__(256) k_csr_keys(const cac_uedge* __restrict__ e, uint64_t m, uint32_t bits,
+                                                  uint64_t* __restrict__ keys) {
+  const uint64_t loop = ~0ull;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (uint64_t)gridDim.x * 256) {
+    const cac_uedge x = e[i];
+    const bool lp = x.u == x.v;
+    keys[2 * i] = lp ? loop : ((uint64_t)x.u << bits) | x.v;
+    keys[2 * i + 1] = lp ? loop : ((uint64_t)x.v << bits) | x.u;
+  }
+}
+
+// cols[i] = dst of sorted entry i; off[v] = first entry with src >= v, for every
+// v in (src(i-1), src(i)], and off[v] = nnz past the last source.
+__global__ void __launch_bounds__(256) k_csr_split(const uint64_t* __restrict__ keys, uint64_t nnz, uint32_t n,
+                                                   uint32_t bits, uint32_t* __restrict__ off, uint32_t* __restrict__ cols) {
+  const uint64_t mask = (1ull << bits) - 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i <= nnz; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t src = i < nnz ? keys[i] >> bits : n;
+    const uint64_t prev = i > 0 ? (keys[i - 1] >> bits) : ~0ull;
+    if (i < nnz) cols[i] = (uint32_t)(keys[i] & mask);
+    if (i == 0) {
+      for (uint64_t v = 0; v <= src; ++v) off[v] = 0;
+    } else if (src != prev) {
+      for (uint64_t v = prev + 1; v <= src; ++v) off[v] = (uint32_t)i;
+    }
+  }
+}
+
 // ---- approximate min cut (ApproximateCut::run) -------------------------------
 // (1 - 2^-i)^w in 0.64 fixed point, floor(a*b/2^64) per multiply; ~0 stands for 1.
 __device__ inline uint64_t keep_threshold(uint32_t i, uint64_t w) {
@@ -412,6 +505,65 @@ int cc_launch(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, in
   return CAC_OK;
 }
 
+// Afforest on a symmetric CSR; same scratch layout as cc_launch (head, then the
+// rank tables), plus n + 64 words for the queue of long adjacency lists.
+size_t cc_csr_tmp_words(uint32_t n, int label_mode) {
+  return kCcTmpHead + ((size_t)n + 64) + (label_mode == CAC_LABEL_RANK ? cc_rank_tmp_words(n) : 0);
+}
+
+int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, uint32_t n, int label_mode,
+                  uint32_t* d_lab, uint32_t* d_tmp, hipStream_t s) {
+  uint32_t* d_counter = d_tmp;        // [0] components, [1] sampled root, [2] queue length
+  uint32_t* d_heavy = d_tmp + kCcTmpHead;
+  uint32_t* d_rank_tmp = d_heavy + ((size_t)n + 64);
+  const unsigned cap = 256 * 64;
+  prof_begin(ctx, "af_init", s);
+  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, n);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counter, 3);
+  prof_end(ctx, s);
+  // Low ids first, each prefix pointer-jumped before the next range links to it:
+  // the first neighbours of most vertices are low-id hubs, whose chains are then
+  // one hop long (0.99 ms -> 0.58 ms for the round on R-MAT 2^24 / 2^28).
+  prof_begin(ctx, "af_round", s);
+  uint32_t lo = 0;
+  for (uint32_t hb : {1u << 8, 1u << 12, 1u << 16}) {
+    const uint32_t hi = n < hb ? n : hb;
+    if (hi <= lo) continue;
+    hipLaunchKernelGGL(k_af_round, dim3(grid_for(hi - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols, lo, hi,
+                       d_lab);
+    hipLaunchKernelGGL(k_cc_settle, dim3(grid_for(hi, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, hi);
+    lo = hi;
+  }
+  if (lo < n)
+    hipLaunchKernelGGL(k_af_round, dim3(grid_for(n - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols, lo, n,
+                       d_lab);
+  prof_end(ctx, s);
+  prof_begin(ctx, "af_settle", s);
+  hipLaunchKernelGGL(k_cc_settle, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, n);
+  hipLaunchKernelGGL(k_cc_mode, dim3(1), dim3(1024), 0, s, (const uint32_t*)d_lab, n, d_counter + 1);
+  prof_end(ctx, s);
+  prof_begin(ctx, "af_final", s);
+  hipLaunchKernelGGL(k_af_final, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols, n, d_lab,
+                     (const uint32_t*)(d_counter + 1), d_heavy, d_counter + 2);
+  hipLaunchKernelGGL(k_af_heavy, dim3(1024), dim3(kCcBlock), 0, s, d_off, d_cols, d_lab, (const uint32_t*)d_heavy,
+                     (const uint32_t*)(d_counter + 2));
+  prof_end(ctx, s);
+  prof_begin(ctx, "af_compress", s);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n, kCcBlock, 4096)), dim3(kCcBlock), 0, s, d_lab, n, d_counter);
+  prof_end(ctx, s);
+  if (label_mode == CAC_LABEL_RANK) {
+    uint32_t* rank_of = d_rank_tmp;
+    uint32_t* partial = d_rank_tmp + ((n + 63) & ~63u);
+    hipError_t e = device_exclusive_scan<uint32_t>(RootFlag{d_lab}, RootRankStore{rank_of}, n, partial,
+                                                   (uint32_t*)nullptr, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rank scan");
+    hipLaunchKernelGGL(k_cc_apply_rank, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab,
+                       (const uint32_t*)rank_of, n);
+  }
+  CAC_LAUNCH_CHECK(ctx, "csr cc kernels");
+  return CAC_OK;
+}
+
 }  // namespace cac
 
 extern "C" {
@@ -472,6 +624,97 @@ int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int lab
   CAC_HIP(ctx, hipStreamSynchronize(s));
   *ncomp_out = *(uint32_t*)ctx->pinned;
   return CAC_OK;
+}
+
+int cac_cc_csr_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_cols, uint32_t n, int label_mode,
+                   uint32_t* d_labels, uint32_t* ncomp_out, void* stream) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!d_offsets || (n && (!d_labels || !d_cols)) || !ncomp_out) return cac::fail(ctx, CAC_EINVAL, "null pointer");
+  if (label_mode != CAC_LABEL_MIN_ID && label_mode != CAC_LABEL_RANK)
+    return cac::fail(ctx, CAC_EINVAL, "label_mode must be CAC_LABEL_MIN_ID or CAC_LABEL_RANK");
+  if (n == 0) {
+    *ncomp_out = 0;
+    return CAC_OK;
+  }
+  hipStream_t s = cac::pick_stream(ctx, stream);
+  void* base = nullptr;
+  rc = cac::scratch_reserve(ctx, cac::cc_csr_tmp_words(n, label_mode) * 4, &base);
+  if (rc) return rc;
+  uint32_t* counter = (uint32_t*)base;
+  rc = cac::cc_csr_launch(ctx, d_offsets, d_cols, n, label_mode, d_labels, counter, s);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, counter, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  *ncomp_out = *(uint32_t*)ctx->pinned;
+  return CAC_OK;
+}
+
+int cac_csr_build_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, uint32_t* d_offsets,
+                      uint32_t* d_cols, uint64_t* nnz_out, void* stream) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if ((m && !d_edges) || !d_offsets || !nnz_out || (m && !d_cols)) return cac::fail(ctx, CAC_EINVAL, "null pointer");
+  if (2 * m >= 0xFFFFFFFFull) return cac::fail(ctx, CAC_EINVAL, "a CSR needs fewer than 2^31 edges");
+  hipStream_t s = cac::pick_stream(ctx, stream);
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < n) ++bits;
+  const uint64_t nd = 2 * m;
+  uint64_t *k0 = nullptr, *k1 = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  uint32_t* d_nnz = nullptr;
+  hipError_t e = hipMalloc((void**)&k0, 8 * (nd ? nd : 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&k1, 8 * (nd ? nd : 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&d_nnz, 64);
+  if (e == hipSuccess && m) {  // ids < n, checked before any offset is written
+    uint32_t bad = 0;
+    hipLaunchKernelGGL(cac::k_zero_u32, dim3(1), dim3(64), 0, s, d_nnz, 1);
+    hipLaunchKernelGGL(cac::k_check_ids, dim3(cac::grid_for(m, 256, 4096)), dim3(256), 0, s, d_edges, m, n, d_nnz);
+    e = hipMemcpyAsync(&bad, d_nnz, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && bad) {
+      (void)hipFree(d_nnz);
+      (void)hipFree(k1);
+      (void)hipFree(k0);
+      return cac::fail(ctx, CAC_EINVAL, "edge endpoint >= n");
+    }
+  }
+  rocprim::double_buffer<uint64_t> kb(k0, k1);
+  if (e == hipSuccess && nd) e = rocprim::radix_sort_keys(nullptr, tmp_bytes, kb, nd, 0, 2 * bits, s);
+  if (e == hipSuccess && nd) e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1);
+  uint64_t nnz = 0;
+  if (e == hipSuccess && nd) {
+    hipLaunchKernelGGL(cac::k_csr_keys, dim3(cac::grid_for(m, 256, 8192)), dim3(256), 0, s, d_edges, m, bits, k0);
+    e = rocprim::radix_sort_keys(tmp, tmp_bytes, kb, nd, 0, 2 * bits, s);
+  }
+  if (e == hipSuccess && nd) {
+    // loops sort last (all ones): nnz = number of keys below the loop marker,
+    // by binary search over single-key reads
+    uint64_t lo = 0, hi = nd;
+    while (lo < hi && e == hipSuccess) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      uint64_t key = 0;
+      e = hipMemcpyAsync(&key, kb.current() + mid, 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (key == ~0ull) hi = mid;
+      else lo = mid + 1;
+    }
+    nnz = lo;
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(cac::k_csr_split, dim3(cac::grid_for(nnz + 1, 256, 8192)), dim3(256), 0, s,
+                       (const uint64_t*)kb.current(), nnz, n, bits, d_offsets, d_cols);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "csr build");
+  if (tmp) (void)hipFree(tmp);
+  if (d_nnz) (void)hipFree(d_nnz);
+  if (k1) (void)hipFree(k1);
+  if (k0) (void)hipFree(k0);
+  if (rc == CAC_OK) *nnz_out = nnz;
+  return rc;
 }
 
 }  // extern "C"

@@ -99,6 +99,25 @@ int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int lab
 int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
                uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
 
+/* ---- connected components on a symmetric CSR -------------------------------
+ * The same labels as cac_cc_dev (CAC_LABEL_MIN_ID / CAC_LABEL_RANK, component
+ * count), for a graph held as a symmetric CSR: d_offsets[n + 1], d_cols[nnz],
+ * every undirected edge in both adjacency lists (loops and repeats allowed).
+ * Afforest (Sutton, Ben-Nun and Barak, IPDPS 2018): every vertex links with its
+ * first two neighbours, the most frequent root is found by sampling, and only
+ * vertices outside it link their other neighbours, so most of the adjacency of
+ * a graph with a giant component is never read.  Replaces the same
+ * connectedComponents call as cac_cc_dev (src/UnweightedIteratedSparseSampling.cpp:40-72)
+ * for callers that hold a CSR.  Ids are not validated. */
+int cac_cc_csr_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_cols, uint32_t n, int label_mode,
+                   uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
+/* Symmetric CSR of an undirected device edge list (ids < n, checked:
+ * CAC_EINVAL otherwise; fewer than 2^31 edges): both orientations of every non-loop edge, each adjacency list sorted
+ * by neighbour id; d_offsets[n + 1], d_cols[2 m] (the first *nnz_out used).
+ * Set-up work (radix sort), no counterpart in the reference. */
+int cac_csr_build_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, uint32_t* d_offsets,
+                      uint32_t* d_cols, uint64_t* nnz_out, void* stream);
+
 /* ---- dense Karger-Stein --------------------------------------------------
  * Replaces comincut::minimum_cut(adjacency_matrix<T>*, double, int) and
  * minimum_cut_try (src/karger-stein/co_mincut.h:466-490; trials co_mincut.h:315-342).
