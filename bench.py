@@ -81,7 +81,9 @@ def kernel_bytes(name: str, m: int, n: int) -> float:
         "cc_hook": 8.0 * m,                # read every edge once
         "cc_compress": 8.0 * n,            # read + write labels
         "af_init": 4.0 * n,                # write parent
-        "af_round": 16.0 * n,              # offsets, two neighbours and the parent of every vertex
+        "af_hubs": 0.0,                    # the low-id prefix ranges (2^16 vertices): negligible
+        "af_round": 28.0 * n,              # per vertex: offsets 4, two neighbours 8, own parent read+write 8,
+                                           # the two neighbours' parents 8
         "af_settle": 8.0 * n,              # read + write parent
         "af_final": 8.0 * n,               # offsets and parent of every vertex
         "af_compress": 8.0 * n,            # read + write labels
