@@ -22,7 +22,9 @@ the metric, each on its own workload:
   c5  served Poisson stream, 70% CC / 30% min cut, G(n, d/n), n log-uniform in
       [10^3, 10^6], d uniform in [8, 64], weights U[1, 100];
   ks  karger_stein (sparse Karger-Stein, p = 0.90) on Watts-Strogatz graphs of
-      the reference's ws_32k_16 / ws_96k_16 shape, beside its published times.
+      the reference's ws_32k_16 / ws_96k_16 shape, beside its published times;
+  sq  square_root (p = 0.95) on the same two graphs, beside the reference's
+      published 1-core and 1440-core times.
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle's single-thread union-find on the same edge list).
@@ -58,14 +60,15 @@ def parse():
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC byte-counter scaling)")
-    ap.add_argument("--extras", default="c1,c3,c4,c5,ks",
-                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks) or 'none'")
+    ap.add_argument("--extras", default="c1,c3,c4,c5,ks,sq",
+                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks,sq) or 'none'")
     ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
     ap.add_argument("--c4-graphs", type=int, default=4096)
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
     ap.add_argument("--c5-queries", type=int, default=200)
     ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
     ap.add_argument("--ks-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the ks leg")
+    ap.add_argument("--sq-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the sq leg")
     ap.add_argument("--cc-input", choices=("csr", "edges"), default="csr",
                     help="input the headline CC step reads: the symmetric CSR (Afforest) or the edge list")
     return ap.parse_args()
@@ -249,6 +252,50 @@ def bench_ks(ctx, cac, sizes, cpu_budget_s: float):
                                      "sample": f"one trial of the reference's comincut::minimum_cut (p=0) on the "
                                                f"same graph, {per_trial:.1f} s; x{T} trials = "
                                                f"{per_trial * T:.0f} s"}
+        out[f"ws_{n // 1000}k_16"] = r
+    return out
+
+
+# The reference's square_root (p = 0.95) on its Watts-Strogatz inputs, median
+# seconds over its runs: evaluation/data/e1/32k.csv and 96k.csv (1 core and
+# 1440 cores of Piz Daint's multicore partition).
+PUBLISHED_SQ_WS_S = {32000: {"1_core": 298.3, "1440_cores": 0.5548}, 96000: {"1_core": 2818.0, "1440_cores": 3.845}}
+
+
+def bench_sq(ctx, cac, sizes, cpu_budget_s: float):
+    """square_root (sampling contraction + dense Karger-Stein per trial, p = 0.95)
+    on Watts-Strogatz graphs shaped like the reference's ws_{32k,96k}_16."""
+    out = {}
+    for n in sizes:
+        e = cac.gen_ws(n, 16, 0)
+        m = len(e)
+        ctx.mincut(e, n, F32_95, 1)  # warm-up: same graph, another seed
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        cut, T = ctx.mincut(e, n, F32_95, 0)
+        dt = time.perf_counter() - t0
+        deg = np.bincount(e["u"], e["w"], n) + np.bincount(e["v"], e["w"], n)
+        pub = PUBLISHED_SQ_WS_S.get(n)
+        r = {"workload": f"square_root p=0.95 seed 0, Watts-Strogatz n={n}, 16 ring neighbours a side, beta 0.3, "
+                         f"weight 100, m={m}",
+             "seconds": dt, "solves_per_s": 1.0 / dt, "trials": T, "t": cac.intermediate_size(n, m),
+             "trials_per_s": T / dt, "cut": int(cut), "min_degree": int(deg.min()),
+             "published_reference_seconds": pub,
+             "speedup_vs_published_1_core": pub["1_core"] / dt if pub else None,
+             "speedup_vs_published_1440_cores": pub["1440_cores"] / dt if pub else None}
+        if cpu_budget_s > 0 and n == min(sizes):
+            import oracle  # the CPU restatement: baseline only
+            e3 = np.stack([e["u"], e["v"], e["w"]], axis=1).astype(np.uint64)
+            t = r["t"]
+            t0 = time.perf_counter()
+            k = 0
+            while k < T and time.perf_counter() - t0 < cpu_budget_s:
+                oracle.sqrt_trial(n, e3, t, 0, k)
+                k += 1
+            per_trial = (time.perf_counter() - t0) / k
+            r["cpu_baseline"] = {"value": 1.0 / per_trial, "unit": "trials/s", "cores": 1, "kind": "port",
+                                 "sample": f"{k} of the {T} trials timed single-threaded ({per_trial:.2f} s each); "
+                                           f"x{T} trials = {per_trial * T:.0f} s"}
         out[f"ws_{n // 1000}k_16"] = r
     return out
 
@@ -468,7 +515,9 @@ def main():
                     ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride)),
                     ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate)),
                     ("ks", lambda: bench_ks(ctx, cac, [int(x) for x in args.ks_sizes.split(",")],
-                                                    min(args.cpu_baseline_seconds, 20.0)))):
+                                                    min(args.cpu_baseline_seconds, 20.0))),
+                    ("sq", lambda: bench_sq(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],
+                                                    min(args.cpu_baseline_seconds, 15.0)))):
         if key in wanted:
             t0 = time.perf_counter()
             extras[key] = fn()
