@@ -148,6 +148,7 @@ int cac_ctx_destroy(cac_ctx* ctx) {
     if (b) (void)hipFree(b);
   if (ctx->sp_dense_buf) (void)hipFree(ctx->sp_dense_buf);
   if (ctx->sp_tmp) (void)hipFree(ctx->sp_tmp);
+  if (ctx->lt_buf) (void)hipFree(ctx->lt_buf);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

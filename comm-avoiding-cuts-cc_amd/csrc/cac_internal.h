@@ -45,6 +45,10 @@ struct cac_ctx {
   size_t sp_tmp_bytes = 0;
   void* sp_dense_buf = nullptr;
   size_t sp_dense_bytes = 0;
+  // Square-root-cut trials beyond the dense engine: sampling scratch and the
+  // merged contracted edge list of one trial.
+  void* lt_buf = nullptr;
+  size_t lt_bytes = 0;
 };
 
 namespace cac {

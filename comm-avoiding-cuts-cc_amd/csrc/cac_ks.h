@@ -55,6 +55,16 @@ int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, 
 // raw cut of trial_ids[i] to d_results[i], which must already hold the bound.
 int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
                   const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s);
+// Sparse trials of possibly different graphs side by side: root i is trial
+// `trial` of (edges, m, n) from node 1; d_results[slot] must hold its bound.
+struct SparseRoot {
+  const cac_edge* edges;
+  uint64_t m;
+  uint32_t n, trial, slot;
+  int64_t bound;
+};
+int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t seed, int64_t* d_results,
+                    hipStream_t s);
 int ks_check_matrix(cac_ctx* ctx, const int64_t* adj, uint32_t n);
 
 // One sampling contraction (cac_sample_contract): canonical labels and the merged

@@ -23,10 +23,15 @@
 #include <cmath>
 #include <vector>
 
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include "cac_common.h"
 #include "cac_internal.h"
 #include "cac_ks.h"
 #include "cac_ks_kernels.h"
+#include "cac_scan.h"
 
 namespace cac {
 namespace {
@@ -264,6 +269,67 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
   }
 }
 
+// ---- trials whose contracted graph exceeds the dense engine (t > 8192) ------
+// The contracted graph becomes an edge list, as SequentialKargerSteinCut gets
+// it (AdjacencyListGraph::compact, src/AdjacencyListGraph.cpp:34-112): endpoints
+// ranked by canonical label, loops dropped, parallel edges summed, sorted by
+// (u, v); one sparse Karger-Stein trial runs on it.
+struct LabLive {
+  const uint32_t* lab;
+  __device__ uint32_t operator()(uint64_t i) const { return lab[i] == (uint32_t)i ? 1u : 0u; }
+};
+struct LabRank {
+  uint32_t* rank;
+  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t live) const {
+    if (live) rank[i] = ex;
+  }
+};
+
+// key (a << 32) | b of the ranked endpoints (a < b), ~0 for a loop; value w.
+__global__ void __launch_bounds__(256) k_lt_keys(const cac_edge* __restrict__ edges, uint64_t m,
+                                                 const uint32_t* __restrict__ lab, const uint32_t* __restrict__ rank,
+                                                 uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < m; e += (uint64_t)gridDim.x * 256) {
+    const cac_edge x = edges[e];
+    const uint32_t a = rank[lab[x.u]], b = rank[lab[x.v]];
+    keys[e] = a == b ? ~0ull : ((uint64_t)min(a, b) << 32) | max(a, b);
+    vals[e] = x.w;
+  }
+}
+
+// Merged keys to edges, and the weighted degrees.
+__global__ void __launch_bounds__(256) k_lt_edges(const uint64_t* __restrict__ ukeys, const uint64_t* __restrict__ sums,
+                                                  uint64_t count, cac_edge* __restrict__ out,
+                                                  unsigned long long* __restrict__ deg) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t k = ukeys[i];
+    const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+    out[i] = cac_edge{a, b, sums[i]};
+    atomicAdd(&deg[a], (unsigned long long)sums[i]);
+    atomicAdd(&deg[b], (unsigned long long)sums[i]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lt_zero(unsigned long long* p, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) p[i] = 0;
+}
+
+// *out = min(deg[0..k)) (out preset to ~0), and results[slot] = that bound.
+__global__ void __launch_bounds__(256) k_lt_min(const unsigned long long* deg, uint32_t k, unsigned long long* out) {
+  __shared__ unsigned long long w[4];
+  unsigned long long b = ~0ull;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < k; i += (uint64_t)gridDim.x * 256) b = min(b, deg[i]);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) b = min(b, (unsigned long long)__shfl_xor(b, d, 64));
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMin(out, min(min(w[0], w[1]), min(w[2], w[3])));
+}
+
+__global__ void k_lt_set(int64_t* p, const unsigned long long* v) {
+  if (threadIdx.x == 0) p[0] = (int64_t)v[0];
+}
+
 }  // namespace
 
 // Host rules, same floating point as the reference (SquareRootCut.cpp:63-83).
@@ -309,9 +375,165 @@ size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t);
 // one sampling launch, one densify launch and one breadth-first Karger-Stein
 // pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
 // i goes to d_results[jobs[i].slot].
+// Trials with t > 8192, in chunks sized to the free memory: one sampling launch
+// for the chunk, then per trial the merged contracted edge list (rank, keys,
+// radix sort, reduce by key; the sort buffers are shared) and its bound, then
+// one sparse Karger-Stein pass over all of the chunk's trials (node 1 each).
+// Raw cut of each job to d_results[slot].
+static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results,
+                            hipStream_t s) {
+  auto job_bytes = [](const SqrtJob& j) {
+    const uint64_t m = std::max<uint64_t>(j.m, 1);
+    return carve_size(sizeof(TrialScratch)) + 3 * carve_size((size_t)j.n * 4) + carve_size(m * 8) +
+           carve_size((size_t)j.n * 8) + carve_size(64) + carve_size(m * sizeof(cac_edge)) +
+           carve_size(((size_t)j.t + 2) * 8);
+  };
+  size_t avail = 0;
+  int rc = device_memory_available(ctx, &avail);
+  if (rc) return rc;
+  avail += ctx->lt_bytes;
+  const size_t budget = (size_t)(0.35 * (double)avail);
+  size_t b0 = 0;
+  while (b0 < jobs.size()) {
+    // the chunk: shared sort buffers for its largest graph, plus per-job scratch
+    size_t nb = 0, per = 0, shared = 0;
+    uint64_t mmax = 1, nmaxs = 1;
+    size_t sort_tmp = 0, red_tmp = 0;
+    while (b0 + nb < jobs.size()) {
+      const SqrtJob& j = jobs[b0 + nb];
+      const uint64_t m2 = std::max<uint64_t>(mmax, std::max<uint64_t>(j.m, 1));
+      uint32_t bk = 1;
+      while (bk < 32 && (1ull << bk) < j.t) ++bk;
+      size_t st = 0, rt = 0;
+      rocprim::double_buffer<uint64_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+      CAC_HIP(ctx, rocprim::radix_sort_pairs(nullptr, st, kb, vb, m2, 0, 32 + bk, s));
+      CAC_HIP(ctx, rocprim::reduce_by_key(nullptr, rt, (const uint64_t*)nullptr, (const uint64_t*)nullptr, m2,
+                                          (uint64_t*)nullptr, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                          rocprim::plus<uint64_t>(), rocprim::equal_to<uint64_t>(), s));
+      const size_t sh = 6 * carve_size(m2 * 8) + carve_size(std::max(std::max(st, rt), std::max(sort_tmp, red_tmp))) +
+                        carve_size((scan_tiles(std::max<uint64_t>(nmaxs, j.n)) + 64) * 4) + 4096;
+      const size_t add = job_bytes(j);
+      if (nb > 0 && per + add + sh > budget) break;
+      per += add;
+      shared = sh;
+      mmax = m2;
+      nmaxs = std::max<uint64_t>(nmaxs, j.n);
+      sort_tmp = std::max(sort_tmp, st);
+      red_tmp = std::max(red_tmp, rt);
+      ++nb;
+    }
+    rc = grow_buffer(ctx, &ctx->lt_buf, &ctx->lt_bytes, per + shared);
+    if (rc) return rc;
+    Carver cv(ctx->lt_buf);
+    uint64_t* k0 = cv.take<uint64_t>(mmax);
+    uint64_t* k1 = cv.take<uint64_t>(mmax);
+    uint64_t* v0 = cv.take<uint64_t>(mmax);
+    uint64_t* v1 = cv.take<uint64_t>(mmax);
+    uint64_t* ukeys = cv.take<uint64_t>(mmax);
+    uint64_t* sums = cv.take<uint64_t>(mmax);
+    void* tmp = cv.take<unsigned char>(std::max(sort_tmp, red_tmp));
+    uint32_t* partial = cv.take<uint32_t>(scan_tiles(nmaxs) + 64);
+    TrialScratch* d_scr = cv.take<TrialScratch>(nb);
+    std::vector<TrialScratch> scr(nb);
+    std::vector<uint32_t*> rank(nb);
+    std::vector<cac_edge*> merged(nb);
+    std::vector<unsigned long long*> deg(nb);
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      TrialScratch& x = scr[i];
+      x = TrialScratch{};
+      x.edges = j.edges;
+      x.m = j.m;
+      x.n = j.n;
+      x.t = j.t;
+      x.trial = j.trial;
+      x.slot = j.slot;
+      x.lab = cv.take<uint32_t>(j.n);
+      x.par = cv.take<uint32_t>(j.n);
+      rank[i] = cv.take<uint32_t>(j.n);
+      x.pre = cv.take<uint64_t>(std::max<uint64_t>(j.m, 1));
+      x.draws = cv.take<uint2>(j.n);
+      x.nv_out = cv.take<uint32_t>(16);
+      merged[i] = cv.take<cac_edge>(std::max<uint64_t>(j.m, 1));
+      deg[i] = cv.take<unsigned long long>((size_t)j.t + 2);  // [t] degrees, [t] count, [t + 1] min
+    }
+    CAC_HIP(ctx, hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s));
+    prof_begin(ctx, "sample_trials", s);
+    hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
+    prof_end(ctx, s);
+    CAC_LAUNCH_CHECK(ctx, "sample trials");
+    std::vector<uint32_t> nv(nb);
+    for (size_t i = 0; i < nb; ++i) CAC_HIP(ctx, hipMemcpyAsync(&nv[i], scr[i].nv_out, 4, hipMemcpyDeviceToHost, s));
+    CAC_HIP(ctx, hipStreamSynchronize(s));
+    std::vector<SparseRoot> roots;
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      if (nv[i] > j.t || nv[i] < 2) {  // disconnected: a cut of weight 0 exists
+        const int64_t zero = 0;
+        CAC_HIP(ctx, hipMemcpyAsync(d_results + j.slot, &zero, 8, hipMemcpyHostToDevice, s));
+        CAC_HIP(ctx, hipStreamSynchronize(s));
+        continue;
+      }
+      const uint32_t k = nv[i];
+      uint32_t bk = 1;
+      while (bk < 32 && (1ull << bk) < j.t) ++bk;
+      prof_begin(ctx, "contracted_edges", s);
+      CAC_HIP(ctx, device_exclusive_scan<uint32_t>(LabLive{scr[i].lab}, LabRank{rank[i]}, j.n, partial,
+                                                   (uint32_t*)nullptr, s));
+      hipLaunchKernelGGL(k_lt_keys, dim3(grid_for(j.m, 256, 8192)), dim3(256), 0, s, j.edges, j.m,
+                         (const uint32_t*)scr[i].lab, (const uint32_t*)rank[i], k0, v0);
+      rocprim::double_buffer<uint64_t> kb(k0, k1), vb(v0, v1);
+      size_t st = sort_tmp, rt = red_tmp;
+      CAC_HIP(ctx, rocprim::radix_sort_pairs(tmp, st, kb, vb, j.m, 0, 32 + bk, s));
+      uint64_t* d_count = (uint64_t*)(deg[i] + j.t);
+      CAC_HIP(ctx, rocprim::reduce_by_key(tmp, rt, (const uint64_t*)kb.current(), (const uint64_t*)vb.current(), j.m,
+                                          ukeys, sums, d_count, rocprim::plus<uint64_t>(),
+                                          rocprim::equal_to<uint64_t>(), s));
+      uint64_t count = 0;
+      CAC_HIP(ctx, hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, s));
+      CAC_HIP(ctx, hipStreamSynchronize(s));
+      uint64_t last = 0;
+      if (count) {
+        CAC_HIP(ctx, hipMemcpyAsync(&last, ukeys + count - 1, 8, hipMemcpyDeviceToHost, s));
+        CAC_HIP(ctx, hipStreamSynchronize(s));
+      }
+      const uint64_t mm = count && last == ~0ull ? count - 1 : count;  // the loop key sorts last
+      hipLaunchKernelGGL(k_lt_zero, dim3(grid_for((uint64_t)j.t + 2, 256, 1024)), dim3(256), 0, s, deg[i],
+                         (uint64_t)j.t + 2);
+      hipLaunchKernelGGL(k_lt_edges, dim3(grid_for(mm, 256, 8192)), dim3(256), 0, s, (const uint64_t*)ukeys,
+                         (const uint64_t*)sums, mm, merged[i], deg[i]);
+      unsigned long long* d_min = deg[i] + j.t + 1;
+      const unsigned long long init = ~0ull;
+      CAC_HIP(ctx, hipMemcpyAsync(d_min, &init, 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_lt_min, dim3(grid_for(k, 256, 256)), dim3(256), 0, s, (const unsigned long long*)deg[i], k,
+                         d_min);
+      hipLaunchKernelGGL(k_lt_set, dim3(1), dim3(64), 0, s, d_results + j.slot, (const unsigned long long*)d_min);
+      prof_end(ctx, s);
+      CAC_LAUNCH_CHECK(ctx, "contracted edge list");
+      unsigned long long bound = 0;
+      CAC_HIP(ctx, hipMemcpyAsync(&bound, d_min, 8, hipMemcpyDeviceToHost, s));
+      CAC_HIP(ctx, hipStreamSynchronize(s));
+      if (mm > 0) roots.push_back({merged[i], mm, k, j.trial, j.slot, (int64_t)bound});
+    }
+    rc = ks_sparse_roots(ctx, roots, seed, d_results, s);
+    if (rc) return rc;
+    b0 += nb;
+  }
+  return CAC_OK;
+}
+
 int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s,
                    size_t budget) {
   if (jobs.empty()) return CAC_OK;
+  bool any_large = false;
+  for (const SqrtJob& j : jobs) any_large = any_large || j.t > kKsMaxS;
+  if (any_large) {
+    std::vector<SqrtJob> dense, large;
+    for (const SqrtJob& j : jobs) (j.t <= kKsMaxS ? dense : large).push_back(j);
+    const int rc = sqrt_large_batch(ctx, large, seed, d_results, s);
+    if (rc) return rc;
+    return sqrt_cut_batch(ctx, dense, seed, d_results, s, budget);
+  }
   if (budget == 0) {
     size_t avail = 0;
     int rc0 = device_memory_available(ctx, &avail);

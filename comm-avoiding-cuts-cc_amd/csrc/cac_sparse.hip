@@ -496,6 +496,7 @@ struct SpNode {
   const cac_edge* edges;
   uint64_t m;
   uint32_t n, trial, nid, slot;
+  int64_t bound;  // the trial's bound (minimum weighted degree of its root)
 };
 
 // Device bytes of one child: what stays alive for the next depth (its edge
@@ -511,7 +512,6 @@ size_t dense_bytes(const SpNode& nd) {
 struct SpRun {
   cac_ctx* ctx;
   uint32_t seed;
-  int64_t bound;
   int64_t* d_results;
   hipStream_t s;
   size_t depth_budget;  // bytes of children kept alive per depth
@@ -548,7 +548,7 @@ struct SpRun {
       for (size_t i = 0; i < count; ++i) {
         const SpNode& nd = nodes[i0 + i];
         hd[i] = SpDense{nd.edges, nd.m, nd.n, 0, cv.take<int64_t>((size_t)nd.n * nd.n), cv.take<int64_t>(nd.n)};
-        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, bound};
+        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, nd.bound};
         r.nid = nd.nid;
         roots.push_back(r);
         mmax = std::max(mmax, nd.m);
@@ -638,7 +638,7 @@ struct SpRun {
       std::vector<SpNode> next;
       for (size_t i = 0; i < nchild; ++i) {
         if (mout[i] == kSpDisconnected) continue;  // its cut of 0 is already recorded
-        next.push_back({ch[i].out, mout[i], ch[i].tgt, ch[i].trial, ch[i].nid, ch[i].slot});
+        next.push_back({ch[i].out, mout[i], ch[i].tgt, ch[i].trial, ch[i].nid, ch[i].slot, sp[i0 + i / 2].bound});
       }
       rc = level(next, depth + 1);
       if (rc) return rc;
@@ -657,22 +657,31 @@ size_t sparse_depth(uint32_t n) {
 
 }  // namespace
 
-int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
-                  const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s) {
+int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t seed, int64_t* d_results,
+                    hipStream_t s) {
+  if (roots.empty()) return CAC_OK;
   size_t avail = 0;
   int rc = device_memory_available(ctx, &avail);
   if (rc) return rc;
   for (size_t i = 0; i < ctx->sp_bufs.size(); ++i) avail += ctx->sp_bytes[i];
   avail += ctx->sp_dense_bytes + ctx->sp_tmp_bytes;
-  const size_t depth = sparse_depth(n);
-  SpRun run{ctx, seed, bound, d_results, s, (size_t)(0.4 * (double)avail) / (depth + 1),
-            (size_t)(0.15 * (double)avail)};
-  std::vector<SpNode> roots;
-  for (size_t i = 0; i < trial_ids.size(); ++i) roots.push_back({d_edges, m, n, trial_ids[i], 1u, (uint32_t)i});
-  rc = run.level(roots, 0);
+  uint32_t nmax = 0;
+  for (const SparseRoot& r : roots) nmax = std::max(nmax, r.n);
+  const size_t depth = sparse_depth(nmax);
+  SpRun run{ctx, seed, d_results, s, (size_t)(0.4 * (double)avail) / (depth + 1), (size_t)(0.15 * (double)avail)};
+  std::vector<SpNode> level;
+  for (const SparseRoot& r : roots) level.push_back({r.edges, r.m, r.n, r.trial, 1u, r.slot, r.bound});
+  rc = run.level(level, 0);
   if (rc) return rc;
   CAC_HIP(ctx, hipStreamSynchronize(s));
   return CAC_OK;
+}
+
+int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
+                  const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s) {
+  std::vector<SparseRoot> roots;
+  for (size_t i = 0; i < trial_ids.size(); ++i) roots.push_back({d_edges, m, n, trial_ids[i], (uint32_t)i, bound});
+  return ks_sparse_roots(ctx, roots, seed, d_results, s);
 }
 
 }  // namespace cac
@@ -809,8 +818,10 @@ int cac_karger_stein(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n
   if (!(p >= 0.0 && p < 1.0)) return cac::fail(ctx, CAC_EINVAL, "success probability must be in [0, 1)");
   const uint32_t T = cac::ks_trials_host(n, p);
   if (trials) *trials = T;
-  if (m >= (uint64_t)n * n / 32) {  // the reference's dense switch (co_mincut.h:401)
-    if (n > cac::kKsMaxS) return cac::fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
+  // the reference's dense switch (co_mincut.h:401); a dense graph larger than
+  // the dense engine's 8192 vertices starts in the sparse recursion, which
+  // hands it over once a node is small enough
+  if (m >= (uint64_t)n * n / 32 && n <= cac::kKsMaxS) {
     std::vector<int64_t> A((size_t)n * n, 0);
     unsigned __int128 total = 0;
     for (uint64_t i = 0; i < m; ++i) {

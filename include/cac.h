@@ -164,8 +164,9 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
 /* Replaces SequentialKargerSteinCut::compute (src/SequentialKargerSteinCut.cpp:6-27)
  * with comincut::minimum_cut_trials_start's dense switch (co_mincut.h:396-414):
  * T = cac_ks_trials(n, p) trials (ids 0..T-1; p = 0 gives one trial); the dense
- * engine when m >= n^2 / 32 (n <= 8192), else cac_ks_sparse.  *cut = minimum,
- * *trials = T. */
+ * engine when m >= n^2 / 32 and n <= 8192, else cac_ks_sparse (which hands a
+ * node over to the dense engine once it is small and dense enough).
+ * *cut = minimum, *trials = T. */
 int cac_karger_stein(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, double p, uint32_t seed,
                      int64_t* cut, uint32_t* trials);
 /* The race key of the sparse contraction: floor(2^20 * L / w) with

@@ -526,11 +526,72 @@ uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32
     return k;
 }
 
+#define SQRT_DENSE_MAX 8192u /* the dense engine's largest matrix */
+int64_t orc_ks_sparse_trial(uint32_t n, uint64_t m, const uint64_t* E, uint32_t seed, uint32_t trial);
+
+static int cmp_u64x3(const void* a, const void* b) {
+    const uint64_t* x = (const uint64_t*)a;
+    const uint64_t* y = (const uint64_t*)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+    return 0;
+}
+
+/* The contracted graph as an edge list, the form SequentialKargerSteinCut gets
+ * (AdjacencyListGraph::compact, src/AdjacencyListGraph.cpp:34-112): endpoints
+ * ranked by canonical label, loops dropped, parallel edges summed, sorted by
+ * (u, v).  Returns the list (caller frees), *m_out entries. */
+static uint64_t* merged_contracted(uint32_t n, uint64_t m, const uint64_t* edges, const uint32_t* lab,
+                                   uint64_t* m_out) {
+    uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t r = 0;
+    for (uint32_t x = 0; x < n; ++x)
+        if (lab[x] == x) rank[x] = r++;
+    uint64_t* T = (uint64_t*)malloc(sizeof(uint64_t) * 3 * (m ? m : 1));
+    uint64_t k = 0;
+    for (uint64_t e = 0; e < m; ++e) {
+        uint32_t a = rank[lab[edges[3 * e]]], b = rank[lab[edges[3 * e + 1]]];
+        if (a == b) continue;
+        T[3 * k] = a < b ? a : b;
+        T[3 * k + 1] = a < b ? b : a;
+        T[3 * k + 2] = edges[3 * e + 2];
+        ++k;
+    }
+    qsort(T, k, 3 * sizeof(uint64_t), cmp_u64x3);
+    uint64_t out = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+        if (out > 0 && T[3 * (out - 1)] == T[3 * i] && T[3 * (out - 1) + 1] == T[3 * i + 1]) {
+            T[3 * (out - 1) + 2] += T[3 * i + 2];
+        } else {
+            T[3 * out] = T[3 * i];
+            T[3 * out + 1] = T[3 * i + 1];
+            T[3 * out + 2] = T[3 * i + 2];
+            ++out;
+        }
+    }
+    free(rank);
+    *m_out = out;
+    return T;
+}
+
 int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial) {
     uint32_t* lab = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
     orc_sample_contract(n, m, edges, t, seed, trial, lab);
     uint32_t k = 0;
     for (uint32_t x = 0; x < n; ++x) k += lab[x] == x;
+    if (t > SQRT_DENSE_MAX) {
+        /* beyond the dense engine: the merged contracted edge list through the
+         * sparse recursion (trial `trial`, node 1); disconnected: cut 0 */
+        int64_t cut = 0;
+        if (k <= t) {
+            uint64_t mm = 0;
+            uint64_t* E = merged_contracted(n, m, edges, lab, &mm);
+            cut = orc_ks_sparse_trial(k, mm, E, seed, trial);
+            free(E);
+        }
+        free(lab);
+        return cut;
+    }
     int64_t* M = (int64_t*)calloc((size_t)k * k, sizeof(int64_t));
     orc_densify(n, m, edges, lab, M);
     int64_t cut = orc_ks_trial(k, M, seed, trial);

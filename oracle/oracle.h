@@ -110,7 +110,10 @@ uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint
 uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32_t* labels, int64_t* out);
 /* One low-concurrency sqrt-cut trial: sample_contract to t, densify, one dense KS
  * trial with the same seed and trial id (SequentialSquareRootCut::compute,
- * src/SequentialSquareRootCut.cpp:4-19). */
+ * src/SequentialSquareRootCut.cpp:4-19).  For t > 8192 (beyond the dense
+ * engine) the contracted graph goes, as an edge list ranked by canonical label,
+ * loops dropped, parallel edges summed and sorted by (u, v), through one sparse
+ * Karger-Stein trial (orc_ks_sparse_trial, node 1); a disconnected graph gives 0. */
 int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial);
 
 /* ---- approximate minimum cut ------------------------------------------------

@@ -104,3 +104,19 @@ def test_mincut_rejects_bad_ids(ctx):
     with pytest.raises(cac.CacError) as ei:
         ctx.mincut(np.array([[0, 7, 1]]), 5, 0.9, 0)
     assert ei.value.code == cac.CAC_EINVAL
+
+
+def test_sqrt_trial_beyond_dense_engine(ctx, orc):
+    # m > 16.7M edges gives t = 2 sqrt(m) + 1 > 8192: the contracted graph goes
+    # through the sparse Karger-Stein as a merged edge list (bit-exact raw cut)
+    rng = np.random.default_rng(1)
+    n, m = 9000, 17_000_000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    v = np.where(u == v, (v + 1) % n, v)
+    E = np.stack([np.minimum(u, v), np.maximum(u, v), rng.integers(1, 10, m)], 1).astype(np.uint64)
+    t = orc.intermediate_size(n, m)
+    assert t > 8192
+    print(f"sqrt trial beyond dense: n={n} m={m} t={t}", flush=True)
+    raw = ctx.sqrt_cut_trials(E, n, t, 3, 0, 1)
+    assert raw[0] == orc.sqrt_trial(n, E, t, 3, 0)
