@@ -24,7 +24,10 @@ the metric, each on its own workload:
   ks  karger_stein (sparse Karger-Stein, p = 0.90) on Watts-Strogatz graphs of
       the reference's ws_32k_16 / ws_96k_16 shape, beside its published times;
   sq  square_root (p = 0.95) on the same two graphs, beside the reference's
-      published 1-core and 1440-core times.
+      published 1-core and 1440-core times;
+  rm  square_root and karger_stein on R-MAT graphs of the reference's
+      rmat_16k_{1024,4096} shape (2^14 vertices, 16.4M / 65.5M edges), beside
+      its published times.
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle's single-thread union-find on the same edge list).
@@ -60,8 +63,8 @@ def parse():
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC byte-counter scaling)")
-    ap.add_argument("--extras", default="c1,c3,c4,c5,ks,sq",
-                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks,sq) or 'none'")
+    ap.add_argument("--extras", default="c1,c3,c4,c5,ks,sq,rm",
+                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks,sq,rm) or 'none'")
     ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
     ap.add_argument("--c4-graphs", type=int, default=4096)
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
@@ -69,6 +72,7 @@ def parse():
     ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
     ap.add_argument("--ks-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the ks leg")
     ap.add_argument("--sq-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the sq leg")
+    ap.add_argument("--rm-degrees", default="1000,4000", help="edges per vertex of the rm leg (R-MAT 2^14 vertices)")
     ap.add_argument("--cc-input", choices=("csr", "edges"), default="csr",
                     help="input the headline CC step reads: the symmetric CSR (Afforest) or the edge list")
     return ap.parse_args()
@@ -300,6 +304,51 @@ def bench_sq(ctx, cac, sizes, cpu_budget_s: float):
     return out
 
 
+# The reference on its R-MAT inputs rmat_16k_{1024,2048,4096} (n = 16000,
+# m = 16.4M / 32.8M / 65.5M, unit weights), median seconds:
+# square_root at 144 and 1296 cores (evaluation/data/e4/data.csv), karger_stein
+# on one core (evaluation/data/e4/baselines.csv).
+PUBLISHED_RMAT16K_S = {1000: {"square_root_144_cores": 11.37, "square_root_1296_cores": 5.885, "karger_stein_1_core": 12.13},
+                       2000: {"square_root_144_cores": 23.2, "square_root_1296_cores": 7.812, "karger_stein_1_core": 31.32},
+                       4000: {"square_root_144_cores": 32.93, "square_root_1296_cores": 9.931, "karger_stein_1_core": 46.39}}
+
+
+def bench_rm(ctx, cac, degrees):
+    """square_root (p = 0.95) and karger_stein (p = 0.90) on R-MAT graphs shaped
+    like the reference's rmat_16k_{1024,2048,4096}: 2^14 vertices, 16.384M x
+    (1, 2, 4) edges, unit weights (the generator's (0.45, 0.22, 0.22, 0.11))."""
+    out = {}
+    scale = 14
+    n = 1 << scale
+    for deg in degrees:
+        m = n * deg
+        d = ctx.alloc(m * 8)
+        ctx.gen_rmat_dev(scale, m, 0, cac.rmat_thresholds(0.45, 0.22, 0.22), d.ptr)
+        ue = d.download(np.uint32, 2 * m).reshape(m, 2)
+        d.free()
+        e = np.empty(m, dtype=cac.EDGE)
+        e["u"], e["v"], e["w"] = ue[:, 0], ue[:, 1], 1
+        del ue
+        r = {"workload": f"R-MAT 2^{scale} vertices, {m} edges, unit weights, seed 0"}
+        ctx.mincut(e, n, F32_95, 1)  # warm-up: same graph, another seed
+        t0 = time.perf_counter()
+        cut, T = ctx.mincut(e, n, F32_95, 0)
+        r["square_root"] = {"seconds": time.perf_counter() - t0, "trials": T, "t": cac.intermediate_size(n, m),
+                            "cut": int(cut)}
+        ctx.karger_stein(e, n, 0.90, 1)
+        t0 = time.perf_counter()
+        cut2, T2 = ctx.karger_stein(e, n, 0.90, 0)
+        r["karger_stein"] = {"seconds": time.perf_counter() - t0, "trials": T2, "cut": int(cut2)}
+        pub = PUBLISHED_RMAT16K_S.get(deg)
+        if pub:
+            r["published_reference_seconds"] = pub
+            r["square_root_speedup_vs_1296_cores"] = pub["square_root_1296_cores"] / r["square_root"]["seconds"]
+            r["karger_stein_speedup_vs_1_core"] = pub["karger_stein_1_core"] / r["karger_stein"]["seconds"]
+        out[f"rmat_16k_x{deg}"] = r
+        del e
+    return out
+
+
 def bench_c4(ctx, cac, count: int, stride: int):
     """Batched square-root min cut over planted-cut graphs vs one graph per call."""
     t0 = time.perf_counter()
@@ -517,7 +566,8 @@ def main():
                     ("ks", lambda: bench_ks(ctx, cac, [int(x) for x in args.ks_sizes.split(",")],
                                                     min(args.cpu_baseline_seconds, 20.0))),
                     ("sq", lambda: bench_sq(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],
-                                                    min(args.cpu_baseline_seconds, 15.0)))):
+                                                    min(args.cpu_baseline_seconds, 15.0))),
+                    ("rm", lambda: bench_rm(ctx, cac, [int(x) for x in args.rm_degrees.split(",")]))):
         if key in wanted:
             t0 = time.perf_counter()
             extras[key] = fn()
