@@ -81,6 +81,30 @@ __global__ void __launch_bounds__(256) k_copy(uint4* __restrict__ dst, const uin
   if (blockIdx.x == 0 && threadIdx.x < tail) dtail[threadIdx.x] = stail[threadIdx.x];
 }
 
+// Frees every grow-only buffer except `keep` whose id is not pinned.
+static void release_unpinned(cac_ctx* ctx, void** keep) {
+  (void)hipDeviceSynchronize();
+  auto pinned = [&](int id) {
+    for (int p : ctx->pins)
+      if (p == id) return true;
+    return false;
+  };
+  auto drop = [&](int id, void** b, size_t* c) {
+    if (b == keep || !*b || pinned(id)) return;
+    (void)hipFree(*b);
+    *b = nullptr;
+    *c = 0;
+  };
+  drop(kBufKs0, &ctx->ks_buf[0], &ctx->ks_bytes[0]);
+  drop(kBufKs1, &ctx->ks_buf[1], &ctx->ks_bytes[1]);
+  drop(kBufKsRetry, &ctx->ks_retry_buf, &ctx->ks_retry_bytes);
+  drop(kBufSmp, &ctx->smp_buf, &ctx->smp_bytes);
+  drop(kBufLt, &ctx->lt_buf, &ctx->lt_bytes);
+  drop(kBufSpDense, &ctx->sp_dense_buf, &ctx->sp_dense_bytes);
+  drop(kBufSpTmp, &ctx->sp_tmp, &ctx->sp_tmp_bytes);
+  for (size_t d = 0; d < ctx->sp_bufs.size(); ++d) drop(kBufSpDepth + (int)d, &ctx->sp_bufs[d], &ctx->sp_bytes[d]);
+}
+
 int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes) {
   if (bytes <= *cap && *buf) return CAC_OK;
   if (*buf) CAC_HIP(ctx, hipFree(*buf));
@@ -95,7 +119,12 @@ int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes) {
   }
   (void)hipGetLastError();
   *buf = nullptr;
-  CAC_HIP(ctx, hipMalloc(buf, bytes));
+  if (hipMalloc(buf, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *buf = nullptr;
+    release_unpinned(ctx, buf);
+    CAC_HIP(ctx, hipMalloc(buf, bytes));
+  }
   *cap = bytes;
   return CAC_OK;
 }

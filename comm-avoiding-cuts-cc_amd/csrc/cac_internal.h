@@ -49,6 +49,9 @@ struct cac_ctx {
   // merged contracted edge list of one trial.
   void* lt_buf = nullptr;
   size_t lt_bytes = 0;
+  // Grow-only buffers whose contents are live (ids below); the others may be
+  // released when an allocation fails.
+  std::vector<int> pins;
 };
 
 namespace cac {
@@ -64,7 +67,28 @@ int bind(cac_ctx* ctx);
 int scratch_reserve(cac_ctx* ctx, size_t bytes, void** base);
 
 // Grow-only device buffer (*buf, *cap): reallocated only when 'bytes' exceeds it.
+// If the allocation fails, every grow-only buffer of the context that is not
+// pinned is released and the allocation is tried once more.
 int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes);
+
+// Ids of the context's grow-only buffers, for pinning.
+enum : int { kBufKs0 = 0, kBufKs1, kBufKsRetry, kBufSmp, kBufLt, kBufSpDense, kBufSpTmp, kBufSpDepth = 16 };
+
+// Keeps a grow-only buffer from being released while its contents are live.
+struct BufPin {
+  cac_ctx* ctx;
+  int id;
+  BufPin(cac_ctx* c, int i) : ctx(c), id(i) { c->pins.push_back(i); }
+  ~BufPin() {
+    for (size_t k = ctx->pins.size(); k-- > 0;)
+      if (ctx->pins[k] == id) {
+        ctx->pins.erase(ctx->pins.begin() + (long)k);
+        break;
+      }
+  }
+  BufPin(const BufPin&) = delete;
+  BufPin& operator=(const BufPin&) = delete;
+};
 
 // Free device memory plus what the context's grow-only buffers already hold.
 int device_memory_available(cac_ctx* ctx, size_t* bytes);

@@ -738,6 +738,8 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
                                      (int)contract_lds_bytes(512 * kKsChunk, 512));
   });
   CAC_HIP(ctx, attr_err);
+  // parents and children of a level live in the two level buffers
+  BufPin pin0(ctx, kBufKs0), pin1(ctx, kBufKs1);
 
   std::vector<Node> level;
   for (const KsRoot& r : roots) {

@@ -424,6 +424,7 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
     }
     rc = grow_buffer(ctx, &ctx->lt_buf, &ctx->lt_bytes, per + shared);
     if (rc) return rc;
+    BufPin pin(ctx, kBufLt);  // the merged edge lists are the sparse roots
     Carver cv(ctx->lt_buf);
     uint64_t* k0 = cv.take<uint64_t>(mmax);
     uint64_t* k1 = cv.take<uint64_t>(mmax);
@@ -555,6 +556,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) + 1024;
     rc = grow_buffer(ctx, &ctx->smp_buf, &ctx->smp_bytes, bytes);
     if (rc) return rc;
+    BufPin pin(ctx, kBufSmp);  // the sampled matrices are the dense roots
     Carver cv(ctx->smp_buf);
     hipError_t e = hipSuccess;
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);

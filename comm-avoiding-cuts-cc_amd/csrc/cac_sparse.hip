@@ -540,6 +540,7 @@ struct SpRun {
       const size_t count = i1 - i0;
       rc = grow_buffer(ctx, &ctx->sp_dense_buf, &ctx->sp_dense_bytes, mats);
       if (rc) return rc;
+      BufPin pin(ctx, kBufSpDense);
       Carver cv(ctx->sp_dense_buf);
       SpDense* d_nodes = cv.take<SpDense>(count);
       std::vector<SpDense> hd(count);
@@ -598,6 +599,7 @@ struct SpRun {
       const size_t count = i1 - i0, nchild = 2 * count;
       int rc = grow_buffer(ctx, &ctx->sp_bufs[depth], &ctx->sp_bytes[depth], bytes);
       if (rc) return rc;
+      BufPin pin(ctx, kBufSpDepth + (int)depth);  // this chunk's children, until their subtree is done
       rc = grow_buffer(ctx, &ctx->sp_tmp, &ctx->sp_tmp_bytes, tmp);
       if (rc) return rc;
       Carver cv(ctx->sp_bufs[depth]);
