@@ -91,8 +91,8 @@ def kernel_bytes(name: str, m: int, n: int) -> float:
         "af_hubs": 0.0,                    # the low-id prefix ranges (2^16 vertices): negligible
         "af_round": 28.0 * n,              # per vertex: offsets 4, two neighbours 8, own parent read+write 8,
                                            # the two neighbours' parents 8
-        "af_settle": 8.0 * n,              # read + write parent
-        "af_final": 8.0 * n,               # offsets and parent of every vertex
+        "af_mode": 0.0,                    # 4096 sampled roots: negligible
+        "af_final": 12.0 * n,              # offsets and parent of every vertex, parent rewritten to its root
         "af_compress": 8.0 * n,            # read + write labels
     }.get(name, 0.0)
 

@@ -152,19 +152,28 @@ __global__ void __launch_bounds__(kCcBlock) k_cc_runs(const cac_uedge* __restric
   }
 }
 
-__global__ void __launch_bounds__(kCcBlock) k_cc_settle(uint32_t* par, uint32_t n) {
+// Pointer jumping over [0, n): every vertex is re-pointed at its root; returns
+// the number of roots this thread saw.
+__device__ inline uint32_t jump_pass(uint32_t* par, uint32_t n) {
+  uint32_t roots = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
-    uint32_t v = (uint32_t)i;
-    uint32_t p = ld_par_fresh(par, v);
-    if (p == v) continue;
-    uint32_t r = uf_find_fresh(par, p);
+    const uint32_t v = (uint32_t)i;
+    const uint32_t p = ld_par_fresh(par, v);
+    if (p == v) {
+      ++roots;
+      continue;
+    }
+    const uint32_t r = uf_find_fresh(par, p);
     if (r != p) st_par(par, v, r);
   }
+  return roots;
 }
 
-// Most frequent root among kModeSamples vertices spread over [0, n) (parents
-// are settled, so par[v] is v's root).  Only a performance choice: any root
-// gives correct labels.
+__global__ void __launch_bounds__(kCcBlock) k_cc_settle(uint32_t* par, uint32_t n) { (void)jump_pass(par, n); }
+
+// Most frequent root among kModeSamples vertices spread over [0, n), each
+// sample climbing to its root (one hop on a settled forest).  Only a
+// performance choice: any root gives correct labels.
 __global__ void __launch_bounds__(1024) k_cc_mode(const uint32_t* __restrict__ par, uint32_t n,
                                                   uint32_t* __restrict__ out_root) {
   __shared__ uint32_t key[kModeTable], cnt[kModeTable];
@@ -179,8 +188,11 @@ __global__ void __launch_bounds__(1024) k_cc_mode(const uint32_t* __restrict__ p
     h ^= h >> 15;
     h *= 0x85EBCA6Bu;
     h ^= h >> 13;
-    const uint32_t v = (uint32_t)(((uint64_t)h * n) >> 32);
-    const uint32_t r = ld_par_fresh(par, v);
+    uint32_t r = (uint32_t)(((uint64_t)h * n) >> 32), up = ld_par_fresh(par, r);
+    while (up < r) {
+      r = up;
+      up = ld_par_fresh(par, r);
+    }
     uint32_t slot = (r * 0x9E3779B1u) >> 19;  // 13 bits = kModeTable slots
     for (;;) {
       const uint32_t old = atomicCAS(&key[slot], ~0u, r);
@@ -260,17 +272,7 @@ __global__ void __launch_bounds__(kCcBlock) k_cc_hook(const cac_uedge* __restric
 
 // Final pointer jumping; counts roots (components).
 __global__ void __launch_bounds__(kCcBlock) k_cc_compress(uint32_t* par, uint32_t n, uint32_t* ncomp) {
-  uint32_t roots = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
-    uint32_t v = (uint32_t)i;
-    uint32_t p = ld_par_fresh(par, v);
-    if (p == v) {
-      ++roots;
-      continue;
-    }
-    uint32_t r = uf_find_fresh(par, p);
-    if (r != p) st_par(par, v, r);
-  }
+  uint32_t roots = jump_pass(par, n);
   // block reduction, then one atomic per block: same-address atomics serialise,
   // so one per wave (2^18 of them at n = 2^24) would dominate this kernel
   __shared__ uint32_t wave_sum[kCcBlock / 64];
@@ -342,8 +344,12 @@ __global__ void __launch_bounds__(kCcBlock) k_af_round(const uint32_t* __restric
   }
 }
 
-// Vertices whose settled root is not the sampled root link their neighbours
-// from index kAfRounds on; long lists are queued for k_af_heavy.
+// Every vertex is re-pointed at its root (the pointer jumping of jump_pass,
+// fused here); those whose root is not the sampled root R then link their
+// neighbours from index kAfRounds on, and long lists are queued for
+// k_af_heavy.  A root found here may be hooked under a smaller one by a
+// concurrent link: it stays an ancestor, so the store is still valid, and the
+// vertex at worst links neighbours that were already joined.
 __global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cols, uint32_t n, uint32_t* par,
                                                        const uint32_t* __restrict__ root, uint32_t* heavy,
@@ -351,8 +357,10 @@ __global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restric
   const uint32_t R = root[0];
   for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
     const uint32_t u = (uint32_t)i;
-    const uint32_t b = off[u] + kAfRounds, e = off[u + 1];
-    if (b >= e || ld_par_fresh(par, u) == R) continue;
+    const uint32_t p = ld_par_fresh(par, u), b = off[u] + kAfRounds, e = off[u + 1];
+    const uint32_t r = p == u ? u : uf_find_fresh(par, p);
+    if (r != p) st_par(par, u, r);
+    if (b >= e || r == R) continue;
     if (e - b > kAfHeavy) {
       heavy[atomicAdd(heavy_count, 1u)] = u;
       continue;
@@ -529,7 +537,7 @@ int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, u
   // one hop long (0.99 ms -> 0.58 ms for the round on R-MAT 2^24 / 2^28).
   prof_begin(ctx, "af_hubs", s);
   uint32_t lo = 0;
-  for (uint32_t hb : {1u << 8, 1u << 12, 1u << 16}) {
+  for (uint32_t hb : {1u << 12, 1u << 16}) {
     const uint32_t hi = n < hb ? n : hb;
     if (hi <= lo) continue;
     hipLaunchKernelGGL(k_af_round<false>, dim3(grid_for(hi - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols,
@@ -544,12 +552,11 @@ int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, u
                        n, d_lab);
     prof_end(ctx, s);
   }
-  prof_begin(ctx, "af_settle", s);
-  hipLaunchKernelGGL(k_cc_settle, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, n);
+  prof_begin(ctx, "af_mode", s);
   hipLaunchKernelGGL(k_cc_mode, dim3(1), dim3(1024), 0, s, (const uint32_t*)d_lab, n, d_counter + 1);
   prof_end(ctx, s);
   prof_begin(ctx, "af_final", s);
-  hipLaunchKernelGGL(k_af_final, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols, n, d_lab,
+  hipLaunchKernelGGL(k_af_final, dim3(grid_for(n, kCcBlock, 4096)), dim3(kCcBlock), 0, s, d_off, d_cols, n, d_lab,
                      (const uint32_t*)(d_counter + 1), d_heavy, d_counter + 2);
   hipLaunchKernelGGL(k_af_heavy, dim3(1024), dim3(kCcBlock), 0, s, d_off, d_cols, d_lab, (const uint32_t*)d_heavy,
                      (const uint32_t*)(d_counter + 2));
