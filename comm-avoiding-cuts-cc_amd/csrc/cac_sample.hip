@@ -21,6 +21,7 @@
 // which feeds one dense Karger-Stein trial per sqrt-cut trial.
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 #include <vector>
 
 #include <cstring>
@@ -53,7 +54,36 @@ struct TrialScratch {
   int64_t* mat;           // [t x t]
   int64_t* rsum;          // [t] row sums of mat (may be null)
   uint32_t* nv_out;       // [1]
+  // [1] of the job's graph, nonzero when an edge is a loop or two weights
+  // differ (may be null: then every round builds the prefix)
+  const uint32_t* mixed;
 };
+
+// A distinct graph of a sampling launch and its flag word.
+struct EdgeSpan {
+  const cac_edge* edges;
+  uint64_t m;
+  uint32_t* mixed;
+};
+
+__global__ void k_mixed_zero(const EdgeSpan* spans, uint32_t count) {
+  for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < count; g += gridDim.x * 256)
+    __hip_atomic_store(spans[g].mixed, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid (slices, graphs): flags graph blockIdx.y when one of its edges is a loop
+// or weighs other than its first edge (or the first weighs 0).
+__global__ void __launch_bounds__(256) k_mixed_weights(const EdgeSpan* spans) {
+  const EdgeSpan sp = spans[blockIdx.y];
+  if (sp.m == 0) return;
+  const uint64_t w0 = sp.edges[0].w;
+  bool bad = w0 == 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < sp.m; e += (uint64_t)gridDim.x * 256) {
+    const cac_edge x = sp.edges[e];
+    bad = bad || x.w != w0 || x.u == x.v;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(sp.mixed, 1u);
+}
 
 __device__ inline uint64_t wsum64(uint64_t x) {
 #pragma unroll
@@ -95,9 +125,18 @@ __device__ inline bool uf_union(uint32_t* par, uint32_t a, uint32_t b) {
   return false;
 }
 
-// Union-find over the first L draws from a clean forest; returns the hooks made.
-__device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t* par, uint32_t n, uint64_t* lds) {
-  for (uint32_t x = threadIdx.x; x < n; x += kSmpBlock) par[x] = x;
+// Union-find over the first L draws; returns the hooks made.  Every label of
+// the round starts as its own root: at the round's first probe they are the
+// roots the previous round left (undo = 0), and a later probe first undoes the
+// previous one, whose first `undo` draws' endpoints are the only entries it
+// changed (hooks join roots found from endpoints, and path splitting only
+// re-points nodes on their paths, all endpoints).
+__device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t undo, uint32_t* par, uint64_t* lds) {
+  for (uint32_t d = threadIdx.x; d < undo; d += kSmpBlock) {
+    const uint2 x = draws[d];
+    par[x.x] = x.x;
+    par[x.y] = x.y;
+  }
   block_sync_global();  // no wave may see the previous probe's forest
   uint64_t mine = 0;
   for (uint32_t d = threadIdx.x; d < L; d += kSmpBlock) mine += uf_union(par, draws[d].x, draws[d].y) ? 1 : 0;
@@ -111,14 +150,23 @@ __global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, cons
   const cac_edge* __restrict__ edges = S.edges;
   const uint64_t m = S.m;
   const uint32_t n = S.n, t = S.t, trial = S.trial;
-  for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = v;
+  for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) {
+    S.lab[v] = v;
+    S.par[v] = v;
+  }
   block_sync_global();
+  // One weight wu on every edge and no loop: the first round's prefix is
+  // wu * (e + 1), so its draws need neither the prefix nor a search.
+  uint64_t wu = 0;
+  if (S.mixed && m > 0 && __hip_atomic_load(S.mixed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    wu = edges[0].w;
   uint32_t nv = n;
   for (uint32_t round = 0; nv > t; ++round) {
+    const bool flat = round == 0 && wu != 0;
     // 1. masked inclusive prefix of the weights
-    uint64_t carry = 0;
+    uint64_t carry = flat ? wu * m : 0;
     constexpr int kItems = 4;
-    for (uint64_t base = 0; base < m; base += (uint64_t)kSmpBlock * kItems) {
+    for (uint64_t base = 0; !flat && base < m; base += (uint64_t)kSmpBlock * kItems) {
       const uint64_t e0 = base + (uint64_t)threadIdx.x * kItems;
       uint64_t w[kItems], s = 0;
 #pragma unroll
@@ -166,7 +214,7 @@ __global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, cons
     for (uint32_t d = threadIdx.x; d < D; d += kSmpBlock) {
       const u32x4 rnd = philox4x32(trial, round, d, 0u, seed, kDomSample);
       const uint64_t r = uniform1(lo64(rnd), W);
-      uint64_t lo = 0, hi = m - 1;
+      uint64_t lo = flat ? (r - 1) / wu : 0, hi = flat ? lo : m - 1;
       while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if (S.pre[mid] >= r) hi = mid;
@@ -177,18 +225,18 @@ __global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, cons
     }
     block_sync_global();
     // 3. shortest draw prefix that leaves t super-vertices
-    uint32_t L = D;
-    const uint32_t all = probe(S.draws, D, S.par, n, lds);
-    if (nv - all <= t) {
-      uint32_t lo = 1, hi = D;
+    uint32_t hooks = probe(S.draws, D, 0, S.par, lds);
+    if (nv - hooks <= t) {
+      uint32_t lo = 1, hi = D, last = D;
       while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
-        if (nv - probe(S.draws, mid, S.par, n, lds) <= t) hi = mid;
+        hooks = probe(S.draws, mid, last, S.par, lds);
+        last = mid;
+        if (nv - hooks <= t) hi = mid;
         else lo = mid + 1;
       }
-      L = lo;
-    }
-    const uint32_t hooks = probe(S.draws, L, S.par, n, lds);
+      if (last != lo) hooks = probe(S.draws, lo, last, S.par, lds);
+    }  // else all D draws are kept, and the forest of the first probe stays
     // 4. canonical relabel
     for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = uf_root(S.par, S.lab[v]);
     block_sync_global();
@@ -330,6 +378,39 @@ __global__ void k_lt_set(int64_t* p, const unsigned long long* v) {
   if (threadIdx.x == 0) p[0] = (int64_t)v[0];
 }
 
+
+// Points scr[i].mixed at a flag word of its graph (one per distinct edge list)
+// and launches the kernels that set the flags; spans and flags hold scr.size()
+// entries.  The flags are ready for any later launch on s.
+int launch_mixed_flags(cac_ctx* ctx, std::vector<TrialScratch>& scr, EdgeSpan* d_spans, uint32_t* d_flags,
+                       hipStream_t s) {
+  std::unordered_map<const cac_edge*, uint32_t> index;
+  std::vector<EdgeSpan> spans;
+  uint64_t mmax = 0;
+  for (TrialScratch& x : scr) {
+    auto it = index.find(x.edges);
+    uint32_t g = 0;
+    if (it == index.end()) {
+      g = (uint32_t)spans.size();
+      index.emplace(x.edges, g);
+      spans.push_back({x.edges, x.m, d_flags + g});
+      mmax = std::max(mmax, x.m);
+    } else {
+      g = it->second;
+    }
+    x.mixed = d_flags + g;
+  }
+  if (spans.empty()) return CAC_OK;
+  const uint32_t count = (uint32_t)spans.size();
+  CAC_HIP(ctx, hipMemcpyAsync(d_spans, spans.data(), count * sizeof(EdgeSpan), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mixed_zero, dim3(grid_for(count, 256, 64)), dim3(256), 0, s, (const EdgeSpan*)d_spans, count);
+  for (uint32_t g0 = 0; g0 < count; g0 += 65535u)  // grid y holds at most 65535 graphs
+    hipLaunchKernelGGL(k_mixed_weights, dim3(grid_for(mmax, 256 * 16, 256), std::min(count - g0, 65535u)), dim3(256),
+                       0, s, (const EdgeSpan*)(d_spans + g0));
+  CAC_LAUNCH_CHECK(ctx, "weight flags");
+  return CAC_OK;
+}
+
 }  // namespace
 
 // Host rules, same floating point as the reference (SquareRootCut.cpp:63-83).
@@ -386,7 +467,7 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
     const uint64_t m = std::max<uint64_t>(j.m, 1);
     return carve_size(sizeof(TrialScratch)) + 3 * carve_size((size_t)j.n * 4) + carve_size(m * 8) +
            carve_size((size_t)j.n * 8) + carve_size(64) + carve_size(m * sizeof(cac_edge)) +
-           carve_size(((size_t)j.t + 2) * 8);
+           carve_size(((size_t)j.t + 2) * 8) + carve_size(sizeof(EdgeSpan)) + carve_size(4);
   };
   size_t avail = 0;
   int rc = device_memory_available(ctx, &avail);
@@ -435,6 +516,8 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
     void* tmp = cv.take<unsigned char>(std::max(sort_tmp, red_tmp));
     uint32_t* partial = cv.take<uint32_t>(scan_tiles(nmaxs) + 64);
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
+    EdgeSpan* d_spans = cv.take<EdgeSpan>(nb);
+    uint32_t* d_flags = cv.take<uint32_t>(nb);
     std::vector<TrialScratch> scr(nb);
     std::vector<uint32_t*> rank(nb);
     std::vector<cac_edge*> merged(nb);
@@ -458,6 +541,8 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
       merged[i] = cv.take<cac_edge>(std::max<uint64_t>(j.m, 1));
       deg[i] = cv.take<unsigned long long>((size_t)j.t + 2);  // [t] degrees, [t] count, [t + 1] min
     }
+    rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
+    if (rc) return rc;
     CAC_HIP(ctx, hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s));
     prof_begin(ctx, "sample_trials", s);
     hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
@@ -553,7 +638,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       ks_bytes += k;
       ++nb;
     }
-    const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) + 1024;
+    const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) +
+                         carve_size(nb * sizeof(EdgeSpan)) + carve_size(nb * 4) + 1024;
     rc = grow_buffer(ctx, &ctx->smp_buf, &ctx->smp_bytes, bytes);
     if (rc) return rc;
     BufPin pin(ctx, kBufSmp);  // the sampled matrices are the dense roots
@@ -562,6 +648,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
     uint32_t* d_nv = cv.take<uint32_t>(nb);
     int64_t* d_bounds = cv.take<int64_t>(nb);
+    EdgeSpan* d_spans = cv.take<EdgeSpan>(nb);
+    uint32_t* d_flags = cv.take<uint32_t>(nb);
     std::vector<TrialScratch> scr(nb);
     for (size_t i = 0; i < nb; ++i) {
       const SqrtJob& j = jobs[b0 + i];
@@ -580,6 +668,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       x.rsum = cv.take<int64_t>(j.t);
       x.nv_out = d_nv + i;
     }
+    rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
+    if (rc) return rc;
     e = hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
     if (rc == CAC_OK) {
@@ -632,10 +722,11 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
                         uint32_t trial, SampleOut* out, hipStream_t s) {
   *out = SampleOut{};
   const size_t bytes = carve_size((size_t)n * 4) * 2 + carve_size(std::max<uint64_t>(m, 1) * 8) +
-                       carve_size((size_t)n * 8) + 4 * carve_size(64) + carve_size(sizeof(TrialScratch));
+                       carve_size((size_t)n * 8) + 4 * carve_size(64) + carve_size(sizeof(TrialScratch)) +
+                       carve_size(sizeof(EdgeSpan)) + carve_size(4);
   CAC_HIP(ctx, hipMalloc(&out->arena, bytes));
   Carver cv(out->arena);
-  TrialScratch h;
+  TrialScratch h{};
   h.edges = d_edges;
   h.m = m;
   h.n = n;
@@ -649,8 +740,14 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   h.draws = cv.take<uint2>(n);
   h.nv_out = cv.take<uint32_t>(1);
   int64_t* d_misc = cv.take<int64_t>(2);
+  EdgeSpan* d_span = cv.take<EdgeSpan>(1);
+  uint32_t* d_flag = cv.take<uint32_t>(1);
   h.mat = nullptr;
   h.rsum = nullptr;
+  std::vector<TrialScratch> one{h};
+  int rc = launch_mixed_flags(ctx, one, d_span, d_flag, s);
+  if (rc) return rc;
+  h = one[0];
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
   CAC_LAUNCH_CHECK(ctx, "sample trial");
