@@ -21,6 +21,7 @@
 // which feeds one dense Karger-Stein trial per sqrt-cut trial.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -245,14 +246,19 @@ __global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, cons
   if (threadIdx.x == 0) *S.nv_out = nv;
 }
 
+// Dynamic LDS of k_densify_trials: the row sums of a matrix of up to kKsMaxS rows.
+constexpr size_t kDensifyLds = (size_t)kKsMaxS * 8;
+constexpr int kDensifyEdges = 4;  // edges per thread in flight
+
 // Rank the canonical labels, accumulate the surviving edges into the k x k
-// matrix (k == t unless the sampling stopped early on a disconnected graph) and
-// compute the trial's bound (minimum weighted degree).
-// bounds is indexed by job (blockIdx.x), results by the job's slot.
+// matrix (k == t unless the sampling stopped early on a disconnected graph),
+// with its row sums in LDS, and compute the trial's bound (minimum weighted
+// degree).  bounds is indexed by job (blockIdx.x), results by the job's slot.
 __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch* __restrict__ scratch, int64_t* bounds,
                                                               int64_t* results) {
   __shared__ uint32_t wtot[kSmpWaves];
   __shared__ uint64_t lds[kSmpWaves + 2];
+  extern __shared__ unsigned long long row_sum[];  // [k]
   const TrialScratch S = scratch[blockIdx.x];
   const cac_edge* __restrict__ edges = S.edges;
   const uint64_t m = S.m;
@@ -289,23 +295,61 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
     __syncthreads();
   }
   for (uint64_t i = threadIdx.x; i < (uint64_t)k * k; i += kSmpBlock) S.mat[i] = 0;
+  const bool lds_rows = k <= kKsMaxS;  // else the row sums are read back from the matrix
+  for (uint32_t r = threadIdx.x; lds_rows && r < k; r += kSmpBlock) row_sum[r] = 0;
   block_sync_global();  // ranks and zeroed rows before the accumulation
-  for (uint64_t e = threadIdx.x; e < m; e += kSmpBlock) {
-    const cac_edge x = edges[e];
-    const uint32_t a = rank[S.lab[x.u]], b = rank[S.lab[x.v]];
-    if (a == b) continue;
-    atomicAdd((unsigned long long*)&S.mat[(uint64_t)a * k + b], (unsigned long long)x.w);
-    atomicAdd((unsigned long long*)&S.mat[(uint64_t)b * k + a], (unsigned long long)x.w);
+  // kDensifyEdges edges per thread: their label and rank loads are in flight together
+  for (uint64_t base = 0; base < m; base += (uint64_t)kSmpBlock * kDensifyEdges) {
+    cac_edge x[kDensifyEdges];
+    uint32_t a[kDensifyEdges], b[kDensifyEdges];
+#pragma unroll
+    for (int j = 0; j < kDensifyEdges; ++j) {
+      const uint64_t e = base + (uint64_t)j * kSmpBlock + threadIdx.x;
+      x[j] = e < m ? edges[e] : cac_edge{0u, 0u, 0ull};  // vertex 0 stands in: same label, skipped
+    }
+#pragma unroll
+    for (int j = 0; j < kDensifyEdges; ++j) {
+      a[j] = S.lab[x[j].u];
+      b[j] = S.lab[x[j].v];
+    }
+#pragma unroll
+    for (int j = 0; j < kDensifyEdges; ++j) {
+      a[j] = rank[a[j]];
+      b[j] = rank[b[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kDensifyEdges; ++j) {
+      if (a[j] == b[j]) continue;
+      const unsigned long long w = x[j].w;
+      atomicAdd((unsigned long long*)&S.mat[(uint64_t)a[j] * k + b[j]], w);
+      atomicAdd((unsigned long long*)&S.mat[(uint64_t)b[j] * k + a[j]], w);
+      if (lds_rows) {
+        atomicAdd(&row_sum[a[j]], w);
+        atomicAdd(&row_sum[b[j]], w);
+      }
+    }
   }
   block_sync_global();
-  // row sums (the Karger-Stein capacities) and bound = min row sum: one wave per row
+  // row sums (the Karger-Stein capacities) and bound = min row sum
   uint64_t best = ~0ull;
-  for (uint32_t r = threadIdx.x >> 6; r < k; r += kSmpWaves) {
-    uint64_t acc = 0;
-    for (uint32_t x = threadIdx.x & 63; x < k; x += 64) acc += (uint64_t)S.mat[(uint64_t)r * k + x];
-    acc = wsum64(acc);
-    if (S.rsum && (threadIdx.x & 63) == 0) S.rsum[r] = (int64_t)acc;
-    best = acc < best ? acc : best;
+  if (lds_rows) {
+    for (uint32_t r = threadIdx.x; r < k; r += kSmpBlock) {
+      const uint64_t acc = row_sum[r];
+      if (S.rsum) S.rsum[r] = (int64_t)acc;
+      best = acc < best ? acc : best;
+    }
+  } else {  // one wave per row
+    for (uint32_t r = threadIdx.x >> 6; r < k; r += kSmpWaves) {
+      uint64_t acc = 0;
+      for (uint32_t x = threadIdx.x & 63; x < k; x += 64) acc += (uint64_t)S.mat[(uint64_t)r * k + x];
+      acc = wsum64(acc);
+      if (S.rsum && (threadIdx.x & 63) == 0) S.rsum[r] = (int64_t)acc;
+      best = acc < best ? acc : best;
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint64_t o = __shfl_xor(best, d, 64);
+    best = o < best ? o : best;
   }
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = best;
   __syncthreads();
@@ -408,6 +452,19 @@ int launch_mixed_flags(cac_ctx* ctx, std::vector<TrialScratch>& scr, EdgeSpan* d
     hipLaunchKernelGGL(k_mixed_weights, dim3(grid_for(mmax, 256 * 16, 256), std::min(count - g0, 65535u)), dim3(256),
                        0, s, (const EdgeSpan*)(d_spans + g0));
   CAC_LAUNCH_CHECK(ctx, "weight flags");
+  return CAC_OK;
+}
+
+
+// The densify kernel's dynamic LDS exceeds the default limit: raise it once.
+int densify_attr(cac_ctx* ctx) {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    err = hipFuncSetAttribute((const void*)k_densify_trials, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kDensifyLds);
+  });
+  CAC_HIP(ctx, err);
   return CAC_OK;
 }
 
@@ -677,8 +734,10 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
       prof_end(ctx, s);
       prof_begin(ctx, "densify_trials", s);
-      hipLaunchKernelGGL(k_densify_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, (const TrialScratch*)d_scr,
-                         d_bounds, d_results);
+      rc = densify_attr(ctx);
+      if (rc == CAC_OK)
+        hipLaunchKernelGGL(k_densify_trials, dim3((unsigned)nb), dim3(kSmpBlock), kDensifyLds, s,
+                           (const TrialScratch*)d_scr, d_bounds, d_results);
       prof_end(ctx, s);
       e = hipGetLastError();
       if (e != hipSuccess) rc = hip_fail(ctx, e, "sampling kernels");
@@ -757,7 +816,9 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   CAC_HIP(ctx, hipMalloc((void**)&h.mat, std::max<size_t>((size_t)k * k, 1) * 8));
   h.t = k;
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), 0, s, (const TrialScratch*)d_scr, d_misc,
+  rc = densify_attr(ctx);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), kDensifyLds, s, (const TrialScratch*)d_scr, d_misc,
                      d_misc + 1);
   CAC_LAUNCH_CHECK(ctx, "densify trial");
   CAC_HIP(ctx, hipStreamSynchronize(s));
