@@ -46,6 +46,37 @@ __device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
   }
 }
 
+// One DPP move of a 64-bit value as two 32-bit halves; lanes whose source row
+// is masked off or out of range read 0, the identity of an unsigned maximum.
+template <int Ctrl, int RowMask>
+__device__ inline uint64_t dpp_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, Ctrl, RowMask, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Unsigned maximum over the wave, returned to every lane: inside each row of 16
+// lanes by two quad permutations and two row rotations, then across rows by
+// the row broadcasts of lane 15 (into rows 1 and 3) and lane 31 (into rows 2
+// and 3), which leave the total in lane 63.  VALU data moves only, no LDS.
+__device__ inline uint64_t wave_max_u64(uint64_t v) {
+  uint64_t o = dpp_u64<0xb1, 0xf>(v);  // quad_perm [1,0,3,2]
+  v = o > v ? o : v;
+  o = dpp_u64<0x4e, 0xf>(v);  // quad_perm [2,3,0,1]
+  v = o > v ? o : v;
+  o = dpp_u64<0x124, 0xf>(v);  // row_ror:4
+  v = o > v ? o : v;
+  o = dpp_u64<0x128, 0xf>(v);  // row_ror:8
+  v = o > v ? o : v;
+  o = dpp_u64<0x142, 0xa>(v);  // row_bcast:15
+  v = o > v ? o : v;
+  o = dpp_u64<0x143, 0xc>(v);  // row_bcast:31
+  v = o > v ? o : v;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ inline int64_t wave_sum_i64(int64_t x) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
@@ -170,36 +201,64 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
   const KsLeaf lf = leaves[sw_list[blockIdx.x]];
   const uint32_t V = lf.k;
   const int lane = threadIdx.x;
+  int64_t total = 0;  // sum of all entries: no key of the maximum adjacency order exceeds it
   for (uint32_t r = 0; r < V; ++r)
-    for (uint32_t x = lane; x < V; x += 64) A[r * kLeafMax + x] = lf.mat[(size_t)r * V + x];
+    for (uint32_t x = lane; x < V; x += 64) {
+      const int64_t a = lf.mat[(size_t)r * V + x];
+      A[r * kLeafMax + x] = a;
+      total += a;
+    }
+  total = wave_sum_i64(total);
+  // keys below 2^55 pack with their vertex as (key << 8) | (255 - vertex), so the
+  // largest key, then the smallest vertex, is one unsigned wave maximum
+  const bool packed = total < (int64_t)1 << 55;
   __syncthreads();
   bool alive[2] = {(uint32_t)lane < V, (uint32_t)(lane + 64) < V};
   int64_t best = INT64_MAX;
   uint32_t nalive = V;
   // PR-1: repeatedly contract an edge of weight >= bound (the lowest live row
   // holding one, its lowest live column); merged-away columns hold stale values
-  // and are skipped
-  while (nalive > 1) {
+  // and are skipped.  col[h] is the first such column of row lane + 64 h.  A
+  // merge of last into prev (prev < last) changes only row prev and column
+  // prev among the live entries, and only upwards, so a row other than prev
+  // keeps its first column unless column prev now qualifies before it; a row
+  // whose first column was last finds prev there (it took last's weight), and
+  // row prev alone is scanned again.
+  auto first_col = [&](uint32_t u) {
     const uint64_t live_lo = __ballot(alive[0]), live_hi = __ballot(alive[1]);
-    uint32_t col[2] = {kNoSlot, kNoSlot};
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t u = lane + 64 * h;
-      if (!alive[h]) continue;
-      for (uint32_t x = 0; x < V; ++x) {
-        const bool live_x = ((x < 64 ? live_lo : live_hi) >> (x & 63)) & 1ull;
-        if (live_x && x != u && A[u * kLeafMax + x] >= lf.bound) {
-          col[h] = x;
-          break;
-        }
+    uint32_t c = kNoSlot;
+    for (uint32_t x = 0; x < V; ++x) {
+      const bool live_x = ((x < 64 ? live_lo : live_hi) >> (x & 63)) & 1ull;
+      if (live_x && x != u && A[u * kLeafMax + x] >= lf.bound) {
+        c = x;
+        break;
       }
     }
+    return c;
+  };
+  uint32_t col[2] = {kNoSlot, kNoSlot};
+  {
+    const uint32_t c0 = first_col((uint32_t)lane), c1 = first_col((uint32_t)lane + 64);
+    col[0] = alive[0] ? c0 : kNoSlot;
+    col[1] = alive[1] ? c1 : kNoSlot;
+  }
+  while (nalive > 1) {
     const uint64_t lo_mask = __ballot(col[0] != kNoSlot), hi_mask = __ballot(col[1] != kNoSlot);
     if (!lo_mask && !hi_mask) break;
     const uint32_t u = lo_mask ? (uint32_t)(__ffsll((long long)lo_mask) - 1)
                                : 64u + (uint32_t)(__ffsll((long long)hi_mask) - 1);
     const uint32_t x = __shfl(u < 64 ? col[0] : col[1], (int)(u & 63), 64);
-    leaf_merge(A, V, u < x ? u : x, u < x ? x : u, alive);
+    const uint32_t prev = u < x ? u : x, last = u < x ? x : u;
+    leaf_merge(A, V, prev, last, alive);
     --nalive;
+    const uint32_t again = first_col(prev);  // the same on every lane; kept by prev's owner
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t r = lane + 64 * h;
+      if (!alive[h]) col[h] = kNoSlot;
+      else if (r == prev) col[h] = again;
+      else if (col[h] == last) col[h] = prev;
+      else if (prev < col[h] && A[r * kLeafMax + prev] >= lf.bound) col[h] = prev;
+    }
   }
   if (lf.bound <= 0) nalive = 1;  // a non-positive bound is already the answer
   while (nalive > 1) {
@@ -217,13 +276,23 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     for (uint32_t step = 1; step < nalive; ++step) {
       int64_t bk = INT64_MIN;
       uint32_t bi = 0xFFFFFFFFu;
-      for (int h = 0; h < 2; ++h) {
-        if (!added[h] && (key[h] > bk || (key[h] == bk && (uint32_t)lane + 64 * h < bi))) {
-          bk = key[h];
-          bi = lane + 64 * h;
+      if (packed) {
+        const uint64_t p0 = added[0] ? 0 : ((uint64_t)key[0] << 8) | (255u - (uint32_t)lane);
+        const uint64_t p1 = added[1] ? 0 : ((uint64_t)key[1] << 8) | (255u - (uint32_t)lane - 64u);
+        const uint64_t b = wave_max_u64(p0 > p1 ? p0 : p1);
+        if (b != 0) {
+          bk = (int64_t)(b >> 8);
+          bi = 255u - (uint32_t)(b & 255u);
         }
+      } else {
+        for (int h = 0; h < 2; ++h) {
+          if (!added[h] && (key[h] > bk || (key[h] == bk && (uint32_t)lane + 64 * h < bi))) {
+            bk = key[h];
+            bi = lane + 64 * h;
+          }
+        }
+        wave_argmax(bk, bi);
       }
-      wave_argmax(bk, bi);
       if (bi >= V) break;  // no candidate: cannot happen for step < nalive
       prev = last;
       last = bi;

@@ -37,6 +37,32 @@ def test_sample_contract_matches_oracle(ctx, orc, n, t, seed):
     assert np.array_equal(merged, want)
 
 
+@pytest.mark.parametrize("kind", ["one-weight", "one-weight-loop", "one-heavier", "one-weight-large"])
+def test_sampling_first_round_forms(ctx, orc, kind):
+    """A graph whose edges all weigh the same and hold no loop takes the first
+    round's draws by division instead of the prefix search; a loop or a second
+    weight keeps the search.  Both must give the oracle's labels and cuts."""
+    rng = np.random.default_rng(len(kind))
+    n = 400
+    e = erdos_renyi(rng, n, 10.0 / n)
+    e[:, 2] = 7 if kind != "one-weight-large" else 4_000_000_000
+    if kind == "one-weight-loop":
+        e = np.concatenate([e, np.array([[5, 5, 7]], dtype=np.int64)])
+    if kind == "one-heavier":
+        e[len(e) // 2, 2] = 8
+    t = 60
+    _say(f"first-round forms {kind}: n {n} m {len(e)} t {t}")
+    vmap, k, _ = ctx.sample_contract(e, n, t, 3, 2)
+    lab, nv = orc.sample_contract(n, e, t, 3, 2)
+    assert k == nv
+    roots = np.flatnonzero(lab == np.arange(n))
+    rank = np.zeros(n, dtype=np.int64)
+    rank[roots] = np.arange(len(roots))
+    assert np.array_equal(vmap, rank[lab])
+    raw = ctx.sqrt_cut_trials(e, n, t, 11, 0, 6)
+    assert raw.tolist() == [orc.sqrt_trial(n, e, t, 11, k) for k in range(6)]
+
+
 @pytest.mark.parametrize("n,seed", [(200, 0), (1000, 1)])
 def test_sqrt_trials_bit_exact(ctx, orc, n, seed):
     rng = np.random.default_rng(seed)
