@@ -52,6 +52,7 @@ struct cac_ctx {
   // Grow-only buffers whose contents are live (ids below); the others may be
   // released when an allocation fails.
   std::vector<int> pins;
+  size_t smp_wave = 0;  // sampling workgroups resident at once (0: not asked yet)
 };
 
 namespace cac {

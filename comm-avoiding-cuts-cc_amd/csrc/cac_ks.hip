@@ -411,9 +411,12 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
   return true;
 }
 
+// Below 1024 threads, at most 80 VGPRs (6 waves per SIMD): one more workgroup
+// per CU for the mid-sized children.
 template <int B>
-__global__ void __launch_bounds__(B) k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t seed,
-                                                   uint32_t* __restrict__ k_out, int64_t* __restrict__ results) {
+__global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
+    k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t seed, uint32_t* __restrict__ k_out,
+                  int64_t* __restrict__ results) {
   extern __shared__ __align__(16) unsigned char smem[];
   const KsChild c = children[blockIdx.x];
   const uint32_t s = c.s, nb = (s + 63) / 64;

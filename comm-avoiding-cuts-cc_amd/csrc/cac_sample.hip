@@ -145,7 +145,13 @@ __device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t undo, uint32_
   return (uint32_t)bsum64(mine, lds);
 }
 
-__global__ void __launch_bounds__(kSmpBlock) k_sample_trials(uint32_t seed, const TrialScratch* __restrict__ scratch) {
+// kMinWaves = 8 holds the kernel to 64 VGPRs (8 waves per SIMD), so two 16-wave
+// workgroups share a CU, at the price of a few spilled registers: for launches
+// of more workgroups than CUs.  kMinWaves = 1 leaves the registers free, for
+// small launches whose trials each have a CU to themselves.
+template <int kMinWaves>
+__global__ void __launch_bounds__(kSmpBlock, kMinWaves)
+    k_sample_trials(uint32_t seed, const TrialScratch* __restrict__ scratch) {
   __shared__ uint64_t lds[kSmpWaves + 2];
   const TrialScratch S = scratch[blockIdx.x];
   const cac_edge* __restrict__ edges = S.edges;
@@ -468,6 +474,37 @@ int densify_attr(cac_ctx* ctx) {
   return CAC_OK;
 }
 
+
+// Sampling workgroups the device runs at once (workgroups per CU times CUs).
+size_t sampling_wave(cac_ctx* ctx) {
+  if (ctx->smp_wave == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_trials<8>, kSmpBlock, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
+      (void)hipGetLastError();
+      per_cu = 1;
+      cus = 1;
+    }
+    ctx->smp_wave = (size_t)std::max(per_cu, 1) * (size_t)std::max(cus, 1);
+  }
+  return ctx->smp_wave;
+}
+
+
+// One sampling launch of nb trials, in the form suited to its size.
+void launch_sampling(cac_ctx* ctx, size_t nb, uint32_t seed, const TrialScratch* d_scr, hipStream_t s) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
+    (void)hipGetLastError();
+    cus = 1;
+  }
+  if (nb > (size_t)cus)
+    hipLaunchKernelGGL(k_sample_trials<8>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, d_scr);
+  else
+    hipLaunchKernelGGL(k_sample_trials<1>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, d_scr);
+}
+
 }  // namespace
 
 // Host rules, same floating point as the reference (SquareRootCut.cpp:63-83).
@@ -602,7 +639,7 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
     if (rc) return rc;
     CAC_HIP(ctx, hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s));
     prof_begin(ctx, "sample_trials", s);
-    hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
+    launch_sampling(ctx, nb, seed, d_scr, s);
     prof_end(ctx, s);
     CAC_LAUNCH_CHECK(ctx, "sample trials");
     std::vector<uint32_t> nv(nb);
@@ -695,6 +732,10 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       ks_bytes += k;
       ++nb;
     }
+    // whole waves of sampling workgroups: a last partial wave would leave most
+    // of the GPU idle while it runs
+    const size_t wave = sampling_wave(ctx);
+    if (b0 + nb < jobs.size() && nb > wave) nb -= nb % wave;
     const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) +
                          carve_size(nb * sizeof(EdgeSpan)) + carve_size(nb * 4) + 1024;
     rc = grow_buffer(ctx, &ctx->smp_buf, &ctx->smp_bytes, bytes);
@@ -731,7 +772,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
     if (rc == CAC_OK) {
       prof_begin(ctx, "sample_trials", s);
-      hipLaunchKernelGGL(k_sample_trials, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
+      launch_sampling(ctx, nb, seed, d_scr, s);
       prof_end(ctx, s);
       prof_begin(ctx, "densify_trials", s);
       rc = densify_attr(ctx);
@@ -808,7 +849,7 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   if (rc) return rc;
   h = one[0];
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_sample_trials, dim3(1), dim3(kSmpBlock), 0, s, seed, (const TrialScratch*)d_scr);
+  launch_sampling(ctx, 1, seed, d_scr, s);
   CAC_LAUNCH_CHECK(ctx, "sample trial");
   uint32_t k = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&k, h.nv_out, 4, hipMemcpyDeviceToHost, s));
