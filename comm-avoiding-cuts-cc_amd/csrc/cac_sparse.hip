@@ -35,6 +35,7 @@ constexpr uint32_t kDomSparse = 0x53505253u;  // 'SPRS'
 constexpr int kSpBlock = 1024;
 constexpr int kSpWaves = kSpBlock / 64;
 constexpr uint32_t kSpSort = 1024;  // LDS sort size: pivot samples, final candidates
+constexpr int kSpRows = 4;          // a compaction tile is kSpRows x kSpBlock elements, row after row
 constexpr uint32_t kSparseDenseMax = 2048;
 constexpr unsigned long long kSpDisconnected = ~0ull;
 
@@ -77,6 +78,7 @@ struct SpChild {
   uint32_t* parc;             // [n] scratch: committed forest, then component ranks
   uint32_t* part;             // [n] scratch: trial forest (identity between probes), then labels
   unsigned long long* m_out;  // child edge count, kSpDisconnected when disconnected
+  unsigned long long* kmm;    // [2] smallest and largest race key, from k_sp_keys
 };
 
 // Root with path halving.  Links always point to smaller ids, so even a stale
@@ -146,23 +148,39 @@ __device__ inline uint64_t sp_exscan(uint64_t x, uint64_t* red, uint64_t* total)
   return before + inc - x;
 }
 
-// Position of this thread's item among the block's kept items of one
-// 1024-item tile, and the tile's kept count (order-preserving compaction).
-__device__ inline uint32_t sp_tile_rank(bool keep, uint32_t* wcnt, uint32_t* tile_total) {
+// Stable compaction ranks of a tile of kSpRows rows of kSpBlock elements (row j
+// of the thread is element j * kSpBlock + threadIdx.x of the tile): the
+// kSpRows x kSpWaves wave counts, in tile order, are one wave's exclusive scan,
+// so a tile of kSpRows rows costs three barriers instead of two per row.
+// wcnt holds kSpRows * kSpWaves + 1 words.
+static_assert(kSpRows * kSpWaves == 64, "one scan lane per (row, wave) count");
+__device__ inline void sp_tile_ranks(const bool (&keep)[kSpRows], uint32_t (&rank)[kSpRows], uint32_t* wcnt,
+                                     uint32_t* tile_total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t bal = __ballot(keep);
-  const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
-  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(bal);
-  __syncthreads();
-  uint32_t off = 0, tot = 0;
-  for (int w = 0; w < kSpWaves; ++w) {
-    const uint32_t c = wcnt[w];
-    off += w < wave ? c : 0;
-    tot += c;
+  const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j) {
+    const uint64_t bal = __ballot(keep[j]);
+    rank[j] = (uint32_t)__popcll(bal & below);
+    if (lane == 0) wcnt[j * kSpWaves + wave] = (uint32_t)__popcll(bal);
   }
   __syncthreads();
-  *tile_total = tot;
-  return off + below;
+  if (wave == 0) {
+    const uint32_t v = wcnt[lane];
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    wcnt[lane] = inc - v;
+    if (lane == 63) wcnt[64] = inc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j) rank[j] += wcnt[j * kSpWaves + wave];
+  *tile_total = wcnt[64];
+  __syncthreads();  // the next tile rewrites wcnt
 }
 
 // Bitonic sort of kSpSort (key, idx) pairs in LDS, ascending by key then idx.
@@ -198,22 +216,29 @@ template <class Keep>
 __device__ uint64_t sp_filter(const SpChild& c, const uint32_t* src, uint64_t ns, Keep keep, uint32_t* dst,
                               uint64_t* kmin, uint64_t* kmax, uint32_t* wcnt, uint64_t* red) {
   uint64_t base = 0, lo = ~0ull, hi = 0;
-  for (uint64_t t0 = 0; t0 < ns; t0 += kSpBlock) {
-    const uint64_t pos = t0 + threadIdx.x;
-    uint32_t e = 0;
-    bool k = false;
-    if (pos < ns) {
-      e = sp_at(src, pos);
-      const uint64_t key = c.keys[e];
-      k = keep(key);
-      if (k) {
-        lo = min(lo, key);
-        hi = max(hi, key);
+  for (uint64_t t0 = 0; t0 < ns; t0 += (uint64_t)kSpBlock * kSpRows) {
+    uint32_t e[kSpRows], r[kSpRows];
+    bool k[kSpRows];
+#pragma unroll
+    for (int j = 0; j < kSpRows; ++j) {
+      const uint64_t pos = t0 + (uint64_t)j * kSpBlock + threadIdx.x;
+      e[j] = 0;
+      k[j] = false;
+      if (pos < ns) {
+        e[j] = sp_at(src, pos);
+        const uint64_t key = c.keys[e[j]];
+        k[j] = keep(key);
+        if (k[j]) {
+          lo = min(lo, key);
+          hi = max(hi, key);
+        }
       }
     }
     uint32_t tot = 0;
-    const uint32_t r = sp_tile_rank(k, wcnt, &tot);
-    if (k) dst[base + r] = e;
+    sp_tile_ranks(k, r, wcnt, &tot);
+#pragma unroll
+    for (int j = 0; j < kSpRows; ++j)
+      if (k[j]) dst[base + r[j]] = e[j];
     base += tot;
   }
   *kmin = sp_min(lo, red);
@@ -263,30 +288,52 @@ struct KeepAbove {  // key > tau
   __device__ bool operator()(uint64_t k) const { return k > tau; }
 };
 
-__global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
-                                                              int64_t* __restrict__ results) {
-  __shared__ uint64_t red[kSpWaves];
-  __shared__ uint32_t wcnt[kSpWaves];
-  __shared__ uint64_t skey[kSpSort];
-  __shared__ uint32_t sidx[kSpSort];
-  const SpChild c = children[blockIdx.x];
-  const uint32_t tid = threadIdx.x;
+__global__ void k_sp_kmm_init(const SpChild* __restrict__ children, uint32_t count) {
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
+    __hip_atomic_store(&children[i].kmm[0], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&children[i].kmm[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
-  // 1. race keys, identity forests
+// Race keys of every child's parent edges, spread over the whole device (grid
+// y = child), with each child's smallest and largest key.
+__global__ void __launch_bounds__(256) k_sp_keys(const SpChild* __restrict__ children, uint32_t seed) {
+  const SpChild c = children[blockIdx.y];
   uint64_t kmin = ~0ull, kmax = 0;
-  for (uint64_t e = tid; e < c.m; e += kSpBlock) {
+  for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < c.m; e += (uint64_t)gridDim.x * 256) {
     const u32x4 r = philox4x32(c.trial, c.nid, (uint32_t)e, (uint32_t)(e >> 32), seed, kDomSparse);
     const uint64_t k = race_key(lo64(r), c.edges[e].w);
     c.keys[e] = k;
     kmin = min(kmin, k);
     kmax = max(kmax, k);
   }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    kmin = min(kmin, (uint64_t)__shfl_xor(kmin, d, 64));
+    kmax = max(kmax, (uint64_t)__shfl_xor(kmax, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && kmin <= kmax) {
+    atomicMin(&c.kmm[0], (unsigned long long)kmin);
+    atomicMax(&c.kmm[1], (unsigned long long)kmax);
+  }
+}
+
+__global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
+                                                              int64_t* __restrict__ results) {
+  __shared__ uint64_t red[kSpWaves];
+  __shared__ uint32_t wcnt[kSpRows * kSpWaves + 1];
+  __shared__ uint64_t skey[kSpSort];
+  __shared__ uint32_t sidx[kSpSort];
+  const SpChild c = children[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+
+  // 1. identity forests; the race keys and their range come from k_sp_keys
   for (uint32_t v = tid; v < c.n; v += kSpBlock) {
     c.parc[v] = v;
     c.part[v] = v;
   }
-  kmin = sp_min(kmin, red);
-  kmax = sp_max(kmax, red);
+  uint64_t kmin = __hip_atomic_load(&c.kmm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t kmax = __hip_atomic_load(&c.kmm[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   block_sync_global();
 
   if (c.n > c.tgt) {  // else nothing is contracted
@@ -377,23 +424,51 @@ __global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* 
 
   // 5. the child: edges in parent order, relabelled, loops dropped
   uint64_t base = 0;
-  for (uint64_t t0 = 0; t0 < c.m; t0 += kSpBlock) {
-    const uint64_t e = t0 + tid;
-    cac_edge x{0, 0, 0};
-    bool keep = false;
-    if (e < c.m) {
-      x = c.edges[e];
-      const uint32_t a = c.part[x.u], b = c.part[x.v];
-      keep = a != b;
-      x.u = min(a, b);
-      x.v = max(a, b);
+  for (uint64_t t0 = 0; t0 < c.m; t0 += (uint64_t)kSpBlock * kSpRows) {
+    uint32_t a[kSpRows], b[kSpRows], r[kSpRows];
+    bool keep[kSpRows];
+#pragma unroll
+    for (int j = 0; j < kSpRows; ++j) {
+      const uint64_t e = t0 + (uint64_t)j * kSpBlock + tid;
+      a[j] = b[j] = 0;
+      if (e < c.m) {
+        const cac_edge x = c.edges[e];
+        a[j] = x.u;
+        b[j] = x.v;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kSpRows; ++j) {
+      const uint64_t e = t0 + (uint64_t)j * kSpBlock + tid;
+      keep[j] = false;
+      if (e < c.m) {
+        const uint32_t pa = c.part[a[j]], pb = c.part[b[j]];
+        keep[j] = pa != pb;
+        a[j] = min(pa, pb);
+        b[j] = max(pa, pb);
+      }
     }
     uint32_t tot = 0;
-    const uint32_t r = sp_tile_rank(keep, wcnt, &tot);
-    if (keep) c.out[base + r] = x;
+    sp_tile_ranks(keep, r, wcnt, &tot);
+#pragma unroll
+    for (int j = 0; j < kSpRows; ++j)
+      if (keep[j]) c.out[base + r[j]] = cac_edge{a[j], b[j], c.edges[t0 + (uint64_t)j * kSpBlock + tid].w};
     base += tot;
   }
   if (tid == 0) *c.m_out = base;
+}
+
+// The contraction of nchild children (device descriptors d_children, largest
+// parent edge count mmax): keys over the whole device, then one workgroup per
+// child.
+void launch_sparse_contract(const SpChild* d_children, uint32_t nchild, uint64_t mmax, uint32_t seed,
+                            int64_t* d_results, hipStream_t s) {
+  const uint64_t per_child = std::max<uint64_t>(1, 8192 / std::max<uint32_t>(nchild, 1));
+  const unsigned gx = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (mmax + 4095) / 4096), per_child);
+  hipLaunchKernelGGL(k_sp_kmm_init, dim3(grid_for(nchild, 256, 64)), dim3(256), 0, s, d_children, nchild);
+  for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)  // grid y holds at most 65535 children
+    hipLaunchKernelGGL(k_sp_keys, dim3(gx, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0, seed);
+  hipLaunchKernelGGL(k_sparse_contract, dim3(nchild), dim3(kSpBlock), 0, s, d_children, seed, d_results);
 }
 
 // Handover to the dense engine: n x n matrix and row sums from an edge list.
@@ -501,7 +576,7 @@ struct SpNode {
 
 // Device bytes of one child: what stays alive for the next depth (its edge
 // list, descriptor, count) and the scratch it needs only while it contracts.
-size_t child_keep_bytes(const SpNode& nd) { return carve_size(nd.m * sizeof(cac_edge)) + sizeof(SpChild) + 8; }
+size_t child_keep_bytes(const SpNode& nd) { return carve_size(nd.m * sizeof(cac_edge)) + sizeof(SpChild) + 24; }
 size_t child_tmp_bytes(const SpNode& nd) {
   return carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) + 2 * carve_size((size_t)nd.n * 4);
 }
@@ -606,7 +681,9 @@ struct SpRun {
       Carver ct(ctx->sp_tmp);
       SpChild* d_children = cv.take<SpChild>(nchild);
       unsigned long long* d_mout = cv.take<unsigned long long>(nchild);
+      unsigned long long* d_kmm = cv.take<unsigned long long>(2 * nchild);
       std::vector<SpChild> ch(nchild);
+      uint64_t mmax = 0;
       for (size_t i = 0; i < count; ++i) {
         const SpNode& nd = sp[i0 + i];
         for (uint32_t k = 0; k < 2; ++k) {
@@ -626,12 +703,13 @@ struct SpRun {
           x.parc = ct.take<uint32_t>(nd.n);
           x.part = ct.take<uint32_t>(nd.n);
           x.m_out = d_mout + 2 * i + k;
+          x.kmm = d_kmm + 2 * (2 * i + k);
         }
+        mmax = std::max<uint64_t>(mmax, nd.m);
       }
       CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(SpChild), hipMemcpyHostToDevice, s));
       prof_begin(ctx, "sparse_contract", s);
-      hipLaunchKernelGGL(k_sparse_contract, dim3((unsigned)nchild), dim3(kSpBlock), 0, s, (const SpChild*)d_children,
-                         seed, d_results);
+      launch_sparse_contract(d_children, (uint32_t)nchild, mmax, seed, d_results, s);
       prof_end(ctx, s);
       CAC_LAUNCH_CHECK(ctx, "sparse contract");
       std::vector<unsigned long long> mout(nchild);
@@ -774,7 +852,7 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   const size_t mm = m ? m : 1;
   const size_t bytes = cac::carve_size(sizeof(cac::SpChild)) + cac::carve_size(mm * sizeof(cac_edge)) * 2 +
                        cac::carve_size(mm * 8) + 2 * cac::carve_size(mm * 4) + 2 * cac::carve_size((size_t)n * 4) +
-                       cac::carve_size(16) + 256;
+                       cac::carve_size(16) + cac::carve_size(16) + 256;
   void* base = nullptr;
   rc = cac::scratch_reserve(ctx, bytes, &base);
   if (rc) return rc;
@@ -797,10 +875,10 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   unsigned long long* d_mout = cv.take<unsigned long long>(1);
   int64_t* d_res = cv.take<int64_t>(1);
   c.m_out = d_mout;
+  c.kmm = cv.take<unsigned long long>(2);
   if (m) CAC_HIP(ctx, hipMemcpyAsync(d_in, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice, s));
   CAC_HIP(ctx, hipMemcpyAsync(d_child, &c, sizeof(c), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(cac::k_sparse_contract, dim3(1), dim3(cac::kSpBlock), 0, s, (const cac::SpChild*)d_child, seed,
-                     d_res);
+  cac::launch_sparse_contract(d_child, 1, m, seed, d_res, s);
   CAC_LAUNCH_CHECK(ctx, "sparse contract");
   unsigned long long mo = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&mo, d_mout, 8, hipMemcpyDeviceToHost, s));
