@@ -36,6 +36,7 @@ constexpr int kSpBlock = 1024;
 constexpr int kSpWaves = kSpBlock / 64;
 constexpr uint32_t kSpSort = 1024;  // LDS sort size: pivot samples, final candidates
 constexpr int kSpRows = 4;          // a compaction tile is kSpRows x kSpBlock elements, row after row
+constexpr uint64_t kSpTile = (uint64_t)kSpRows * kSpBlock;  // edges per output tile
 constexpr uint32_t kSparseDenseMax = 2048;
 constexpr unsigned long long kSpDisconnected = ~0ull;
 
@@ -79,6 +80,7 @@ struct SpChild {
   uint32_t* part;             // [n] scratch: trial forest (identity between probes), then labels
   unsigned long long* m_out;  // child edge count, kSpDisconnected when disconnected
   unsigned long long* kmm;    // [2] smallest and largest race key, from k_sp_keys
+  uint32_t* tiles;            // [ceil(m / kSpTile)] scratch: kept edges per output tile, then their offsets
 };
 
 // Root with path halving.  Links always point to smaller ids, so even a stale
@@ -292,6 +294,7 @@ __global__ void k_sp_kmm_init(const SpChild* __restrict__ children, uint32_t cou
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) {
     __hip_atomic_store(&children[i].kmm[0], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&children[i].kmm[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(children[i].m_out, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -422,40 +425,84 @@ __global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* 
   for (uint32_t v = tid; v < c.n; v += kSpBlock) c.part[v] = c.parc[c.part[v]];
   block_sync_global();
 
-  // 5. the child: edges in parent order, relabelled, loops dropped
-  uint64_t base = 0;
-  for (uint64_t t0 = 0; t0 < c.m; t0 += (uint64_t)kSpBlock * kSpRows) {
-    uint32_t a[kSpRows], b[kSpRows], r[kSpRows];
-    bool keep[kSpRows];
-#pragma unroll
-    for (int j = 0; j < kSpRows; ++j) {
-      const uint64_t e = t0 + (uint64_t)j * kSpBlock + tid;
-      a[j] = b[j] = 0;
-      if (e < c.m) {
-        const cac_edge x = c.edges[e];
-        a[j] = x.u;
-        b[j] = x.v;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kSpRows; ++j) {
-      const uint64_t e = t0 + (uint64_t)j * kSpBlock + tid;
-      keep[j] = false;
-      if (e < c.m) {
-        const uint32_t pa = c.part[a[j]], pb = c.part[b[j]];
-        keep[j] = pa != pb;
-        a[j] = min(pa, pb);
-        b[j] = max(pa, pb);
-      }
-    }
-    uint32_t tot = 0;
-    sp_tile_ranks(keep, r, wcnt, &tot);
-#pragma unroll
-    for (int j = 0; j < kSpRows; ++j)
-      if (keep[j]) c.out[base + r[j]] = cac_edge{a[j], b[j], c.edges[t0 + (uint64_t)j * kSpBlock + tid].w};
-    base += tot;
+  // 5. the child's edges are written by k_sp_tile_count / k_sp_tile_scan /
+  //    k_sp_tile_write over the whole device
+}
+
+// 5. The child: the parent's edges in parent order, relabelled by the
+// component ranks in part, loops dropped; in three launches over the whole
+// device (tile counts, their offsets, the stable write).  A disconnected child
+// (m_out already kSpDisconnected) is skipped.
+__device__ inline bool sp_disconnected(const SpChild& c) {
+  return __hip_atomic_load(c.m_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == kSpDisconnected;
+}
+
+// Grid (tile, child), 256 threads: kept edges of each tile.
+__global__ void __launch_bounds__(256) k_sp_tile_count(const SpChild* __restrict__ children) {
+  __shared__ uint32_t wsum[4];
+  const SpChild c = children[blockIdx.y];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSpTile;
+  if (t0 >= c.m || sp_disconnected(c)) return;
+  const uint64_t t1 = min(c.m, t0 + kSpTile);
+  uint32_t cnt = 0;
+  for (uint64_t e = t0 + threadIdx.x; e < t1; e += 256) {
+    const cac_edge x = c.edges[e];
+    cnt += c.part[x.u] != c.part[x.v] ? 1u : 0u;
   }
-  if (tid == 0) *c.m_out = base;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) c.tiles[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// One workgroup per child: tile counts to exclusive offsets, and the child's
+// edge count.
+__global__ void __launch_bounds__(kSpBlock) k_sp_tile_scan(const SpChild* __restrict__ children) {
+  __shared__ uint64_t red[kSpWaves];
+  const SpChild c = children[blockIdx.x];
+  if (sp_disconnected(c)) return;
+  const uint64_t nt = (c.m + kSpTile - 1) / kSpTile;
+  uint64_t carry = 0;
+  for (uint64_t t0 = 0; t0 < nt; t0 += kSpBlock) {
+    const uint64_t t = t0 + threadIdx.x;
+    const uint64_t v = t < nt ? c.tiles[t] : 0;
+    uint64_t total = 0;
+    const uint64_t ex = sp_exscan(v, red, &total);
+    if (t < nt) c.tiles[t] = (uint32_t)(carry + ex);
+    carry += total;
+  }
+  if (threadIdx.x == 0) *c.m_out = carry;
+}
+
+// Grid (tile, child), kSpBlock threads: each tile's kept edges at its offset,
+// in order.
+__global__ void __launch_bounds__(kSpBlock) k_sp_tile_write(const SpChild* __restrict__ children) {
+  __shared__ uint32_t wcnt[kSpRows * kSpWaves + 1];
+  const SpChild c = children[blockIdx.y];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSpTile;
+  if (t0 >= c.m || sp_disconnected(c)) return;
+  const uint64_t base = c.tiles[blockIdx.x];
+  uint32_t a[kSpRows], b[kSpRows], r[kSpRows];
+  bool keep[kSpRows];
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j) {
+    const uint64_t e = t0 + (uint64_t)j * kSpBlock + threadIdx.x;
+    keep[j] = false;
+    a[j] = b[j] = 0;
+    if (e < c.m) {
+      const cac_edge x = c.edges[e];
+      const uint32_t pa = c.part[x.u], pb = c.part[x.v];
+      keep[j] = pa != pb;
+      a[j] = min(pa, pb);
+      b[j] = max(pa, pb);
+    }
+  }
+  uint32_t tot = 0;
+  sp_tile_ranks(keep, r, wcnt, &tot);
+#pragma unroll
+  for (int j = 0; j < kSpRows; ++j)
+    if (keep[j]) c.out[base + r[j]] = cac_edge{a[j], b[j], c.edges[t0 + (uint64_t)j * kSpBlock + threadIdx.x].w};
 }
 
 // The contraction of nchild children (device descriptors d_children, largest
@@ -469,6 +516,13 @@ void launch_sparse_contract(const SpChild* d_children, uint32_t nchild, uint64_t
   for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)  // grid y holds at most 65535 children
     hipLaunchKernelGGL(k_sp_keys, dim3(gx, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0, seed);
   hipLaunchKernelGGL(k_sparse_contract, dim3(nchild), dim3(kSpBlock), 0, s, d_children, seed, d_results);
+  const unsigned tiles = (unsigned)std::max<uint64_t>(1, (mmax + kSpTile - 1) / kSpTile);
+  for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)
+    hipLaunchKernelGGL(k_sp_tile_count, dim3(tiles, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0);
+  hipLaunchKernelGGL(k_sp_tile_scan, dim3(nchild), dim3(kSpBlock), 0, s, d_children);
+  for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)
+    hipLaunchKernelGGL(k_sp_tile_write, dim3(tiles, std::min(nchild - c0, 65535u)), dim3(kSpBlock), 0, s,
+                       d_children + c0);
 }
 
 // Handover to the dense engine: n x n matrix and row sums from an edge list.
@@ -578,7 +632,8 @@ struct SpNode {
 // list, descriptor, count) and the scratch it needs only while it contracts.
 size_t child_keep_bytes(const SpNode& nd) { return carve_size(nd.m * sizeof(cac_edge)) + sizeof(SpChild) + 24; }
 size_t child_tmp_bytes(const SpNode& nd) {
-  return carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) + 2 * carve_size((size_t)nd.n * 4);
+  return carve_size(nd.m * 8) + 2 * carve_size(nd.m * 4) + 2 * carve_size((size_t)nd.n * 4) +
+         carve_size((nd.m + kSpTile - 1) / kSpTile * 4);
 }
 size_t dense_bytes(const SpNode& nd) {
   return carve_size((size_t)nd.n * nd.n * 8) + carve_size((size_t)nd.n * 8) + sizeof(SpDense);
@@ -702,6 +757,7 @@ struct SpRun {
           x.list[1] = ct.take<uint32_t>(nd.m);
           x.parc = ct.take<uint32_t>(nd.n);
           x.part = ct.take<uint32_t>(nd.n);
+          x.tiles = ct.take<uint32_t>((nd.m + kSpTile - 1) / kSpTile);
           x.m_out = d_mout + 2 * i + k;
           x.kmm = d_kmm + 2 * (2 * i + k);
         }
@@ -852,7 +908,8 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   const size_t mm = m ? m : 1;
   const size_t bytes = cac::carve_size(sizeof(cac::SpChild)) + cac::carve_size(mm * sizeof(cac_edge)) * 2 +
                        cac::carve_size(mm * 8) + 2 * cac::carve_size(mm * 4) + 2 * cac::carve_size((size_t)n * 4) +
-                       cac::carve_size(16) + cac::carve_size(16) + 256;
+                       cac::carve_size(16) + cac::carve_size(16) + cac::carve_size((mm + cac::kSpTile - 1) / cac::kSpTile * 4) +
+                       256;
   void* base = nullptr;
   rc = cac::scratch_reserve(ctx, bytes, &base);
   if (rc) return rc;
@@ -876,6 +933,7 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   int64_t* d_res = cv.take<int64_t>(1);
   c.m_out = d_mout;
   c.kmm = cv.take<unsigned long long>(2);
+  c.tiles = cv.take<uint32_t>((mm + cac::kSpTile - 1) / cac::kSpTile);
   if (m) CAC_HIP(ctx, hipMemcpyAsync(d_in, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice, s));
   CAC_HIP(ctx, hipMemcpyAsync(d_child, &c, sizeof(c), hipMemcpyHostToDevice, s));
   cac::launch_sparse_contract(d_child, 1, m, seed, d_res, s);
