@@ -91,22 +91,24 @@ __device__ inline uint32_t uf_find_from(uint32_t* par, uint32_t x, uint32_t p /*
 __device__ inline uint32_t uf_find(uint32_t* par, uint32_t x) { return uf_find_from(par, x, ld_par(par, x)); }
 
 // Union of the sets of a and b (pa, pb: their already-loaded parents): CAS the
-// larger root to point at the smaller.
-__device__ inline void uf_link_from(uint32_t* par, uint32_t a, uint32_t pa, uint32_t b, uint32_t pb) {
+// larger root to point at the smaller.  Returns 1 when this call hooked a root
+// (two trees became one), 0 when a and b were already in one tree.
+__device__ inline uint32_t uf_link_from(uint32_t* par, uint32_t a, uint32_t pa, uint32_t b, uint32_t pb) {
   uint32_t ra = uf_find_from(par, a, pa), rb = uf_find_from(par, b, pb);
   while (ra != rb) {
     uint32_t lo = ra < rb ? ra : rb;
     uint32_t hi = ra < rb ? rb : ra;
     uint32_t old = atomicCAS(par + hi, hi, lo);
-    if (old == hi) return;
+    if (old == hi) return 1u;
     // hi was hooked elsewhere meanwhile: climb from its new parent and retry.
     ra = uf_find(par, old);
     rb = uf_find(par, lo);
   }
+  return 0u;
 }
 
-__device__ inline void uf_link(uint32_t* par, uint32_t a, uint32_t b) {
-  uf_link_from(par, a, ld_par(par, a), b, ld_par(par, b));
+__device__ inline uint32_t uf_link(uint32_t* par, uint32_t a, uint32_t b) {
+  return uf_link_from(par, a, ld_par(par, a), b, ld_par(par, b));
 }
 
 __global__ void __launch_bounds__(kCcBlock) k_cc_init(uint32_t* __restrict__ par, uint32_t n) {
@@ -323,62 +325,255 @@ __global__ void __launch_bounds__(kCcBlock) k_check_ids(const cac_uedge* e, uint
 // both adjacency lists, so an edge from a vertex of the big component to one
 // outside it is still linked from the outside end.
 constexpr uint32_t kAfRounds = 2;
-constexpr uint32_t kAfHeavy = 64;  // a vertex with more remaining neighbours goes to a workgroup
+constexpr uint32_t kAfHeavy = 1024;  // a vertex with more remaining neighbours goes to a workgroup
+// Successful hooks are counted per wavefront and added to one of kHookSlots
+// counters 64 bytes apart, so no single word takes an atomic from every wave.  Components = n - hooks: each hook
+// joins two trees into one.
+constexpr uint32_t kHookSlots = 64, kHookStride = 16;
+// CSR scratch head: [0] components, [1] sampled root, [2] heavy-queue length,
+// then the hook slots.
+constexpr size_t kAfHead = 64 + (size_t)kHookSlots * kHookStride;
+
+// Per wavefront, without a barrier: a wave with hooks adds them to the slot of
+// its global wave index.  (A block barrier here would hold every wave of a
+// block until its slowest one, in the divergent final pass, had finished.)
+__device__ inline void add_hooks(uint32_t* slots, uint32_t h) {
+  for (int d = 32; d > 0; d >>= 1) h += __shfl_down(h, d, 64);
+  const uint32_t wave = blockIdx.x * (kCcBlock / 64) + (threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0 && h) atomicAdd(slots + (wave % kHookSlots) * kHookStride, h);
+}
+
+// parent[v] = v with 16-byte stores where the array is aligned, and the scratch
+// head (counters and hook slots) cleared.
+__global__ void __launch_bounds__(kCcBlock) k_af_init(uint32_t* __restrict__ par, uint32_t n,
+                                                      uint32_t* __restrict__ head, uint32_t head_words) {
+  const uint64_t T = (uint64_t)gridDim.x * kCcBlock, tid = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x;
+  if (tid < head_words) head[tid] = 0;
+  uint64_t tail = 0;
+  if ((reinterpret_cast<uintptr_t>(par) & 15) == 0) {
+    uint4* p4 = reinterpret_cast<uint4*>(par);
+    for (uint64_t q = tid; q < n / 4; q += T) {
+      const uint32_t v = (uint32_t)(q * 4);
+#ifdef CAC_AF_NTINIT
+      typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+      const v4u x = {v, v + 1, v + 2, v + 3};
+      __builtin_nontemporal_store(x, reinterpret_cast<v4u*>(p4 + q));
+#else
+      p4[q] = make_uint4(v, v + 1, v + 2, v + 3);
+#endif
+    }
+    tail = (uint64_t)(n / 4) * 4;
+  }
+  for (uint64_t i = tail + tid; i < n; i += T) par[i] = (uint32_t)i;
+}
+
+// Root of the largest component, guessed from kAfSamples vertices of [0, range)
+// by a Boyer-Moore majority vote: per lane over its samples, then across the
+// wavefront (pairwise merge; ties keep the smaller id, so both partners of a
+// butterfly step agree).  Only a performance choice: labels are correct
+// whatever root is returned.  One wavefront; all sample loads are issued first.
+constexpr int kAfSamples = 1024;
+__device__ inline void af_majority_root(uint32_t* par, uint32_t range, uint32_t* out) {
+  constexpr int kPer = kAfSamples / 64;
+  const int lane = threadIdx.x & 63;
+  uint32_t x[kPer], p[kPer];
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    uint32_t h = (uint32_t)(t * 64 + lane) * 0x9E3779B1u;  // scatter the sample positions
+    h ^= h >> 15;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    x[t] = (uint32_t)(((uint64_t)h * range) >> 32);
+    p[t] = ld_par_fresh(par, x[t]);
+  }
+  uint32_t cand = ~0u, cnt = 0;
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const uint32_t r = p[t] == x[t] ? x[t] : uf_find_fresh(par, p[t]);
+    if (cnt == 0) {
+      cand = r;
+      cnt = 1;
+    } else {
+      cnt = cand == r ? cnt + 1 : cnt - 1;
+    }
+  }
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t oc = __shfl_xor(cand, d, 64), on = __shfl_xor(cnt, d, 64);
+    if (oc == cand) {
+      cnt += on;
+    } else if (on > cnt || (on == cnt && oc < cand)) {
+      cand = oc;
+      cnt = on - cnt;
+    } else {
+      cnt -= on;
+    }
+  }
+  if (lane == 0) out[0] = cand;
+}
 
 // Vertices [v_lo, v_hi) link with their first kAfRounds neighbours (lists are
 // sorted by id, so these tend to be the low-id hubs of skewed graphs).  Both
 // rounds share one pass: with ~128 bytes of adjacency per vertex, reading a
 // first neighbour already touches nearly every line of the column array.
-// kBulk: the last range (every vertex above the hub prefixes), instantiated
-// separately so traces report it apart from the small hub launches.
+// kBulk: the last range (every vertex above the hub prefixes [0, v_lo), which
+// are linked and pointer-jumped by now).  Its first wavefront also guesses the
+// root of the largest component from the hub forest (af_majority_root), which
+// saves a launch between this pass and k_af_final.
 template <bool kBulk>
 __global__ void __launch_bounds__(kCcBlock) k_af_round(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cols, uint32_t v_lo, uint32_t v_hi,
-                                                       uint32_t* par) {
+                                                       uint32_t* par, uint32_t* hook_slots, uint32_t* root_out) {
   static_assert(kAfRounds == 2, "k_af_round links two neighbours");
+  if (kBulk && blockIdx.x == 0 && threadIdx.x < 64) af_majority_root(par, v_lo, root_out);
+  uint32_t hooks = 0;
   for (uint64_t i = v_lo + (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < v_hi; i += (uint64_t)gridDim.x * kCcBlock) {
     const uint32_t u = (uint32_t)i, b = off[u], e = off[u + 1];
-    const uint32_t v0 = b < e ? cols[b] : u, v1 = b + 1 < e ? cols[b + 1] : u;
-    if (v0 != u) uf_link(par, u, v0);
-    if (v1 != u) uf_link(par, u, v1);
+    // both first neighbours with one 8-byte load (4-byte aligned is enough)
+    uint32_t v0 = u, v1 = u;
+    if (b + 1 < e) {
+      uint2 x;
+      __builtin_memcpy(&x, cols + b, 8);
+      v0 = x.x;
+      v1 = x.y;
+    } else if (b < e) {
+      v0 = cols[b];
+    }
+    // u is taken to be a root without loading par[u]: the CAS checks it, and a
+    // failed CAS climbs from the value it returns
+    if (v0 != u) hooks += uf_link_from(par, u, u, v0, ld_par(par, v0));
+    if (v1 != u) hooks += uf_link(par, u, v1);
   }
+  add_hooks(hook_slots, hooks);
 }
 
 // Every vertex is re-pointed at its root (the pointer jumping of jump_pass,
 // fused here); those whose root is not the sampled root R then link their
-// neighbours from index kAfRounds on, and long lists are queued for
+// neighbours from index kAfRounds on.  A wavefront links the remaining
+// neighbours of all its vertices together: an exclusive scan of the per-lane
+// counts lays them out as one list, and lane j of each 64-entry step links
+// entry j with its owner (found by a binary search over the scan), so one long
+// list does not leave 63 lanes idle.  Lists longer than kAfHeavy are queued for
 // k_af_heavy.  A root found here may be hooked under a smaller one by a
 // concurrent link: it stays an ancestor, so the store is still valid, and the
-// vertex at worst links neighbours that were already joined.
+// vertex at worst links neighbours that were already joined.  Such a vertex
+// may then hold a stale root, so bit v of `outside` marks every vertex with
+// neighbours whose root was not R; k_af_fix re-points those.  Offsets are read
+// only for vertices outside R's tree.
+constexpr int kAfChunks = 4;  // 64-vertex chunks a wavefront of k_af_final has in flight
 __global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cols, uint32_t n, uint32_t* par,
                                                        const uint32_t* __restrict__ root, uint32_t* heavy,
-                                                       uint32_t* heavy_count) {
+                                                       uint32_t* heavy_count, uint64_t* __restrict__ outside,
+                                                       uint32_t* hook_slots) {
+  constexpr int K = kAfChunks;
   const uint32_t R = root[0];
-  for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
-    const uint32_t u = (uint32_t)i;
-    const uint32_t p = ld_par_fresh(par, u), b = off[u] + kAfRounds, e = off[u + 1];
-    const uint32_t r = p == u ? u : uf_find_fresh(par, p);
-    if (r != p) st_par(par, u, r);
-    if (b >= e || r == R) continue;
-    if (e - b > kAfHeavy) {
-      heavy[atomicAdd(heavy_count, 1u)] = u;
-      continue;
+  const int lane = threadIdx.x & 63;
+  uint32_t hooks = 0;
+  // Whole wavefronts step together (the ballot and the shuffles see all 64
+  // lanes); each step takes K chunks and issues every load of a phase for all of
+  // them before using any: the parents, the parents' parents, the offsets.
+  const uint64_t wave = (uint64_t)blockIdx.x * (kCcBlock / 64) + (threadIdx.x >> 6);
+  for (uint64_t w0 = wave * 64 * K; w0 < n; w0 += (uint64_t)gridDim.x * (kCcBlock / 64) * 64 * K) {
+    uint32_t p[K], g[K], b[K], e[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t i = w0 + (uint64_t)k * 64 + lane;
+      p[k] = i < n ? ld_par_fresh(par, (uint32_t)i) : (uint32_t)i;
     }
-    for (uint32_t k = b; k < e; ++k) uf_link(par, u, cols[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t u = (uint32_t)(w0 + (uint64_t)k * 64 + lane);
+      g[k] = p[k] != u ? ld_par_fresh(par, p[k]) : p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t i = w0 + (uint64_t)k * 64 + lane;
+      const uint32_t u = (uint32_t)i;
+      const uint32_t r = g[k] == p[k] ? p[k] : uf_find_fresh(par, g[k]);
+      if (r != p[k]) st_par(par, u, r);
+      const bool out = i < n && r != R;
+      b[k] = out ? off[u] : 0u;
+      e[k] = out ? off[u + 1] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint64_t c0 = w0 + (uint64_t)k * 64;
+      const uint64_t mask = __ballot(e[k] > b[k]);
+      if (c0 < n && lane == 0) outside[c0 >> 6] = mask;
+      if (!mask) continue;
+      const uint32_t u = (uint32_t)(c0 + lane), bb = b[k] + kAfRounds;
+      uint32_t c = bb < e[k] ? e[k] - bb : 0u;
+      if (c > kAfHeavy) {
+        heavy[atomicAdd(heavy_count, 1u)] = u;
+        c = 0;
+      }
+      uint32_t incl = c;
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+      }
+      const uint32_t total = __shfl(incl, 63, 64);
+      for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        uint32_t L = 0;  // owner lane: the number of lanes whose inclusive count is <= j
+        for (int st = 32; st > 0; st >>= 1)
+          if (__shfl(incl, (int)L + st - 1, 64) <= j) L += st;
+        const uint32_t ou = __shfl(u, (int)L, 64), ob = __shfl(bb, (int)L, 64);
+        const uint32_t ofirst = __shfl(incl - c, (int)L, 64);
+        if (j < total) hooks += uf_link(par, ou, cols[ob + (j - ofirst)]);
+      }
+    }
   }
+  add_hooks(hook_slots, hooks);
 }
 
 // One workgroup per queued vertex (grid-stride over the queue).
 __global__ void __launch_bounds__(kCcBlock) k_af_heavy(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cols, uint32_t* par,
                                                        const uint32_t* __restrict__ heavy,
-                                                       const uint32_t* __restrict__ heavy_count) {
+                                                       const uint32_t* __restrict__ heavy_count, uint32_t* hook_slots) {
   const uint32_t count = heavy_count[0];
+  uint32_t hooks = 0;
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t u = heavy[q];
     const uint32_t b = off[u] + kAfRounds, e = off[u + 1];
-    for (uint32_t k = b + threadIdx.x; k < e; k += kCcBlock) uf_link(par, u, cols[k]);
+    for (uint32_t k = b + threadIdx.x; k < e; k += kCcBlock) hooks += uf_link(par, u, cols[k]);
+  }
+  add_hooks(hook_slots, hooks);
+}
+
+// Final labels.  A vertex whose root was R when k_af_final re-pointed it still
+// has the right label unless R itself was hooked later; a vertex without
+// neighbours never changes.  So only the vertices marked in `outside` are
+// re-pointed at their roots (one bitmap word per lane, its set bits in turn),
+// unless R is no longer a root, in which case every vertex is (one vertex per
+// lane).  Block 0 also writes the component count, n - hooks.
+__global__ void __launch_bounds__(kCcBlock) k_af_fix(uint32_t* par, uint32_t n, const uint64_t* __restrict__ outside,
+                                                     const uint32_t* __restrict__ root, uint32_t* counters,
+                                                     const uint32_t* hook_slots) {
+  const uint32_t R = root[0];
+  const uint64_t T = (uint64_t)gridDim.x * kCcBlock, tid = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x;
+  if (ld_par_fresh(par, R) != R) {
+    for (uint64_t i = tid; i < n; i += T) {
+      const uint32_t u = (uint32_t)i, p = ld_par_fresh(par, u);
+      if (p == u) continue;
+      const uint32_t r = uf_find_fresh(par, p);
+      if (r != p) st_par(par, u, r);
+    }
+  } else {
+    for (uint64_t w = tid; w < ((uint64_t)n + 63) / 64; w += T) {
+      for (uint64_t bits = outside[w]; bits; bits &= bits - 1) {
+        const uint32_t u = (uint32_t)(w * 64 + __builtin_ctzll(bits)), p = ld_par_fresh(par, u);
+        if (p == u) continue;
+        const uint32_t r = uf_find_fresh(par, p);
+        if (r != p) st_par(par, u, r);
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    uint32_t h = ld_par_fresh(hook_slots, threadIdx.x * kHookStride);
+    for (int d = 32; d > 0; d >>= 1) h += __shfl_down(h, d, 64);
+    if (threadIdx.x == 0) counters[0] = n - h;
   }
 }
 
@@ -516,32 +711,47 @@ int cc_launch(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, in
   return CAC_OK;
 }
 
-// Afforest on a symmetric CSR; same scratch layout as cc_launch (head, then the
-// rank tables), plus n + 64 words for the queue of long adjacency lists.
+// Queue of long adjacency lists: n + 64 words, rounded to 64 so what follows
+// stays 256-byte aligned.
+static size_t af_queue_words(uint32_t n) { return ((size_t)n + 64 + 63) & ~size_t(63); }
+
+// Afforest on a symmetric CSR.  Scratch: the head (counters and hook slots), the
+// queue, the `outside` bitmap (one bit per vertex, whole u64 words), then the
+// rank tables.
 size_t cc_csr_tmp_words(uint32_t n, int label_mode) {
-  return kCcTmpHead + ((size_t)n + 64) + (label_mode == CAC_LABEL_RANK ? cc_rank_tmp_words(n) : 0);
+  return kAfHead + af_queue_words(n) + 2 * (((size_t)n + 63) / 64) +
+         (label_mode == CAC_LABEL_RANK ? cc_rank_tmp_words(n) : 0);
 }
 
 int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, uint32_t n, int label_mode,
                   uint32_t* d_lab, uint32_t* d_tmp, hipStream_t s) {
-  uint32_t* d_counter = d_tmp;        // [0] components, [1] sampled root, [2] queue length
-  uint32_t* d_heavy = d_tmp + kCcTmpHead;
-  uint32_t* d_rank_tmp = d_heavy + ((size_t)n + 64);
+  uint32_t* d_counter = d_tmp;  // [0] components, [1] sampled root, [2] queue length
+  uint32_t* d_hooks = d_tmp + 64;
+  uint32_t* d_heavy = d_tmp + kAfHead;
+  uint64_t* d_outside = reinterpret_cast<uint64_t*>(d_heavy + af_queue_words(n));
+  uint32_t* d_rank_tmp = reinterpret_cast<uint32_t*>(d_outside + ((size_t)n + 63) / 64);
   const unsigned cap = 256 * 64;
   prof_begin(ctx, "af_init", s);
-  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, n);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counter, 3);
+  {
+    unsigned g = grid_for(((uint64_t)n + 3) / 4, kCcBlock, cap);
+    const unsigned g_head = (unsigned)((kAfHead + kCcBlock - 1) / kCcBlock);
+    hipLaunchKernelGGL(k_af_init, dim3(g > g_head ? g : g_head), dim3(kCcBlock), 0, s, d_lab, n, d_counter,
+                       (uint32_t)kAfHead);
+  }
   prof_end(ctx, s);
   // Low ids first, each prefix pointer-jumped before the next range links to it:
   // the first neighbours of most vertices are low-id hubs, whose chains are then
   // one hop long (0.99 ms -> 0.58 ms for the round on R-MAT 2^24 / 2^28).
   prof_begin(ctx, "af_hubs", s);
   uint32_t lo = 0;
-  for (uint32_t hb : {1u << 12, 1u << 16}) {
+#ifndef CAC_AF_PREFIXES
+#define CAC_AF_PREFIXES 1u << 12, 1u << 16
+#endif
+  for (uint32_t hb : {CAC_AF_PREFIXES}) {
     const uint32_t hi = n < hb ? n : hb;
     if (hi <= lo) continue;
     hipLaunchKernelGGL(k_af_round<false>, dim3(grid_for(hi - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols,
-                       lo, hi, d_lab);
+                       lo, hi, d_lab, d_hooks, (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_cc_settle, dim3(grid_for(hi, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_lab, hi);
     lo = hi;
   }
@@ -549,20 +759,25 @@ int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, u
   if (lo < n) {
     prof_begin(ctx, "af_round", s);
     hipLaunchKernelGGL(k_af_round<true>, dim3(grid_for(n - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols, lo,
-                       n, d_lab);
+                       n, d_lab, d_hooks, d_counter + 1);
     prof_end(ctx, s);
   }
-  prof_begin(ctx, "af_mode", s);
-  hipLaunchKernelGGL(k_cc_mode, dim3(1), dim3(1024), 0, s, (const uint32_t*)d_lab, n, d_counter + 1);
-  prof_end(ctx, s);
+  if (lo >= n) {  // no bulk range (n within the hub prefixes): a separate sampling launch
+    prof_begin(ctx, "af_mode", s);
+    hipLaunchKernelGGL(k_cc_mode, dim3(1), dim3(1024), 0, s, (const uint32_t*)d_lab, n, d_counter + 1);
+    prof_end(ctx, s);
+  }
   prof_begin(ctx, "af_final", s);
-  hipLaunchKernelGGL(k_af_final, dim3(grid_for(n, kCcBlock, 4096)), dim3(kCcBlock), 0, s, d_off, d_cols, n, d_lab,
-                     (const uint32_t*)(d_counter + 1), d_heavy, d_counter + 2);
-  hipLaunchKernelGGL(k_af_heavy, dim3(1024), dim3(kCcBlock), 0, s, d_off, d_cols, d_lab, (const uint32_t*)d_heavy,
-                     (const uint32_t*)(d_counter + 2));
+  hipLaunchKernelGGL(k_af_final, dim3(grid_for(((uint64_t)n + 64 * kAfChunks - 1) / (64 * kAfChunks), kCcBlock / 64, 4096)),
+                     dim3(kCcBlock), 0, s, d_off, d_cols, n, d_lab,
+                     (const uint32_t*)(d_counter + 1), d_heavy, d_counter + 2, d_outside, d_hooks);
+  hipLaunchKernelGGL(k_af_heavy, dim3(256), dim3(kCcBlock), 0, s, d_off, d_cols, d_lab, (const uint32_t*)d_heavy,
+                     (const uint32_t*)(d_counter + 2), d_hooks);
   prof_end(ctx, s);
-  prof_begin(ctx, "af_compress", s);
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n, kCcBlock, 4096)), dim3(kCcBlock), 0, s, d_lab, n, d_counter);
+  prof_begin(ctx, "af_fix", s);
+  hipLaunchKernelGGL(k_af_fix, dim3(grid_for(n, kCcBlock, 1024)), dim3(kCcBlock), 0, s, d_lab, n,
+                     (const uint64_t*)d_outside, (const uint32_t*)(d_counter + 1), d_counter,
+                     (const uint32_t*)d_hooks);
   prof_end(ctx, s);
   if (label_mode == CAC_LABEL_RANK) {
     uint32_t* rank_of = d_rank_tmp;

@@ -58,10 +58,19 @@ def _graph(kind, rng):
         n = 90000
         p = rng.permutation(n).reshape(30, -1)
         return n, np.concatenate([np.stack([q[:-1], q[1:]], 1) for q in p]).astype(np.uint32)
+    if kind == "hooked-giant":
+        # Star 1000 <- 1001..19999 is the sampled component.  Vertex 30000 (first
+        # neighbours 0, 1) reaches leaf 1001 only through its third neighbour, and
+        # 1001 sees 30000 only after 1000 and 1002, so root 1000 is hooked under 0
+        # in the final pass: every vertex of the star needs its label redone.
+        n = 30001
+        star = np.stack([np.full(18999, 1000), np.arange(1001, 20000)], 1)
+        extra = np.array([[1001, 1002], [1001, 1003], [30000, 0], [30000, 1], [30000, 1001]])
+        return n, np.concatenate([star, extra]).astype(np.uint32)
     raise ValueError(kind)
 
 
-KINDS = ["empty", "single", "random-sparse", "random-loops-dups", "two-stars", "path", "many-paths"]
+KINDS = ["empty", "single", "random-sparse", "random-loops-dups", "two-stars", "path", "many-paths", "hooked-giant"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
