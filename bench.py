@@ -89,11 +89,11 @@ def kernel_bytes(name: str, m: int, n: int) -> float:
         "cc_compress": 8.0 * n,            # read + write labels
         "af_init": 4.0 * n,                # write parent
         "af_hubs": 0.0,                    # the low-id prefix ranges (2^16 vertices): negligible
-        "af_round": 28.0 * n,              # per vertex: offsets 4, two neighbours 8, own parent read+write 8,
-                                           # the two neighbours' parents 8
-        "af_mode": 0.0,                    # 4096 sampled roots: negligible
-        "af_final": 12.0 * n,              # offsets and parent of every vertex, parent rewritten to its root
-        "af_compress": 8.0 * n,            # read + write labels
+        "af_round": 16.0 * n,              # per vertex: offsets 4, first neighbour 4, its parent 4,
+                                           # own parent written 4
+        "af_mode": 0.0,                    # only when n fits the hub prefixes: negligible
+        "af_final": 8.0 * n,               # parent of every vertex read, rewritten to its root
+        "af_fix": n / 8.0,                 # the bitmap of vertices outside the sampled component
     }.get(name, 0.0)
 
 

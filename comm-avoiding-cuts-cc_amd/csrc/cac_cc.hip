@@ -107,6 +107,31 @@ __device__ inline uint32_t uf_link_from(uint32_t* par, uint32_t a, uint32_t pa, 
   return 0u;
 }
 
+// Union without path splitting: the only stores are CAS hooks of roots.  Used
+// once vertices have been re-pointed at their final roots (k_af_final,
+// k_af_heavy): a path-splitting store built from a stale (older) parent could
+// otherwise put back an older ancestor over such a vertex's root.
+__device__ inline uint32_t uf_find_ro(const uint32_t* par, uint32_t x) {
+  uint32_t p = ld_par(par, x);
+  while (p < x) {
+    x = p;
+    p = ld_par(par, x);
+  }
+  return x;
+}
+
+__device__ inline uint32_t uf_link_ro(uint32_t* par, uint32_t a, uint32_t b) {
+  uint32_t ra = uf_find_ro(par, a), rb = uf_find_ro(par, b);
+  while (ra != rb) {
+    const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
+    const uint32_t old = atomicCAS(par + hi, hi, lo);
+    if (old == hi) return 1u;
+    ra = uf_find_ro(par, old);
+    rb = uf_find_ro(par, lo);
+  }
+  return 0u;
+}
+
 __device__ inline uint32_t uf_link(uint32_t* par, uint32_t a, uint32_t b) {
   return uf_link_from(par, a, ld_par(par, a), b, ld_par(par, b));
 }
@@ -319,12 +344,15 @@ __global__ void __launch_bounds__(kCcBlock) k_check_ids(const cac_uedge* e, uint
 
 // ---- connected components on a symmetric CSR (Afforest) ---------------------
 // Sutton, Ben-Nun, Barak, "Optimizing Parallel Graph Connectivity Computation via
-// Subgraph Sampling" (IPDPS 2018): link every vertex with its first kAfRounds
-// neighbours, compress, find the most frequent root by sampling, then link only
-// the remaining neighbours of vertices outside that component.  Every edge is in
-// both adjacency lists, so an edge from a vertex of the big component to one
-// outside it is still linked from the outside end.
-constexpr uint32_t kAfRounds = 2;
+// Subgraph Sampling" (IPDPS 2018): link a sample of edges, find the most frequent
+// root, then link every remaining edge of the vertices outside that component.
+// Every edge is in both adjacency lists, so an edge from a vertex of the big
+// component to one outside it is still linked from the outside end.  The
+// sample: the low-id hub ranges link their first two neighbours, every other
+// vertex its first (smallest) one; so the final pass starts at index kAfSkip.
+// Linking the second neighbour in the bulk pass as well cost more (its parent
+// lookups are random lines of the parent array) than the final pass saves.
+constexpr uint32_t kAfSkip = 1;
 constexpr uint32_t kAfHeavy = 1024;  // a vertex with more remaining neighbours goes to a workgroup
 // Successful hooks are counted per wavefront and added to one of kHookSlots
 // counters 64 bytes apart, so no single word takes an atomic from every wave.  Components = n - hooks: each hook
@@ -354,13 +382,7 @@ __global__ void __launch_bounds__(kCcBlock) k_af_init(uint32_t* __restrict__ par
     uint4* p4 = reinterpret_cast<uint4*>(par);
     for (uint64_t q = tid; q < n / 4; q += T) {
       const uint32_t v = (uint32_t)(q * 4);
-#ifdef CAC_AF_NTINIT
-      typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-      const v4u x = {v, v + 1, v + 2, v + 3};
-      __builtin_nontemporal_store(x, reinterpret_cast<v4u*>(p4 + q));
-#else
       p4[q] = make_uint4(v, v + 1, v + 2, v + 3);
-#endif
     }
     tail = (uint64_t)(n / 4) * 4;
   }
@@ -411,26 +433,24 @@ __device__ inline void af_majority_root(uint32_t* par, uint32_t range, uint32_t*
   if (lane == 0) out[0] = cand;
 }
 
-// Vertices [v_lo, v_hi) link with their first kAfRounds neighbours (lists are
-// sorted by id, so these tend to be the low-id hubs of skewed graphs).  Both
-// rounds share one pass: with ~128 bytes of adjacency per vertex, reading a
-// first neighbour already touches nearly every line of the column array.
-// kBulk: the last range (every vertex above the hub prefixes [0, v_lo), which
-// are linked and pointer-jumped by now).  Its first wavefront also guesses the
-// root of the largest component from the hub forest (af_majority_root), which
-// saves a launch between this pass and k_af_final.
+// Vertices [v_lo, v_hi) link with their first neighbours (lists are sorted by
+// id, so these tend to be the low-id hubs of skewed graphs): two in the hub
+// ranges, one in the bulk range.  kBulk: the last range (every vertex above the
+// hub prefixes [0, v_lo), which are linked and pointer-jumped by now).  Its
+// first wavefront also guesses the root of the largest component from the hub
+// forest (af_majority_root), which saves a launch before k_af_final.
 template <bool kBulk>
 __global__ void __launch_bounds__(kCcBlock) k_af_round(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ cols, uint32_t v_lo, uint32_t v_hi,
                                                        uint32_t* par, uint32_t* hook_slots, uint32_t* root_out) {
-  static_assert(kAfRounds == 2, "k_af_round links two neighbours");
   if (kBulk && blockIdx.x == 0 && threadIdx.x < 64) af_majority_root(par, v_lo, root_out);
   uint32_t hooks = 0;
   for (uint64_t i = v_lo + (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < v_hi; i += (uint64_t)gridDim.x * kCcBlock) {
     const uint32_t u = (uint32_t)i, b = off[u], e = off[u + 1];
-    // both first neighbours with one 8-byte load (4-byte aligned is enough)
     uint32_t v0 = u, v1 = u;
-    if (b + 1 < e) {
+    if (kBulk) {
+      if (b < e) v0 = cols[b];
+    } else if (b + 1 < e) {  // both first neighbours with one 8-byte load (4-byte aligned is enough)
       uint2 x;
       __builtin_memcpy(&x, cols + b, 8);
       v0 = x.x;
@@ -441,14 +461,14 @@ __global__ void __launch_bounds__(kCcBlock) k_af_round(const uint32_t* __restric
     // u is taken to be a root without loading par[u]: the CAS checks it, and a
     // failed CAS climbs from the value it returns
     if (v0 != u) hooks += uf_link_from(par, u, u, v0, ld_par(par, v0));
-    if (v1 != u) hooks += uf_link(par, u, v1);
+    if (!kBulk && v1 != u) hooks += uf_link(par, u, v1);
   }
   add_hooks(hook_slots, hooks);
 }
 
 // Every vertex is re-pointed at its root (the pointer jumping of jump_pass,
 // fused here); those whose root is not the sampled root R then link their
-// neighbours from index kAfRounds on.  A wavefront links the remaining
+// neighbours from index kAfSkip on.  A wavefront links the remaining
 // neighbours of all its vertices together: an exclusive scan of the per-lane
 // counts lays them out as one list, and lane j of each 64-entry step links
 // entry j with its owner (found by a binary search over the scan), so one long
@@ -501,7 +521,7 @@ __global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restric
       const uint64_t mask = __ballot(e[k] > b[k]);
       if (c0 < n && lane == 0) outside[c0 >> 6] = mask;
       if (!mask) continue;
-      const uint32_t u = (uint32_t)(c0 + lane), bb = b[k] + kAfRounds;
+      const uint32_t u = (uint32_t)(c0 + lane), bb = b[k] + kAfSkip;
       uint32_t c = bb < e[k] ? e[k] - bb : 0u;
       if (c > kAfHeavy) {
         heavy[atomicAdd(heavy_count, 1u)] = u;
@@ -520,7 +540,7 @@ __global__ void __launch_bounds__(kCcBlock) k_af_final(const uint32_t* __restric
           if (__shfl(incl, (int)L + st - 1, 64) <= j) L += st;
         const uint32_t ou = __shfl(u, (int)L, 64), ob = __shfl(bb, (int)L, 64);
         const uint32_t ofirst = __shfl(incl - c, (int)L, 64);
-        if (j < total) hooks += uf_link(par, ou, cols[ob + (j - ofirst)]);
+        if (j < total) hooks += uf_link_ro(par, ou, cols[ob + (j - ofirst)]);
       }
     }
   }
@@ -536,8 +556,8 @@ __global__ void __launch_bounds__(kCcBlock) k_af_heavy(const uint32_t* __restric
   uint32_t hooks = 0;
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t u = heavy[q];
-    const uint32_t b = off[u] + kAfRounds, e = off[u + 1];
-    for (uint32_t k = b + threadIdx.x; k < e; k += kCcBlock) hooks += uf_link(par, u, cols[k]);
+    const uint32_t b = off[u] + kAfSkip, e = off[u + 1];
+    for (uint32_t k = b + threadIdx.x; k < e; k += kCcBlock) hooks += uf_link_ro(par, u, cols[k]);
   }
   add_hooks(hook_slots, hooks);
 }
@@ -744,10 +764,7 @@ int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, u
   // one hop long (0.99 ms -> 0.58 ms for the round on R-MAT 2^24 / 2^28).
   prof_begin(ctx, "af_hubs", s);
   uint32_t lo = 0;
-#ifndef CAC_AF_PREFIXES
-#define CAC_AF_PREFIXES 1u << 12, 1u << 16
-#endif
-  for (uint32_t hb : {CAC_AF_PREFIXES}) {
+  for (uint32_t hb : {1u << 12, 1u << 16}) {
     const uint32_t hi = n < hb ? n : hb;
     if (hi <= lo) continue;
     hipLaunchKernelGGL(k_af_round<false>, dim3(grid_for(hi - lo, kCcBlock, cap)), dim3(kCcBlock), 0, s, d_off, d_cols,
