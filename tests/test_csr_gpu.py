@@ -67,10 +67,24 @@ def _graph(kind, rng):
         star = np.stack([np.full(18999, 1000), np.arange(1001, 20000)], 1)
         extra = np.array([[1001, 1002], [1001, 1003], [30000, 0], [30000, 1], [30000, 1001]])
         return n, np.concatenate([star, extra]).astype(np.uint32)
+    if kind == "random-above-hubs":
+        # beyond the 2^16 hub prefixes (bulk pass, sampled root), average degree 3:
+        # many vertices stay outside the sampled component and link in the final pass
+        n = 150000
+        return n, rng.integers(0, n, size=(225000, 2)).astype(np.uint32)
+    if kind == "hooked-giant-above-hubs":
+        # as hooked-giant with a bulk range: 70000 links only its first neighbour 0
+        # in the bulk pass and reaches leaf 1001 in the final pass, hooking the
+        # sampled root 1000 under 0
+        n = 70001
+        star = np.stack([np.full(58999, 1000), np.arange(1001, 60000)], 1)
+        extra = np.array([[1001, 1002], [1001, 1003], [70000, 0], [70000, 1001]])
+        return n, np.concatenate([star, extra]).astype(np.uint32)
     raise ValueError(kind)
 
 
-KINDS = ["empty", "single", "random-sparse", "random-loops-dups", "two-stars", "path", "many-paths", "hooked-giant"]
+KINDS = ["empty", "single", "random-sparse", "random-loops-dups", "two-stars", "path", "many-paths", "hooked-giant",
+         "random-above-hubs", "hooked-giant-above-hubs"]
 
 
 @pytest.mark.parametrize("kind", KINDS)
