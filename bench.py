@@ -381,7 +381,8 @@ def bench_c4(ctx, cac, count: int, stride: int):
             "one_graph_per_call_sample": f"every {stride}th graph ({len(sub)} graphs)",
             "speedup": batched_rate / single_rate,
             "success_fraction": float((cuts == planted).mean()),
-            "batched_equals_single": bool(np.array_equal(np.array(single), cuts[sub]))}
+            "batched_equals_single": bool(np.array_equal(np.array(single), cuts[sub])),
+            "launch_shapes": ctx.tuning_file or "built-in rule (no autotuner table)"}
 
 
 def bench_c5(cac, count: int, rate: float):
@@ -486,6 +487,17 @@ def main():
     csr_build_ms = (time.perf_counter() - t0) * 1e3
     csr = measure(lambda: ctx.cc_csr_dev(d_off.ptr, d_cols.ptr, n, d_lab.ptr), "csr")
     csr["labels_equal_edge_list_path"] = bool(np.array_equal(d_lab.download(np.uint32, n), lab_edges))
+    # the same step replayed from a captured HIP graph (cac_cc_csr_graph_dev)
+    ctx.cc_csr_graph_dev(d_off.ptr, d_cols.ptr, n, d_lab.ptr)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gnc = ctx.cc_csr_graph_dev(d_off.ptr, d_cols.ptr, n, d_lab.ptr)
+    ctx.synchronize()
+    gdt = (time.perf_counter() - t0) / args.steps
+    csr["graph_replay"] = {"ms_per_step": gdt * 1e3, "gteps": m / gdt / 1e9, "components": gnc,
+                           "labels_equal_edge_list_path": bool(np.array_equal(d_lab.download(np.uint32, n),
+                                                                              lab_edges))}
     csr["csr_build_ms"] = csr_build_ms
     csr["nnz"] = nnz
     del lab_edges

@@ -1,0 +1,104 @@
+"""Autotuner for the launch shapes of libcac.so on the device it runs on.
+
+Knob CAC_TUNE_KS_CONTRACT_BLOCK: threads per workgroup of the dense
+Karger-Stein contraction (k_ks_contract<B>), per row-size class (key 512 holds
+children of at most 512 vertices, keys 1024, 2048 and 4096 hold s in
+(key/2, key]).  The built-in rule takes the smallest power of two with at most
+eight row entries per thread; larger blocks give each thread fewer entries and
+a workgroup more waves, which pays or not depending on how many children of
+that size run at once and on their row length.
+
+For each class, largest first, every candidate block size (powers of two from
+the class minimum to 1024) is timed on a dense Karger-Stein solve whose top
+levels lie in that class: a complete graph with n = key vertices and weights
+U[1, 42], the classes above it held at their tuned values and the ones below at
+the built-in rule.  The fastest candidate whose raw per-trial cuts equal the
+built-in rule's wins (results do not depend on the block size; a candidate that
+changed one would be a bug, and the tuner stops).
+
+Writes {"device": ..., "ks_contract_block": {"512": B, ...}, "timings_ms": ...}
+to tuning/gfx950.json next to this file (or --out); every cac.Context applies
+that file when it exists.  Run on the target GPU:
+
+    python comm-avoiding-cuts-cc_amd/autotune.py [--out PATH] [--reps 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+import cac  # noqa: E402
+
+# (class key, trials per timed solve): about 0.05-0.2 s per solve on one MI355X
+CLASSES = ((4096, 32), (2048, 64), (1024, 128), (512, 256))
+BLOCKS = (64, 128, 256, 512, 1024)
+
+
+def candidates(key: int):
+    lo = max(64, key // 8)
+    return [b for b in BLOCKS if b >= lo]
+
+
+def time_solve(ctx, d_adj: int, n: int, trials: int, reps: int):
+    raw, _ = ctx.ks_dense_dev(d_adj, n, 0, 0, trials)  # warm-up (grows the level buffers)
+    times = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r, _ = ctx.ks_dense_dev(d_adj, n, 0, 0, trials)
+        times.append((time.perf_counter() - t0) * 1e3)
+        if r.tolist() != raw.tolist():
+            raise RuntimeError("repeated solve changed a raw cut")
+    times.sort()
+    return times[len(times) // 2], raw
+
+
+def tune(device: int = 0, reps: int = 3, log=print):
+    table, timings = {}, {}
+    with cac.Context(device, tuning=False) as ctx:
+        for key, trials in CLASSES:
+            n = key
+            d_adj = ctx.alloc(n * n * 8)
+            ctx.gen_complete_dev(n, 0, 42, d_adj.ptr)
+            for k2, b2 in table.items():  # classes above: their tuned values
+                ctx.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, int(k2), b2)
+            ctx.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, key, 0)
+            base_ms, base_raw = time_solve(ctx, d_adj.ptr, n, trials, reps)
+            res = {}
+            for b in candidates(key):
+                ctx.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, key, b)
+                ms, raw = time_solve(ctx, d_adj.ptr, n, trials, reps)
+                if raw.tolist() != base_raw.tolist():
+                    raise RuntimeError(f"block {b} for class {key} changed a raw cut")
+                res[b] = ms
+                log(f"class {key}: block {b:4d}  {ms:8.2f} ms  ({trials} trials, n={n})")
+            best = min(res, key=res.get)
+            table[str(key)] = best
+            timings[str(key)] = {"n": n, "trials": trials, "built_in_ms": base_ms,
+                                 "ms": {str(b): v for b, v in res.items()}}
+            log(f"class {key}: picked {best} (built-in rule {base_ms:.2f} ms, picked {res[best]:.2f} ms)")
+            ctx.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, key, 0)
+            d_adj.free()
+    return table, timings
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=str(cac.TUNING_PATH))
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args()
+    table, timings = tune(args.device, args.reps, log=lambda s: print(s, flush=True))
+    out = {"device": "gfx950 (MI355X)", "knob": "ks_contract_block", "ks_contract_block": table,
+           "timings_ms": timings}
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    Path(args.out).write_text(json.dumps(out, indent=1) + "\n")
+    print(f"wrote {args.out}: {table}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -14,6 +14,10 @@ import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libcac.so"
+# Launch shapes written by autotune.py; every Context applies them when present.
+TUNING_PATH = PKG_DIR / "tuning" / "gfx950.json"
+TUNE_KS_CONTRACT_BLOCK = 1
+TUNE_KS_CONTRACT_KEYS = (512, 1024, 2048, 4096)
 
 CAC_OK = 0
 CAC_EINVAL = -1
@@ -60,6 +64,8 @@ SIGNATURES = {
     "cac_memcpy_h2d": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
     "cac_memcpy_d2h": (_i32, [_vp, _vp, _vp, ctypes.c_size_t]),
     "cac_memcpy_d2d": (_i32, [_vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "cac_tune_set": (_i32, [_vp, _i32, _u32, ctypes.c_int32]),
+    "cac_tune_get": (_i32, [_vp, _i32, _u32, ctypes.POINTER(ctypes.c_int32)]),
     "cac_prof_enable": (_i32, [_vp, _i32]),
     "cac_prof_reset": (_i32, [_vp]),
     "cac_prof_count": (_i32, [_vp]),
@@ -67,6 +73,7 @@ SIGNATURES = {
     "cac_cc": (_i32, [_vp, _vp, _u64, _u32, _i32, _pu32, _pu32]),
     "cac_cc_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
     "cac_cc_csr_dev": (_i32, [_vp, _vp, _vp, _u32, _i32, _vp, _pu32, _vp]),
+    "cac_cc_csr_graph_dev": (_i32, [_vp, _vp, _vp, _u32, _i32, _vp, _pu32, _vp]),
     "cac_csr_build_dev": (_i32, [_vp, _vp, _u64, _u32, _vp, _vp, _pu64, _vp]),
     "cac_gen_rmat_dev": (_i32, [_vp, _u32, _u64, _u32, _u32, _u32, _u32, _vp, _pu64]),
     "cac_ks_dense": (_i32, [_vp, _vp, _u32, _u32, _u32, _u32, _pi64, _pi64]),
@@ -164,13 +171,38 @@ class DeviceBuffer:
 class Context:
     """One cac_ctx on one device (not thread-safe, like the C object)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuning: str | os.PathLike | None | bool = None):
+        """tuning: a JSON file written by autotune.py (default: TUNING_PATH when it
+        exists), or False for the built-in launch shapes."""
         self.lib = load_library()
         h = _vp()
         rc = self.lib.cac_ctx_create(device, ctypes.byref(h))
         if rc != CAC_OK:
             raise CacError(rc, f"cac_ctx_create: {self.lib.cac_strerror(rc).decode()}")
         self.handle = h.value
+        self.tuning_file = None
+        if tuning is not False:
+            path = Path(tuning) if tuning else TUNING_PATH
+            if tuning or path.exists():
+                self.load_tuning(path)
+
+    # ---- tuning -----------------------------------------------------------
+    def tune_set(self, knob: int, key: int, value: int):
+        self._check(self.lib.cac_tune_set(self.handle, knob, key, value))
+
+    def tune_get(self, knob: int, key: int) -> int:
+        v = ctypes.c_int32()
+        self._check(self.lib.cac_tune_get(self.handle, knob, key, ctypes.byref(v)))
+        return int(v.value)
+
+    def load_tuning(self, path: str | os.PathLike):
+        """Apply a table written by autotune.py: {"ks_contract_block": {"512": B, ...}}."""
+        import json
+
+        table = json.loads(Path(path).read_text())
+        for key, value in table.get("ks_contract_block", {}).items():
+            self.tune_set(TUNE_KS_CONTRACT_BLOCK, int(key), int(value))
+        self.tuning_file = str(path)
 
     def close(self):
         if self.handle:
@@ -246,6 +278,14 @@ class Context:
         nc = ctypes.c_uint32()
         self._check(self.lib.cac_cc_csr_dev(self.handle, d_offsets, d_cols, n, mode, d_labels, ctypes.byref(nc),
                                             stream))
+        return int(nc.value)
+
+    def cc_csr_graph_dev(self, d_offsets: int, d_cols: int, n: int, d_labels: int, mode: int = LABEL_MIN_ID,
+                         stream=None) -> int:
+        """cc_csr_dev replayed from a captured HIP graph (captured on first use)."""
+        nc = ctypes.c_uint32()
+        self._check(self.lib.cac_cc_csr_graph_dev(self.handle, d_offsets, d_cols, n, mode, d_labels,
+                                                  ctypes.byref(nc), stream))
         return int(nc.value)
 
     def cc_csr(self, edges, n: int, mode: int = LABEL_MIN_ID):

@@ -895,6 +895,67 @@ int cac_cc_csr_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_co
   return CAC_OK;
 }
 
+int cac_cc_csr_graph_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_cols, uint32_t n, int label_mode,
+                         uint32_t* d_labels, uint32_t* ncomp_out, void* stream) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if (!d_offsets || (n && (!d_labels || !d_cols)) || !ncomp_out) return cac::fail(ctx, CAC_EINVAL, "null pointer");
+  if (label_mode != CAC_LABEL_MIN_ID && label_mode != CAC_LABEL_RANK)
+    return cac::fail(ctx, CAC_EINVAL, "label_mode must be CAC_LABEL_MIN_ID or CAC_LABEL_RANK");
+  if (n == 0) {
+    *ncomp_out = 0;
+    return CAC_OK;
+  }
+  hipStream_t s = cac::pick_stream(ctx, stream);
+  void* base = nullptr;
+  rc = cac::scratch_reserve(ctx, cac::cc_csr_tmp_words(n, label_mode) * 4, &base);
+  if (rc) return rc;
+  uint32_t* counter = (uint32_t*)base;
+  const void* ptrs[5] = {d_offsets, d_cols, d_labels, base, (const void*)s};
+  const uint64_t args[2] = {n, (uint64_t)label_mode};
+  bool same = ctx->cc_graph != nullptr && args[0] == ctx->cc_graph_args[0] && args[1] == ctx->cc_graph_args[1];
+  for (int i = 0; same && i < 5; ++i) same = ptrs[i] == ctx->cc_graph_ptrs[i];
+  if (!same) {
+    // Capture the launch sequence of cac_cc_csr_dev (kernels only: no memset
+    // or copy nodes) with per-kernel timing off, then instantiate it.
+    if (ctx->cc_graph) {
+      CAC_HIP(ctx, hipGraphExecDestroy(ctx->cc_graph));
+      ctx->cc_graph = nullptr;
+    }
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      ctx->profiling = prof;
+      return cac::hip_fail(ctx, e, "hipStreamBeginCapture");
+    }
+    rc = cac::cc_csr_launch(ctx, d_offsets, d_cols, n, label_mode, d_labels, counter, s);
+    e = hipStreamEndCapture(s, &g);
+    ctx->profiling = prof;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return cac::hip_fail(ctx, e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return cac::hip_fail(ctx, e, "hipGraphInstantiate");
+    ctx->cc_graph = x;
+    for (int i = 0; i < 5; ++i) ctx->cc_graph_ptrs[i] = ptrs[i];
+    ctx->cc_graph_args[0] = args[0];
+    ctx->cc_graph_args[1] = args[1];
+  }
+  cac::prof_begin(ctx, "cc_csr_graph", s);
+  CAC_HIP(ctx, hipGraphLaunch(ctx->cc_graph, s));
+  cac::prof_end(ctx, s);
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, counter, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  *ncomp_out = *(uint32_t*)ctx->pinned;
+  return CAC_OK;
+}
+
 int cac_csr_build_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, uint32_t* d_offsets,
                       uint32_t* d_cols, uint64_t* nnz_out, void* stream) {
   int rc = cac::bind(ctx);

@@ -168,6 +168,7 @@ int cac_ctx_destroy(cac_ctx* ctx) {
     if (r.stop) (void)hipEventDestroy(r.stop);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->cc_graph) (void)hipGraphExecDestroy(ctx->cc_graph);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   for (int b = 0; b < 2; ++b)
     if (ctx->ks_buf[b]) (void)hipFree(ctx->ks_buf[b]);
@@ -259,6 +260,39 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
                      (const unsigned char*)src + n16 * 16, tail);
   CAC_LAUNCH_CHECK(ctx, "k_copy");
   cac::prof_end(ctx, s);
+  return CAC_OK;
+}
+
+// Tuning table: CAC_TUNE_KS_CONTRACT_BLOCK keys 512, 1024, 2048, 4096 (row-size
+// class upper bounds); a value is 0 (built-in rule) or a power of two from
+// key / 8 (at least 64) to 1024.
+static int tune_slot(int knob, uint32_t key) {
+  if (knob != CAC_TUNE_KS_CONTRACT_BLOCK) return -1;
+  switch (key) {
+    case 512: return 0;
+    case 1024: return 1;
+    case 2048: return 2;
+    case 4096: return 3;
+    default: return -1;
+  }
+}
+
+int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value) {
+  if (!ctx) return CAC_EINVAL;
+  const int slot = tune_slot(knob, key);
+  if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
+  const int32_t lo = key / 8 < 64 ? 64 : (int32_t)(key / 8);
+  if (value != 0 && (value < lo || value > 1024 || (value & (value - 1))))
+    return cac::fail(ctx, CAC_EINVAL, "block size must be 0 or a power of two from key/8 (at least 64) to 1024");
+  ctx->ks_contract_block[slot] = value;
+  return CAC_OK;
+}
+
+int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value) {
+  if (!ctx || !value) return CAC_EINVAL;
+  const int slot = tune_slot(knob, key);
+  if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
+  *value = ctx->ks_contract_block[slot];
   return CAC_OK;
 }
 

@@ -53,6 +53,15 @@ struct cac_ctx {
   // released when an allocation fails.
   std::vector<int> pins;
   size_t smp_wave = 0;  // sampling workgroups resident at once (0: not asked yet)
+  // Tuned launch shapes (cac_tune_set): dense Karger-Stein contraction block
+  // size per row-size class, keys 512, 1024, 2048, 4096; 0 = built-in rule.
+  int32_t ks_contract_block[4] = {0, 0, 0, 0};
+  // The captured CSR connected-components step (cac_cc_csr_graph_dev) and the
+  // arguments it was captured for: offsets, columns, labels, scratch, stream,
+  // n, label mode.
+  hipGraphExec_t cc_graph = nullptr;
+  const void* cc_graph_ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint64_t cc_graph_args[2] = {0, 0};
 };
 
 namespace cac {

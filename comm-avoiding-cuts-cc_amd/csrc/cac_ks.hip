@@ -640,10 +640,14 @@ __global__ void k_zero_words(unsigned long long* p, uint32_t count) {
 }
 
 // Block size for a child of s vertices: the smallest power of two from 64 up
-// with at most kKsChunk row entries per thread.
-inline int contract_block(uint32_t s) {
+// with at most kKsChunk row entries per thread, unless the context holds a
+// tuned size for the row-size class of s (cac_tune_set); any power of two from
+// that minimum to 1024 gives the same results.
+inline int contract_block(const cac_ctx* ctx, uint32_t s) {
   int B = 64;
   while (B < 1024 && (uint32_t)B * kKsChunk < s) B *= 2;
+  const int cls = B == 64 ? 0 : B == 128 ? 1 : B == 256 ? 2 : B == 512 ? 3 : -1;
+  if (cls >= 0 && ctx->ks_contract_block[cls] >= B) B = ctx->ks_contract_block[cls];
   return B;
 }
 
@@ -741,10 +745,10 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
   prof_begin(ctx, "ks_contract", s);
   size_t c0 = 0;
   while (c0 < nchild) {
-    const int B = contract_block(ch[c0].s);
+    const int B = contract_block(ctx, ch[c0].s);
     size_t c1 = c0;
     uint32_t smax = 0;
-    while (c1 < nchild && contract_block(ch[c1].s) == B) smax = std::max(smax, ch[c1++].s);
+    while (c1 < nchild && contract_block(ctx, ch[c1].s) == B) smax = std::max(smax, ch[c1++].s);
     const uint32_t cnt = (uint32_t)(c1 - c0);
     if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
     else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
@@ -833,7 +837,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     if (internal.empty()) break;
     // children sorted by block-size class (1024, 256, 64), two per internal node
     std::vector<std::pair<int, uint32_t>> order;
-    for (uint32_t i = 0; i < internal.size(); ++i) order.push_back({-contract_block(internal[i].s), i});
+    for (uint32_t i = 0; i < internal.size(); ++i) order.push_back({-contract_block(ctx, internal[i].s), i});
     std::stable_sort(order.begin(), order.end());
     const size_t nchild = 2 * internal.size();
     size_t arrays = 0, units = 0, max_units_child = 0;

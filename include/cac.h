@@ -74,6 +74,19 @@ int cac_memcpy_d2h(cac_ctx* ctx, void* dst, const void* src, size_t bytes);
  * counters.  No counterpart in the reference. */
 int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 
+/* Launch shapes picked by the autotuner (comm-avoiding-cuts-cc_amd/autotune.py),
+ * held per context; results never depend on them.  No counterpart in the
+ * reference, whose launch shape is one MPI rank per core.
+ * CAC_TUNE_KS_CONTRACT_BLOCK: threads per workgroup of the dense Karger-Stein
+ * contraction for children of s vertices, keyed by row-size class: key 512
+ * holds s <= 512, keys 1024, 2048 and 4096 hold s in (key/2, key] (s above 4096
+ * always uses 1024).  value 0 restores the built-in rule (the smallest power
+ * of two >= s/8, at least 64); otherwise a power of two from max(64, key/8) to
+ * 1024, CAC_EINVAL else. */
+#define CAC_TUNE_KS_CONTRACT_BLOCK 1
+int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value);
+int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value);
+
 /* Per-kernel timing with HIP events on the stream each kernel is launched on.
  * No counterpart in the reference beyond TimeUtils::measure (src/utils.hpp:21-40). */
 int cac_prof_enable(cac_ctx* ctx, int on);
@@ -103,14 +116,26 @@ int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, i
  * The same labels as cac_cc_dev (CAC_LABEL_MIN_ID / CAC_LABEL_RANK, component
  * count), for a graph held as a symmetric CSR: d_offsets[n + 1], d_cols[nnz],
  * every undirected edge in both adjacency lists (loops and repeats allowed).
- * Afforest (Sutton, Ben-Nun and Barak, IPDPS 2018): every vertex links with its
- * first two neighbours, the most frequent root is found by sampling, and only
- * vertices outside it link their other neighbours, so most of the adjacency of
- * a graph with a giant component is never read.  Replaces the same
+ * Afforest (Sutton, Ben-Nun and Barak, IPDPS 2018): the low-id vertices link
+ * with their first two neighbours and every other vertex with its first, the
+ * root of the largest component is guessed by sampling, and only vertices
+ * outside it link their other neighbours, so most of the adjacency of a graph
+ * with a giant component is never read.  Replaces the same
  * connectedComponents call as cac_cc_dev (src/UnweightedIteratedSparseSampling.cpp:40-72)
  * for callers that hold a CSR.  Ids are not validated. */
 int cac_cc_csr_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_cols, uint32_t n, int label_mode,
                    uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
+
+/* cac_cc_csr_dev through a captured HIP graph: the first call with a given
+ * (d_offsets, d_cols, n, label_mode, d_labels, stream) records the kernel
+ * sequence of cac_cc_csr_dev into a graph held by the context; that call and
+ * every later one with the same arguments replay it as one launch (a call with
+ * other arguments, or after the context's scratch moved, captures again).  The
+ * graph holds kernel nodes only.  Same labels and count as cac_cc_csr_dev.  For
+ * callers that repeat the step on resident data (the serving pattern); no
+ * counterpart in the reference. */
+int cac_cc_csr_graph_dev(cac_ctx* ctx, const uint32_t* d_offsets, const uint32_t* d_cols, uint32_t n,
+                         int label_mode, uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
 /* Symmetric CSR of an undirected device edge list (ids < n, checked:
  * CAC_EINVAL otherwise; fewer than 2^31 edges): both orientations of every non-loop edge, each adjacency list sorted
  * by neighbour id; d_offsets[n + 1], d_cols[2 m] (the first *nnz_out used).

@@ -27,6 +27,25 @@ def test_ctx_create_rejects_null_out():
     assert lib.cac_ctx_create(0, None) == cac.CAC_EINVAL
 
 
+def test_tune_rejects_null_ctx():
+    lib = cac.load_library()
+    assert lib.cac_tune_set(None, cac.TUNE_KS_CONTRACT_BLOCK, 512, 128) == cac.CAC_EINVAL
+    v = ctypes.c_int32()
+    assert lib.cac_tune_get(None, cac.TUNE_KS_CONTRACT_BLOCK, 512, ctypes.byref(v)) == cac.CAC_EINVAL
+
+
+def test_tuning_file_shape():
+    # a committed tuning table names only known classes and legal block sizes
+    import json
+
+    if not cac.TUNING_PATH.exists():
+        return
+    t = json.loads(cac.TUNING_PATH.read_text())["ks_contract_block"]
+    for key, b in t.items():
+        assert int(key) in cac.TUNE_KS_CONTRACT_KEYS
+        assert b == 0 or (max(64, int(key) // 8) <= b <= 1024 and b & (b - 1) == 0)
+
+
 def test_record_layouts():
     assert cac.UEDGE.itemsize == 8 and cac.EDGE.itemsize == 16
     assert cac.EDGE.fields["w"][1] == 8

@@ -151,3 +151,32 @@ def test_csr_build_rejects_bad_ids(ctx):
     assert ei.value.code == cac.CAC_EINVAL
     for b in (de, do, dc):
         b.free()
+
+
+@pytest.mark.parametrize("mode", [cac.LABEL_MIN_ID, cac.LABEL_RANK])
+def test_csr_graph_replays_match_oracle(ctx, orc, mode):
+    # cac_cc_csr_graph_dev: the captured step replayed four times per graph,
+    # with eager launches and synchronisations between the replays, on two
+    # graphs (the second call captures again); the labels buffer is overwritten
+    # before every replay, so each replay must rewrite all of it
+    rng = np.random.default_rng(40 + mode)
+    for kind in ("random-above-hubs", "many-paths"):
+        n, e = _graph(kind, rng)
+        want, wn = orc.cc(e, n, mode)
+        off, cols = _numpy_csr(e, n)
+        do, dc = ctx.alloc(off.nbytes), ctx.alloc(max(cols.nbytes, 4))
+        do.upload(off)
+        dc.upload(cols)
+        dl, de = ctx.alloc(n * 4), ctx.alloc(n * 4)
+        junk = np.full(n, 0xFFFFFFFF, np.uint32)
+        for rep in range(4):
+            _say(f"csr graph replay {kind} mode {mode} rep {rep}")
+            dl.upload(junk)
+            nc = ctx.cc_csr_graph_dev(do.ptr, dc.ptr, n, dl.ptr, mode)
+            assert nc == wn
+            assert np.array_equal(dl.download(np.uint32, n), want)
+            assert ctx.cc_csr_dev(do.ptr, dc.ptr, n, de.ptr, mode) == wn  # eager work between replays
+            ctx.synchronize()
+            assert np.array_equal(de.download(np.uint32, n), want)
+        for b in (do, dc, dl, de):
+            b.free()

@@ -95,3 +95,36 @@ def test_ks_varying_trial_cuts(ctx, orc, n):
     assert raw.tolist() == want
     raw2, _ = ctx.ks_dense(A, 3, 0, T)  # same stream, same cuts: no run-to-run drift
     assert raw2.tolist() == want
+
+
+@pytest.mark.parametrize("table", [{512: 1024, 1024: 1024, 2048: 1024, 4096: 1024},
+                                   {512: 128, 1024: 512, 2048: 256, 4096: 1024}])
+def test_ks_tuned_block_sizes_bit_exact(orc, table):
+    # the contraction block size of each row-size class (cac_tune_set) changes
+    # the launch shape only: weighted cycles, whose raw cuts differ per trial,
+    # give the oracle's cuts under the largest blocks and a mixed table
+    n = 1500
+    rng = np.random.default_rng(77)
+    e = np.array([[i, (i + 1) % n, int(rng.integers(10**6, 2 * 10**6))] for i in range(n)], dtype=np.int64)
+    A = dense(n, e)
+    T = 6
+    _say(f"ks tuned blocks {table} n {n}")
+    with cac.Context(0, tuning=False) as c:
+        for key, b in table.items():
+            c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, key, b)
+        assert {k: c.tune_get(cac.TUNE_KS_CONTRACT_BLOCK, k) for k in table} == table
+        raw, _ = c.ks_dense(A, 11, 0, T)
+    want = [orc.ks_trial(A, 11, t) for t in range(T)]
+    assert raw.tolist() == want
+
+
+def test_ks_tune_rejects_bad_values():
+    with cac.Context(0, tuning=False) as c:
+        for key, b in [(1000, 128), (512, 32), (1024, 64), (2048, 300), (4096, 2048), (8192, 1024)]:
+            with pytest.raises(cac.CacError):
+                c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, key, b)
+        with pytest.raises(cac.CacError):
+            c.tune_set(99, 512, 64)
+        c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, 512, 256)
+        c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, 512, 0)
+        assert c.tune_get(cac.TUNE_KS_CONTRACT_BLOCK, 512) == 0
