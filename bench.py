@@ -114,36 +114,45 @@ def time_kernels(ctx, step, reps: int):
 
 
 def cpu_baseline(edges_host: np.ndarray, n: int, budget_s: float):
-    """The oracle's single-thread union-find on the same edge list.
+    """The oracle's union-find on the same edge list, on the host's cores.
 
-    Runs the whole edge list when one pass fits the budget (repeating it until
-    about a third of the budget is spent), otherwise the longest prefix that
-    fits, sized from a 2^22-edge probe.
+    The headline figure is the multithreaded restatement (orc_cc_mt: one edge
+    chunk per thread, CAS hooking) on min(16, cores) threads -- the CPU share of
+    one GPU on the box -- over the whole edge list, repeated until about a third
+    of the budget is spent.  The single-thread union-find is timed beside it on
+    the longest prefix that fits a third of the budget.
     """
     import oracle  # the CPU restatement: checker and baseline only
 
     m = edges_host.shape[0]
+    threads = max(1, min(16, os.cpu_count() or 1))
+    reps, total = 0, 0.0
+    while True:
+        t0 = time.perf_counter()
+        _, nc_mt = oracle.cc_mt(edges_host, n, 0, threads)
+        total += time.perf_counter() - t0
+        reps += 1
+        if total >= budget_s / 3 or total * (reps + 1) / reps > budget_s / 2:
+            break
     probe = min(m, 1 << 22)
     t0 = time.perf_counter()
     oracle.cc(edges_host[:probe], n)
     est = (time.perf_counter() - t0) * m / probe
-    sample = m if est <= budget_s else max(probe, int(m * budget_s / est))
-    reps, total = 0, 0.0
-    while True:
-        t0 = time.perf_counter()
-        oracle.cc(edges_host[:sample], n)
-        total += time.perf_counter() - t0
-        reps += 1
-        if sample < m or total >= budget_s / 3 or total * (reps + 1) / reps > budget_s:
-            break
+    sample = m if est <= budget_s / 3 else max(probe, int(m * budget_s / 3 / est))
+    t0 = time.perf_counter()
+    oracle.cc(edges_host[:sample], n)
+    t1 = time.perf_counter() - t0
     what = "the whole" if sample == m else f"the first {sample} edges of the"
     return {
-        "value": sample * reps / total / 1e9,
+        "value": m * reps / total / 1e9,
         "unit": "GTEPS",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{what} {m}-edge R-MAT edge list (sorted by (u,v)), {reps} pass(es) of "
-                  f"single-thread union-find with min-id canonicalisation, {total:.1f} s",
+        "sample": f"the whole {m}-edge R-MAT edge list (sorted by (u,v)), {reps} pass(es) of the oracle's "
+                  f"union-find on {threads} threads (CAS hooking, min-id roots), {total:.1f} s; "
+                  f"{nc_mt} components",
+        "single_thread": {"value": sample / t1 / 1e9, "unit": "GTEPS", "cores": 1,
+                          "sample": f"{what} {m}-edge list, one pass, {t1:.1f} s"},
     }
 
 

@@ -108,6 +108,102 @@ uint32_t orc_cc(uint32_t n, uint64_t m, const uint32_t* edges, int mode, uint32_
 }
 
 /* ------------------------------------------------------------------------- */
+/* Connected components on several threads (the multithreaded CPU baseline):   */
+/* the edge list is cut into one contiguous chunk per thread; each thread      */
+/* unions its edges with CAS hooking of the larger root under the smaller      */
+/* (path halving on every find), so every root is the minimum id of its tree.  */
+/* After the join, labels = roots on all threads, then the dense rank (mode 1) */
+/* in one sequential pass.  Same output as orc_cc.  The reference spreads the  */
+/* same edge-list union-find over MPI ranks (src/executables/parallel_cc.cpp,  */
+/* src/UnweightedIteratedSparseSampling.cpp:40-72); here threads share memory.  */
+/* ------------------------------------------------------------------------- */
+#include <pthread.h>
+#include <unistd.h>
+
+typedef struct {
+    uint32_t* p;
+    const uint32_t* edges;
+    uint64_t lo, hi; /* edges [lo, hi) or vertices [lo, hi) */
+    uint32_t* labels;
+} mt_job;
+
+static uint32_t mt_find(uint32_t* p, uint32_t x) {
+    for (;;) {
+        uint32_t px = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+        if (px == x) return x;
+        uint32_t gx = __atomic_load_n(&p[px], __ATOMIC_RELAXED);
+        if (gx != px) __atomic_store_n(&p[x], gx, __ATOMIC_RELAXED); /* an ancestor: valid whatever races */
+        x = gx;
+    }
+}
+
+static void* mt_init(void* arg) {
+    mt_job* j = (mt_job*)arg;
+    for (uint64_t v = j->lo; v < j->hi; ++v) j->p[v] = (uint32_t)v;
+    return NULL;
+}
+
+static void* mt_union(void* arg) {
+    mt_job* j = (mt_job*)arg;
+    for (uint64_t e = j->lo; e < j->hi; ++e) {
+        uint32_t a = j->edges[2 * e], b = j->edges[2 * e + 1];
+        for (;;) {
+            a = mt_find(j->p, a);
+            b = mt_find(j->p, b);
+            if (a == b) break;
+            uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            uint32_t expect = hi;
+            if (__atomic_compare_exchange_n(&j->p[hi], &expect, lo, 0, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) break;
+            a = expect; /* hi was hooked meanwhile: climb from its new parent */
+            b = lo;
+        }
+    }
+    return NULL;
+}
+
+static void* mt_label(void* arg) {
+    mt_job* j = (mt_job*)arg;
+    for (uint64_t v = j->lo; v < j->hi; ++v) j->labels[v] = mt_find(j->p, (uint32_t)v);
+    return NULL;
+}
+
+static void mt_run(void* (*fn)(void*), mt_job* jobs, int T, uint64_t total, uint32_t* p, const uint32_t* edges,
+                   uint32_t* labels) {
+    pthread_t th[256];
+    for (int t = 0; t < T; ++t) {
+        jobs[t].p = p;
+        jobs[t].edges = edges;
+        jobs[t].labels = labels;
+        jobs[t].lo = total * (uint64_t)t / (uint64_t)T;
+        jobs[t].hi = total * (uint64_t)(t + 1) / (uint64_t)T;
+        pthread_create(&th[t], NULL, fn, &jobs[t]);
+    }
+    for (int t = 0; t < T; ++t) pthread_join(th[t], NULL);
+}
+
+uint32_t orc_cc_mt(uint32_t n, uint64_t m, const uint32_t* edges, int mode, uint32_t* labels, int threads) {
+    int T = threads > 0 ? threads : (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (T < 1) T = 1;
+    if (T > 256) T = 256;
+    uint32_t* p = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+    mt_job jobs[256];
+    mt_run(mt_init, jobs, T, n, p, edges, labels);
+    mt_run(mt_union, jobs, T, m, p, edges, labels);
+    mt_run(mt_label, jobs, T, n, p, edges, labels);
+    uint32_t ncomp = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (labels[v] == v) {
+            p[v] = ncomp; /* p now maps a root to its rank */
+            ++ncomp;
+        }
+    }
+    if (mode == 1)
+        for (uint32_t v = 0; v < n; ++v) labels[v] = p[labels[v]];
+    free(p);
+    return ncomp;
+}
+
+/* ------------------------------------------------------------------------- */
 /* R-MAT generator                                                            */
 /* ------------------------------------------------------------------------- */
 #define DOM_RMAT 0x524D4154u /* 'RMAT' */

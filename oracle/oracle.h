@@ -39,6 +39,9 @@ uint64_t orc_uniform1(uint64_t x, uint64_t W);
  * reference's relabel loop at IteratedSparseSampling.hpp:210-220 yields).
  * Returns the number of components.  edges: m pairs (u, v). */
 uint32_t orc_cc(uint32_t n, uint64_t m, const uint32_t* edges, int mode, uint32_t* labels);
+/* The same labels and count on `threads` threads (0 = every online core): the
+ * multithreaded CPU baseline of the connected-components benchmark. */
+uint32_t orc_cc_mt(uint32_t n, uint64_t m, const uint32_t* edges, int mode, uint32_t* labels, int threads);
 
 /* R-MAT generator (spec of the synthetic input): draw d gives 'scale' quadrant
  * choices from philox4x32(ctr={d, level/4, 0, 0}, key={seed, 'RMAT'}), word

@@ -25,6 +25,7 @@ SIGS = {
     "orc_philox4x64_10": (None, [_vp, _vp, _vp]),
     "orc_uniform1": (_u64, [_u64, _u64]),
     "orc_cc": (_u32, [_u32, _u64, _vp, _i32, _vp]),
+    "orc_cc_mt": (_u32, [_u32, _u64, _vp, _i32, _vp, _i32]),
     "orc_rmat": (_u64, [_u32, _u64, _u32, _u32, _u32, _u32, _vp]),
     "orc_stoer_wagner": (_i64, [_u32, _vp]),
     "orc_brute_mincut": (_i64, [_u32, _vp]),
@@ -96,6 +97,14 @@ def cc(edges: np.ndarray, n: int, mode: int = 0):
     e = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1)
     labels = np.empty(n, dtype=np.uint32)
     nc = lib().orc_cc(n, e.size // 2, _p(e), mode, _p(labels))
+    return labels, int(nc)
+
+
+def cc_mt(edges: np.ndarray, n: int, mode: int = 0, threads: int = 0):
+    """cc() on `threads` threads (0: every online core); the same output."""
+    e = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1)
+    labels = np.empty(n, dtype=np.uint32)
+    nc = lib().orc_cc_mt(n, e.size // 2, _p(e), mode, _p(labels), threads)
     return labels, int(nc)
 
 

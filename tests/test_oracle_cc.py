@@ -129,3 +129,20 @@ def test_rmat_oracle_matches_direct(orc):
     e, d = orc.rmat(8, 600, 3, th)
     want, dw = _rmat_direct(orc, 8, 600, 3, th)
     assert d == dw and np.array_equal(e, want)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_cc_multithreaded_equals_sequential(orc, threads, seed):
+    # the multithreaded CPU baseline gives orc_cc's labels and count in both
+    # label forms: many small components, one giant, loops and repeats
+    rng = np.random.default_rng(300 + seed)
+    n = 40000
+    e = np.concatenate([rng.integers(0, n, size=(30000, 2)),
+                        rng.integers(0, n // 8, size=(60000, 2)),
+                        np.repeat(np.arange(50)[:, None], 2, axis=1)]).astype(np.uint32)
+    for mode in (0, 1):
+        want, wn = orc.cc(e, n, mode)
+        got, gn = orc.cc_mt(e, n, mode, threads)
+        assert gn == wn
+        assert np.array_equal(got, want)
