@@ -334,13 +334,19 @@ __device__ inline const int64_t* ks_row(const KsChild& c, const KsPool& pool, ui
   return slot == kNoSlot ? c.parent + (size_t)r * c.s : pool.base + (size_t)slot * kKsUnit;
 }
 
-// This thread's entries x = k * B + t of a row (0 past s).
+// This thread's entries x = k * B + t of a row (0 past s).  The row pointer is
+// generic (it comes from a struct in memory); as a global-address-space
+// pointer the loads are global loads, which unlike flat loads do not count
+// against the LDS wait counter, so the block's LDS waits between picks do not
+// also wait for row loads in flight.
 template <int B>
 __device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int64_t (&v)[kKsChunk]) {
+  typedef const __attribute__((address_space(1))) int64_t g_i64;
+  g_i64* g = (g_i64*)row;
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
     const uint32_t x = k * B + threadIdx.x;
-    v[k] = (k < cc && x < s) ? row[x] : 0;
+    v[k] = (k < cc && x < s) ? g[x] : 0;
   }
 }
 
@@ -545,6 +551,8 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
 
 // Rows per compaction workgroup for a child of t' output vertices (the LDS
 // accumulator holds rows x t' int64, at most 32 KiB unless t' alone is larger).
+constexpr uint32_t kCompactUnroll = 8;  // row elements per thread in flight in k_ks_compact
+
 __host__ __device__ inline uint32_t compact_rows(uint32_t tgt) {
   const uint32_t r = 4096 / (tgt ? tgt : 1);
   return r < 1 ? 1 : (r > 64 ? 64 : r);
@@ -575,14 +583,34 @@ __global__ void __launch_bounds__(256) k_ks_compact(const KsChild* __restrict__ 
   const uint32_t s = c.s;
   for (uint32_t x = threadIdx.x; x < nr * kk; x += 256) acc[x] = 0ull;
   __syncthreads();
+  // Rows and the column map are global memory (the struct's pointers are
+  // generic, which would give flat loads); kCompactUnroll elements per thread
+  // are loaded before any is added, so a wavefront keeps several 512-byte
+  // loads in flight instead of one.
+  typedef const __attribute__((address_space(1))) int64_t g_i64;
+  typedef const __attribute__((address_space(1))) uint32_t g_u32;
+  constexpr uint32_t U = kCompactUnroll;
+  g_u32* cm = (g_u32*)c.colmap;
   for (uint32_t rr = 0; rr < nr; ++rr) {
     const uint32_t e = c.live[r0 + rr];
-    const int64_t* src =
-        (e & kLiveWork) ? pool.base + (size_t)(e & ~kLiveWork) * kKsUnit : c.parent + (size_t)e * s;
+    g_i64* src = (g_i64*)((e & kLiveWork) ? pool.base + (size_t)(e & ~kLiveWork) * kKsUnit : c.parent + (size_t)e * s);
     unsigned long long* a = acc + (size_t)rr * kk;
-    for (uint32_t y = threadIdx.x; y < s; y += 256) {
+    uint32_t y = threadIdx.x;
+    for (; y + (U - 1) * 256 < s; y += U * 256) {
+      int64_t v[U];
+      uint32_t m[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        v[u] = src[y + u * 256];
+        m[u] = cm[y + u * 256];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u)
+        if (v[u]) atomicAdd(&a[m[u]], (unsigned long long)v[u]);
+    }
+    for (; y < s; y += 256) {
       const int64_t v = src[y];
-      if (v) atomicAdd(&a[c.colmap[y]], (unsigned long long)v);
+      if (v) atomicAdd(&a[cm[y]], (unsigned long long)v);
     }
   }
   __syncthreads();
