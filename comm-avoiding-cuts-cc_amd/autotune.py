@@ -33,8 +33,12 @@ from pathlib import Path
 sys.path.insert(0, str(Path(__file__).resolve().parent))
 import cac  # noqa: E402
 
-# (class key, trials per timed solve): about 0.05-0.2 s per solve on one MI355X
-CLASSES = ((4096, 32), (2048, 64), (1024, 128), (512, 256))
+# (class key, trials per timed solve).  Enough trials that thousands of children
+# of each class run at once, as at the deep levels of a large solve (169 trials
+# at n = 8192 run 2^10 times more children of 260 vertices than the root has):
+# with only hundreds, larger blocks look as fast, though they cut occupancy
+# where it matters.  About 0.2-1 s per solve on one MI355X.
+CLASSES = ((4096, 128), (2048, 512), (1024, 2048), (512, 4096))
 BLOCKS = (64, 128, 256, 512, 1024)
 
 
