@@ -355,8 +355,8 @@ __global__ void __launch_bounds__(kCcBlock) k_check_ids(const cac_uedge* e, uint
 constexpr uint32_t kAfSkip = 1;
 constexpr uint32_t kAfHeavy = 1024;  // a vertex with more remaining neighbours goes to a workgroup
 // Successful hooks are counted per wavefront and added to one of kHookSlots
-// counters 64 bytes apart, so no single word takes an atomic from every wave.  Components = n - hooks: each hook
-// joins two trees into one.
+// counters 64 bytes apart, so no single word takes an atomic from every wave.
+// Components = n - hooks: each hook joins two trees into one.
 constexpr uint32_t kHookSlots = 64, kHookStride = 16;
 // CSR scratch head: [0] components, [1] sampled root, [2] heavy-queue length,
 // then the hook slots.
