@@ -8,6 +8,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <thread>
@@ -30,6 +32,81 @@ inline bool next_u64(const char*& p, const char* end, uint64_t* v) {
   }
   *v = x;
   return true;
+}
+
+// Optional binary cache of a parsed edge list, beside the text file as
+// <path>.cacbin: this 64-byte header, then m cac_edge records.  It is used only
+// while the text file keeps the size and modification time recorded here.
+struct CacheHeader {
+  char magic[8];  // "CACEDGE1"
+  uint32_t n, pad;
+  uint64_t m, text_size;
+  int64_t text_mtime_ns;
+  uint64_t reserved[3];
+};
+static_assert(sizeof(CacheHeader) == 64, "cache header layout");
+constexpr char kCacheMagic[8] = {'C', 'A', 'C', 'E', 'D', 'G', 'E', '1'};
+
+bool cache_enabled() {
+  const char* e = std::getenv("CAC_EDGE_CACHE");
+  return e && *e && std::strcmp(e, "0") != 0;
+}
+
+int64_t mtime_ns(const struct stat& st) { return (int64_t)st.st_mtim.tv_sec * 1000000000 + st.st_mtim.tv_nsec; }
+
+bool read_all(int fd, void* buf, size_t bytes) {
+  char* p = (char*)buf;
+  while (bytes) {
+    const ssize_t r = ::read(fd, p, bytes);
+    if (r <= 0) return false;
+    p += r;
+    bytes -= (size_t)r;
+  }
+  return true;
+}
+
+bool write_all(int fd, const void* buf, size_t bytes) {
+  const char* p = (const char*)buf;
+  while (bytes) {
+    const ssize_t r = ::write(fd, p, bytes);
+    if (r <= 0) return false;
+    p += r;
+    bytes -= (size_t)r;
+  }
+  return true;
+}
+
+bool read_cache(const std::string& path, const struct stat& text, uint32_t* n, std::vector<cac_edge>* edges) {
+  const int fd = ::open((path + ".cacbin").c_str(), O_RDONLY);
+  if (fd < 0) return false;
+  CacheHeader h;
+  struct stat st;
+  bool ok = fstat(fd, &st) == 0 && read_all(fd, &h, sizeof h) && std::memcmp(h.magic, kCacheMagic, 8) == 0 &&
+            h.text_size == (uint64_t)text.st_size && h.text_mtime_ns == mtime_ns(text) &&
+            (uint64_t)st.st_size == sizeof h + h.m * sizeof(cac_edge);
+  if (ok) {
+    edges->resize(h.m);
+    ok = read_all(fd, edges->data(), h.m * sizeof(cac_edge));
+    if (ok) *n = h.n;
+  }
+  ::close(fd);
+  return ok;
+}
+
+// Best effort: written to a private temporary name, then renamed into place.
+void write_cache(const std::string& path, const struct stat& text, uint32_t n, const std::vector<cac_edge>& edges) {
+  CacheHeader h{};
+  std::memcpy(h.magic, kCacheMagic, 8);
+  h.n = n;
+  h.m = edges.size();
+  h.text_size = (uint64_t)text.st_size;
+  h.text_mtime_ns = mtime_ns(text);
+  const std::string tmp = path + ".cacbin.tmp" + std::to_string((long)getpid());
+  const int fd = ::open(tmp.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (fd < 0) return;
+  const bool ok = write_all(fd, &h, sizeof h) && write_all(fd, edges.data(), edges.size() * sizeof(cac_edge));
+  ::close(fd);
+  if (!ok || std::rename(tmp.c_str(), (path + ".cacbin").c_str()) != 0) ::unlink(tmp.c_str());
 }
 
 }  // namespace
@@ -67,6 +144,11 @@ int read_edge_list(const std::string& path, uint32_t* n_out, std::vector<cac_edg
     ::close(fd);
     *err = "cannot read " + path;
     return CAC_EIO;
+  }
+  const bool cache = cache_enabled();
+  if (cache && read_cache(path, st, n_out, edges)) {
+    ::close(fd);
+    return CAC_OK;
   }
   const size_t size = (size_t)st.st_size;
   const char* base = (const char*)mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
@@ -142,6 +224,7 @@ int read_edge_list(const std::string& path, uint32_t* n_out, std::vector<cac_edg
           edges->push_back(x);
         }
       *n_out = (uint32_t)n;
+      if (cache) write_cache(path, st, *n_out, *edges);
     }
   }
   munmap((void*)base, size);

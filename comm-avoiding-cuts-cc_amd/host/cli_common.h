@@ -13,7 +13,11 @@ namespace cli {
 // Reads the reference edge-list format (src/input/GraphInputIterator.cpp:3-36):
 // the first line is ignored, then "n m", then m lines "from to weight".  The file
 // is memory-mapped and parsed in parallel chunks.  Returns CAC_EIO with a message
-// on a missing file or malformed content.
+// on a missing file or malformed content.  With CAC_EDGE_CACHE set (and not "0")
+// in the environment, the parsed list is also written beside the file as
+// <path>.cacbin (a 64-byte header and the cac_edge records), and later reads
+// take it instead of parsing while the text file keeps its size and
+// modification time; the text format stays the canonical input.
 int read_edge_list(const std::string& path, uint32_t* n, std::vector<cac_edge>* edges, std::string* err);
 
 // In-memory CLICK graph (src/input/CLICK.hpp:10-49): complete graph on n vertices,

@@ -123,3 +123,44 @@ def test_karger_stein_sparse_ws_matches_oracle(tmp_path, orc):
     T = orc.ks_trials(n, 0.90)
     assert int(out[3]) == n and int(out[4]) == len(E)
     assert int(out[8]) == min(orc.ks_sparse_trial(n, E, 6, t) for t in range(T))
+
+
+def test_edge_list_cache(tmp_path):
+    # CAC_EDGE_CACHE: the first run writes <file>.cacbin holding the parsed
+    # edges; a later run reads it while the text keeps its size and mtime,
+    # shown by garbling the text in place (same size, mtime restored): with the
+    # cache the garbage is not seen, without it the parse fails.  The file is
+    # read before any GPU call, so this runs on the CPU (where the tool then
+    # stops for want of a GPU).
+    import os
+    import shutil
+
+    src = tmp_path / "g.graph"
+    shutil.copy(GOLD / "cc_13.graph", src)
+    env = dict(os.environ, CAC_EDGE_CACHE="1")
+
+    def tool(use_cache):
+        return subprocess.run([str(BIN / "parallel_cc"), str(src), "0"], capture_output=True, text=True, timeout=300,
+                              env=env if use_cache else {k: v for k, v in os.environ.items() if k != "CAC_EDGE_CACHE"})
+
+    tool(True)
+    cache = tmp_path / "g.graph.cacbin"
+    raw = cache.read_bytes()
+    assert raw[:8] == b"CACEDGE1"
+    n, m = int.from_bytes(raw[8:12], "little"), int.from_bytes(raw[16:24], "little")
+    lines = src.read_text().splitlines()
+    want_n, want_m = (int(x) for x in lines[1].split())
+    want = np.array([[int(x) for x in ln.split()] for ln in lines[2:2 + want_m]], dtype=np.uint64)
+    rec = np.frombuffer(raw[64:], dtype=np.dtype([("u", "<u4"), ("v", "<u4"), ("w", "<u8")]))
+    assert (n, m) == (want_n, want_m) and len(rec) == m
+    assert np.array_equal(rec["u"], want[:, 0]) and np.array_equal(rec["v"], want[:, 1])
+    assert np.array_equal(rec["w"], want[:, 2])
+    st = src.stat()
+    text = src.read_bytes()
+    body = text.index(b"\n", text.index(b"\n") + 1) + 1
+    garbled = text[:body] + text[body:].replace(b"1", b"x", 1)
+    assert len(garbled) == len(text)
+    src.write_bytes(garbled)
+    os.utime(src, ns=(st.st_atime_ns, st.st_mtime_ns))
+    assert "malformed" not in tool(True).stderr
+    assert "malformed" in tool(False).stderr
