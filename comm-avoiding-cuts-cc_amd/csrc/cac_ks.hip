@@ -251,7 +251,15 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     const uint32_t prev = u < x ? u : x, last = u < x ? x : u;
     leaf_merge(A, V, prev, last, alive);
     --nalive;
-    const uint32_t again = first_col(prev);  // the same on every lane; kept by prev's owner
+    // row prev again, scanned by the whole wavefront: lane l tests columns l
+    // and l + 64, and the first set bit of the ballots is the first column
+    uint32_t again;
+    {
+      const bool q0 = alive[0] && (uint32_t)lane != prev && A[prev * kLeafMax + lane] >= lf.bound;
+      const bool q1 = alive[1] && (uint32_t)lane + 64 != prev && A[prev * kLeafMax + lane + 64] >= lf.bound;
+      const uint64_t b0 = __ballot(q0), b1 = __ballot(q1);
+      again = b0 ? (uint32_t)(__ffsll((long long)b0) - 1) : b1 ? 64u + (uint32_t)(__ffsll((long long)b1) - 1) : kNoSlot;
+    }
     for (int h = 0; h < 2; ++h) {
       const uint32_t r = lane + 64 * h;
       if (!alive[h]) col[h] = kNoSlot;
