@@ -369,8 +369,13 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
                          const int64_t (&v)[kKsChunk], uint32_t cc, int64_t& w) {
   const uint32_t s = c.s;
   const uint32_t other = held == a ? b : a;
+  // The other row's address is fixed before the slots change, so its load is
+  // issued first and lands while the slots and representatives are updated
+  // (the barriers below wait for LDS operations only).
+  int64_t u[kKsChunk];
+  load_row<B>(ks_row(c, pool, L.wslot[other], other), cc, s, u);
+  if (B > 64) __syncthreads();  // every wave has read the slot before thread 0 changes it
   if (threadIdx.x == 0) {
-    L.misc[2] = L.wslot[other];
     uint32_t db = L.wslot[b];
     uint32_t overflow = 0;
     if (db == kNoSlot && L.wslot[a] != kNoSlot) {
@@ -392,14 +397,11 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
   }
   __syncthreads();  // every thread has read rep[k] and the slots before they change
   if (L.misc[4]) return false;
-  const uint32_t so = L.misc[2], db = L.misc[3];
+  const uint32_t db = L.misc[3];
   for (uint32_t x = threadIdx.x; x < s; x += B)
     if (L.rep[x] == a) L.rep[x] = b;
   __syncthreads();
-  const int64_t* ro = ks_row(c, pool, so, other);
   int64_t* dst = pool.base + (size_t)db * kKsUnit;
-  int64_t u[kKsChunk];
-  load_row<B>(ro, cc, s, u);
   uint64_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
