@@ -393,30 +393,34 @@ __global__ void __launch_bounds__(kCcBlock) k_af_init(uint32_t* __restrict__ par
 // by a Boyer-Moore majority vote: per lane over its samples, then across the
 // wavefront (pairwise merge; ties keep the smaller id, so both partners of a
 // butterfly step agree).  Only a performance choice: labels are correct
-// whatever root is returned.  One wavefront; all sample loads are issued first.
+// whatever root is returned.  One wavefront; the samples' loads are issued
+// four at a time (this runs inside the bulk pass, whose register budget, and
+// so occupancy, it must not raise).
 constexpr int kAfSamples = 1024;
 __device__ inline void af_majority_root(uint32_t* par, uint32_t range, uint32_t* out) {
-  constexpr int kPer = kAfSamples / 64;
+  constexpr int kPer = kAfSamples / 64, kBatch = 4;
   const int lane = threadIdx.x & 63;
-  uint32_t x[kPer], p[kPer];
-#pragma unroll
-  for (int t = 0; t < kPer; ++t) {
-    uint32_t h = (uint32_t)(t * 64 + lane) * 0x9E3779B1u;  // scatter the sample positions
-    h ^= h >> 15;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    x[t] = (uint32_t)(((uint64_t)h * range) >> 32);
-    p[t] = ld_par_fresh(par, x[t]);
-  }
   uint32_t cand = ~0u, cnt = 0;
+  for (int t0 = 0; t0 < kPer; t0 += kBatch) {
+    uint32_t x[kBatch], p[kBatch];
 #pragma unroll
-  for (int t = 0; t < kPer; ++t) {
-    const uint32_t r = p[t] == x[t] ? x[t] : uf_find_fresh(par, p[t]);
-    if (cnt == 0) {
-      cand = r;
-      cnt = 1;
-    } else {
-      cnt = cand == r ? cnt + 1 : cnt - 1;
+    for (int t = 0; t < kBatch; ++t) {
+      uint32_t h = (uint32_t)((t0 + t) * 64 + lane) * 0x9E3779B1u;  // scatter the sample positions
+      h ^= h >> 15;
+      h *= 0x85EBCA6Bu;
+      h ^= h >> 13;
+      x[t] = (uint32_t)(((uint64_t)h * range) >> 32);
+      p[t] = ld_par_fresh(par, x[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < kBatch; ++t) {
+      const uint32_t r = p[t] == x[t] ? x[t] : uf_find_fresh(par, p[t]);
+      if (cnt == 0) {
+        cand = r;
+        cnt = 1;
+      } else {
+        cnt = cand == r ? cnt + 1 : cnt - 1;
+      }
     }
   }
   for (int d = 1; d < 64; d <<= 1) {
