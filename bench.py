@@ -30,7 +30,8 @@ the metric, each on its own workload:
       its published times.
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
-CPU baseline (the oracle's single-thread union-find on the same edge list).
+CPU baseline (the oracle's union-find on the same edge list on min(16, cores)
+threads, with its single-thread figure beside it).
 """
 from __future__ import annotations
 
