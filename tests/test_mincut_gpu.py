@@ -37,11 +37,14 @@ def test_sample_contract_matches_oracle(ctx, orc, n, t, seed):
     assert np.array_equal(merged, want)
 
 
-@pytest.mark.parametrize("kind", ["one-weight", "one-weight-loop", "one-heavier", "one-weight-large"])
+@pytest.mark.parametrize("kind", ["one-weight", "one-weight-loop", "one-heavier", "one-weight-large", "skewed"])
 def test_sampling_first_round_forms(ctx, orc, kind):
     """A graph whose edges all weigh the same and hold no loop takes the first
     round's draws by division instead of the prefix search; a loop or a second
-    weight keeps the search.  Both must give the oracle's labels and cuts."""
+    weight keeps the search.  Both must give the oracle's labels and cuts.
+    "skewed" puts nearly all the weight on a few edges amid weight-1 edges (and
+    an edge count that is no multiple of 8), so most draws land on a few
+    edges of the prefix search."""
     rng = np.random.default_rng(len(kind))
     n = 400
     e = erdos_renyi(rng, n, 10.0 / n)
@@ -50,6 +53,10 @@ def test_sampling_first_round_forms(ctx, orc, kind):
         e = np.concatenate([e, np.array([[5, 5, 7]], dtype=np.int64)])
     if kind == "one-heavier":
         e[len(e) // 2, 2] = 8
+    if kind == "skewed":
+        e = e[: len(e) - (len(e) % 8) - 3]
+        e[:, 2] = 1
+        e[rng.choice(len(e), 9, replace=False), 2] = 1_000_000_000
     t = 60
     _say(f"first-round forms {kind}: n {n} m {len(e)} t {t}")
     vmap, k, _ = ctx.sample_contract(e, n, t, 3, 2)
