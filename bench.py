@@ -27,7 +27,9 @@ the metric, each on its own workload:
       published 1-core and 1440-core times;
   rm  square_root and karger_stein on R-MAT graphs of the reference's
       rmat_16k_{1024,4096} shape (2^14 vertices, 16.4M / 65.5M edges), beside
-      its published times.
+      its published times;
+  ap  approx_cut (p = 0.95) on the two Watts-Strogatz graphs (the reference
+      publishes no approx_cut time), beside the oracle on one core.
 
 Prints ONE JSON line (rank 0) with the roofline of the dominant kernel and a
 CPU baseline (the oracle's union-find on the same edge list on min(16, cores)
@@ -64,8 +66,8 @@ def parse():
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (PMC byte-counter scaling)")
-    ap.add_argument("--extras", default="c1,c3,c4,c5,ks,sq,rm",
-                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks,sq,rm) or 'none'")
+    ap.add_argument("--extras", default="c1,c3,c4,c5,ks,sq,rm,ap",
+                    help="comma list of min-cut / serving workloads to add (c1,c3,c4,c5,ks,sq,rm,ap) or 'none'")
     ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
     ap.add_argument("--c4-graphs", type=int, default=4096)
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
@@ -310,6 +312,38 @@ def bench_sq(ctx, cac, sizes, cpu_budget_s: float):
             r["cpu_baseline"] = {"value": 1.0 / per_trial, "unit": "trials/s", "cores": 1, "kind": "port",
                                  "sample": f"{k} of the {T} trials timed single-threaded ({per_trial:.2f} s each); "
                                            f"x{T} trials = {per_trial * T:.0f} s"}
+        out[f"ws_{n // 1000}k_16"] = r
+    return out
+
+
+def bench_ap(ctx, cac, sizes, cpu_budget_s: float):
+    """approx_cut (binomial subsampling, CC over five disjoint copies per level,
+    p = 0.95) on the Watts-Strogatz graphs of the sq leg.  The edge list is
+    uploaded inside each call, as the approx_cut command line does."""
+    out = {}
+    for n in sizes:
+        e = cac.gen_ws(n, 16, 0)
+        m = len(e)
+        ctx.approx_cut(e, n, F32_95, 1)  # warm-up: same graph, another seed
+        ctx.synchronize()
+        reps = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            a, T = ctx.approx_cut(e, n, F32_95, 0)
+            reps.append(time.perf_counter() - t0)
+        dt = sorted(reps)[len(reps) // 2]
+        r = {"workload": f"approx_cut p=0.95 seed 0, Watts-Strogatz n={n}, 16 ring neighbours a side, beta 0.3, "
+                         f"weight 100, m={m}",
+             "seconds": dt, "seconds_of_5": reps, "solves_per_s": 1.0 / dt, "trials": T, "approximation": a}
+        if cpu_budget_s > 0 and n == min(sizes):
+            import oracle  # the CPU restatement: baseline and check
+            e3 = np.stack([e["u"], e["v"], e["w"]], axis=1).astype(np.uint64)
+            t0 = time.perf_counter()
+            want = oracle.approx_cut(n, e3, F32_95, 0)
+            cpu_s = time.perf_counter() - t0
+            r["equals_oracle"] = want == (a, T)
+            r["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                                 "sample": f"one whole solve on one core ({cpu_s:.2f} s)"}
         out[f"ws_{n // 1000}k_16"] = r
     return out
 
@@ -589,7 +623,9 @@ def main():
                                                     min(args.cpu_baseline_seconds, 20.0))),
                     ("sq", lambda: bench_sq(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],
                                                     min(args.cpu_baseline_seconds, 15.0))),
-                    ("rm", lambda: bench_rm(ctx, cac, [int(x) for x in args.rm_degrees.split(",")]))):
+                    ("rm", lambda: bench_rm(ctx, cac, [int(x) for x in args.rm_degrees.split(",")])),
+                    ("ap", lambda: bench_ap(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],
+                                                    min(args.cpu_baseline_seconds, 10.0)))):
         if key in wanted:
             t0 = time.perf_counter()
             extras[key] = fn()
