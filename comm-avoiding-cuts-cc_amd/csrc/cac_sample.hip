@@ -8,7 +8,8 @@
 // SquareRootCut::runLowConcurrencyMaster (src/SquareRootCut.cpp:86-148).
 //
 // One workgroup runs one trial's rounds (trials run side by side):
-//   1. prefix of the edge weights with loops masked to 0 (u64, block scan);
+//   1. prefix of the edge weights with loops masked to 0 (block scan; u64,
+//      or u32 once a previous round's total fits in 32 bits);
 //   2. |V| draws: Philox(ctr = {trial, round, draw, 0}, key = {seed, 'SAMP'}),
 //      r = 1 + floor(x W / 2^64), edge = first prefix >= r (binary search);
 //   3. "contract in draw order until |V| == t" (a prefix condition): binary
@@ -168,8 +169,14 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
   if (S.mixed && m > 0 && __hip_atomic_load(S.mixed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
     wu = edges[0].w;
   uint32_t nv = n;
+  // A round's masked total never exceeds the previous round's (edges only turn
+  // into loops), so once a total fits in 32 bits the prefix is kept as u32:
+  // half the prefix bytes, the same comparisons.
+  uint64_t prev_w = ~0ull;
+  uint32_t* pre32 = reinterpret_cast<uint32_t*>(S.pre);
   for (uint32_t round = 0; nv > t; ++round) {
     const bool flat = round == 0 && wu != 0;
+    const bool narrow = prev_w <= 0xffffffffull;
     // 1. masked inclusive prefix of the weights
     uint64_t carry = flat ? wu * m : 0;
     constexpr int kItems = 4;
@@ -208,13 +215,17 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
       for (int k = 0; k < kItems; ++k) {
         const uint64_t e = e0 + k;
         run += w[k];
-        if (e < m) S.pre[e] = run;
+        if (e < m) {
+          if (narrow) pre32[e] = (uint32_t)run;
+          else S.pre[e] = run;
+        }
       }
       carry += tot;
       __syncthreads();
     }
     block_sync_global();  // the draws read the prefix written by every wave
     const uint64_t W = carry;
+    prev_w = W;
     if (W == 0) break;  // only loops left: more than t components
     // 2. draws
     const uint32_t D = nv;
@@ -224,7 +235,7 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
       uint64_t lo = flat ? (r - 1) / wu : 0, hi = flat ? lo : m - 1;
       while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
-        if (S.pre[mid] >= r) hi = mid;
+        if ((narrow ? (uint64_t)pre32[mid] : S.pre[mid]) >= r) hi = mid;
         else lo = mid + 1;
       }
       const cac_edge x = edges[lo];
