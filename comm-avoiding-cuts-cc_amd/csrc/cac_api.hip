@@ -13,6 +13,11 @@
 namespace cac {
 namespace {
 
+// Flags bad ids (bit 0) and weights >= 2^40 (bit 1), and adds up the weights
+// as two u64 counters, the low and the high 32 bits of every wavefront's sum
+// (wsum[0], wsum[1]), so the total cannot wrap: with weights below 2^40 a
+// wavefront's sum stays below 2^58 and the counters below 2^46 and 2^40 for
+// fewer than 2^32 edges.
 __global__ void k_check_edges(const cac_edge* e, uint64_t m, uint32_t n, uint32_t* bad, unsigned long long* wsum) {
   uint32_t b = 0;
   unsigned long long s = 0;
@@ -20,11 +25,14 @@ __global__ void k_check_edges(const cac_edge* e, uint64_t m, uint32_t n, uint32_
     const cac_edge x = e[i];
     if (x.u >= n || x.v >= n) b = 1;
     if (x.w >= (1ull << 40)) b |= 2;  // single weights this large would risk int64 sums
-    s += x.w;
+    else s += x.w;
   }
   if (b) atomicOr(bad, b);
   for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(wsum, s);
+  if ((threadIdx.x & 63) == 0 && s) {
+    atomicAdd(wsum, s & 0xFFFFFFFFull);
+    atomicAdd(wsum + 1, s >> 32);
+  }
 }
 
 __global__ void k_zero_words(uint32_t* p, int count) {
@@ -77,21 +85,24 @@ int upload_edges(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_t n, ca
   }
   uint32_t* flag = (uint32_t*)((char*)d + std::max<uint64_t>(m, 1) * sizeof(cac_edge));
   unsigned long long* wsum = (unsigned long long*)(flag + 2);
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, flag, 4);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, flag, 6);
   if (m) hipLaunchKernelGGL(k_check_edges, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, d, m, n, flag, wsum);
-  uint32_t hb[4] = {0, 0, 0, 0};
-  hipError_t e = hipMemcpyAsync(hb, flag, 16, hipMemcpyDeviceToHost, s);
+  uint32_t hb[6] = {0, 0, 0, 0, 0, 0};
+  hipError_t e = hipMemcpyAsync(hb, flag, 24, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(d);
     return hip_fail(ctx, e, "edge check");
   }
-  const unsigned long long total = (unsigned long long)hb[2] | ((unsigned long long)hb[3] << 32);
+  const unsigned long long lo = (unsigned long long)hb[2] | ((unsigned long long)hb[3] << 32);
+  const unsigned long long hi = (unsigned long long)hb[4] | ((unsigned long long)hb[5] << 32);
+  // total = hi * 2^32 + lo >= 2^61  <=>  hi + lo / 2^32 >= 2^29 (neither term wraps)
+  const bool too_big = hi + (lo >> 32) >= (1ull << 29);
   if (hb[0] & 1u) {
     (void)hipFree(d);
     return fail(ctx, CAC_EINVAL, "edge endpoint >= n");
   }
-  if ((hb[0] & 2u) || total >= (1ull << 61)) {
+  if ((hb[0] & 2u) || too_big) {
     (void)hipFree(d);
     return fail(ctx, CAC_ERANGE, "edge weights too large for int64 cut accumulation");
   }

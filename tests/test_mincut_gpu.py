@@ -139,6 +139,26 @@ def test_mincut_rejects_bad_ids(ctx):
     assert ei.value.code == cac.CAC_EINVAL
 
 
+def test_weight_sum_that_wraps_64_bits_is_rejected(ctx):
+    # 2^24 + 2^20 edges of weight 2^40 - 1 weigh about 2^64 + 2^60: a plain u64
+    # sum wraps to about 2^60 (< 2^61) and would pass; the high-word counter
+    # sees the true total
+    m = (1 << 24) + (1 << 20)
+    e = np.zeros(m, dtype=cac.EDGE)
+    e["u"] = np.arange(m, dtype=np.uint64) % 1000
+    e["v"] = (e["u"] + 1) % 1000
+    e["w"] = (1 << 40) - 1
+    wrapped = (m * ((1 << 40) - 1)) % (1 << 64)
+    assert wrapped < (1 << 61)
+    _say(f"weight-sum wrap: m {m}")
+    with pytest.raises(cac.CacError) as ei:
+        ctx.mincut(e, 1000, 0.9, 0)
+    assert ei.value.code == cac.CAC_ERANGE
+    with pytest.raises(cac.CacError) as ei:
+        ctx.sample_contract(e, 1000, 100, 0, 0)
+    assert ei.value.code == cac.CAC_ERANGE
+
+
 def test_sqrt_trial_beyond_dense_engine(ctx, orc):
     # m > 16.7M edges gives t = 2 sqrt(m) + 1 > 8192: the contracted graph goes
     # through the sparse Karger-Stein as a merged edge list (bit-exact raw cut)
