@@ -28,6 +28,7 @@ CAC_ERANGE = -5
 CAC_EIO = -6
 LABEL_MIN_ID = 0
 LABEL_RANK = 1
+CC_STATIC_MAX_N = 1 << 25  # CAC_CC_STATIC_MAX_N
 
 # Edge records, identical to cac_uedge / cac_edge.
 UEDGE = np.dtype([("u", "<u4"), ("v", "<u4")])
@@ -72,6 +73,7 @@ SIGNATURES = {
     "cac_prof_get": (_i32, [_vp, _i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float)]),
     "cac_cc": (_i32, [_vp, _vp, _u64, _u32, _i32, _pu32, _pu32]),
     "cac_cc_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
+    "cac_cc_static_dev": (_i32, [_vp, _vp, _u64, _u32, _i32, _vp, _pu32, _vp]),
     "cac_cc_csr_dev": (_i32, [_vp, _vp, _vp, _u32, _i32, _vp, _pu32, _vp]),
     "cac_cc_csr_graph_dev": (_i32, [_vp, _vp, _vp, _u32, _i32, _vp, _pu32, _vp]),
     "cac_csr_build_dev": (_i32, [_vp, _vp, _u64, _u32, _vp, _vp, _pu64, _vp]),
@@ -264,6 +266,14 @@ class Context:
     def cc_dev(self, d_edges: int, m: int, n: int, d_labels: int, mode: int = LABEL_MIN_ID, stream=None) -> int:
         nc = ctypes.c_uint32()
         self._check(self.lib.cac_cc_dev(self.handle, d_edges, m, n, mode, d_labels, ctypes.byref(nc), stream))
+        return int(nc.value)
+
+    def cc_static_dev(self, d_edges: int, m: int, n: int, d_labels: int, mode: int = LABEL_MIN_ID,
+                      stream=None) -> int:
+        """cc_dev without device allocation (static scratch; n <= CC_STATIC_MAX_N)."""
+        nc = ctypes.c_uint32()
+        self._check(self.lib.cac_cc_static_dev(self.handle, d_edges, m, n, mode, d_labels, ctypes.byref(nc),
+                                               stream))
         return int(nc.value)
 
     def csr_build_dev(self, d_edges: int, m: int, n: int, d_offsets: int, d_cols: int, stream=None) -> int:

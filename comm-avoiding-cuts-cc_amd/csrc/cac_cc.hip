@@ -28,6 +28,7 @@
 // Any edge order is correct; the run sampling pays off when equal u are
 // adjacent (the order of the edge-list files and of the R-MAT generator).
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/rocprim.hpp>
 
@@ -813,6 +814,76 @@ int cc_csr_launch(cac_ctx* ctx, const uint32_t* d_off, const uint32_t* d_cols, u
   return CAC_OK;
 }
 
+// ---- connected components without allocation (cac_cc_static_dev) -----------
+// The scratch of cc_launch for up to kStaticMaxN vertices lives in one static
+// device array: the head (counters), the giant-component bitmap, and for rank
+// labels the per-word root prefix and the scan partials.  Rank labels are
+// computed from a root bitmap (which reuses the giant bitmap's words, free once
+// the hook pass has run) instead of cc_launch's n-entry rank table:
+// rank(r) = roots in the words before r's + roots below r in its word.
+constexpr uint32_t kStaticMaxN = CAC_CC_STATIC_MAX_N;
+constexpr size_t kStaticBitsWords = (size_t)kStaticMaxN / 64 * 2;  // one u64 per 64 vertices
+constexpr size_t kStaticPreWords = (size_t)kStaticMaxN / 64;       // one u32 per bitmap word
+constexpr size_t kStaticPartWords = ((size_t)kStaticMaxN / 64 + kScanTile - 1) / kScanTile + 64;
+constexpr size_t kStaticWords = kCcTmpHead + kStaticBitsWords + kStaticPreWords + kStaticPartWords;
+static_assert(kStaticMaxN % 64 == 0, "whole bitmap words");
+
+static __device__ __align__(256) uint32_t cc_static_scratch[kStaticWords];
+
+namespace {
+
+// The array's device address, one per device (each GPU has its own copy),
+// taken by hipGetSymbolAddress with that device current.
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mu;
+void* g_static_ptr[kStaticMaxDevices] = {};
+
+// bits[w] bit b = (lab[64 w + b] is a root); one ballot per 64 vertices.
+__global__ void __launch_bounds__(kCcBlock) k_cc_root_bits(const uint32_t* __restrict__ lab, uint32_t n,
+                                                           uint64_t* __restrict__ bits) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * (kCcBlock / 64);
+  for (uint64_t w = (uint64_t)blockIdx.x * (kCcBlock / 64) + (threadIdx.x >> 6); w * 64 < n; w += waves) {
+    const uint64_t v = w * 64 + lane;
+    const uint64_t mask = __ballot(v < n && lab[v] == (uint32_t)v);
+    if (lane == 0) bits[w] = mask;
+  }
+}
+
+struct WordPop {
+  const uint64_t* bits;
+  __device__ uint32_t operator()(uint64_t i) const { return (uint32_t)__popcll(bits[i]); }
+};
+struct WordPreStore {
+  uint32_t* pre;
+  __device__ void operator()(uint64_t i, uint32_t ex, uint32_t) const { pre[i] = ex; }
+};
+
+// lab[v] = rank of its root r among all roots, from the bitmap alone (lab is
+// read and written only at v, so the pass has no cross-thread hazard).
+__global__ void __launch_bounds__(kCcBlock) k_cc_rank_bits(uint32_t* lab, uint32_t n, const uint64_t* __restrict__ bits,
+                                                           const uint32_t* __restrict__ pre) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kCcBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kCcBlock) {
+    const uint32_t r = lab[i], w = r >> 6;
+    lab[i] = pre[w] + (uint32_t)__popcll(bits[w] & ((1ull << (r & 63)) - 1));
+  }
+}
+
+}  // namespace
+
+int cc_static_init(cac_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kStaticMaxDevices)
+    return fail(ctx, CAC_EDEVICE, "static connected-components scratch: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (g_static_ptr[ctx->device]) return CAC_OK;
+  void* p = nullptr;
+  CAC_HIP(ctx, hipSetDevice(ctx->device));
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(cc_static_scratch));
+  if (e != hipSuccess || !p) return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress");
+  g_static_ptr[ctx->device] = p;
+  return CAC_OK;
+}
+
 }  // namespace cac
 
 extern "C" {
@@ -833,6 +904,50 @@ int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, i
   rc = cac::cc_launch(ctx, d_edges, m, n, label_mode, d_labels, counter, s);
   if (rc) return rc;
   CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, counter, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  *ncomp_out = *(uint32_t*)ctx->pinned;
+  return CAC_OK;
+}
+
+int cac_cc_static_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
+                      uint32_t* d_labels, uint32_t* ncomp_out, void* stream) {
+  int rc = cac::bind(ctx);
+  if (rc) return rc;
+  if ((m && !d_edges) || (n && !d_labels) || !ncomp_out) return cac::fail(ctx, CAC_EINVAL, "null pointer");
+  if (label_mode != CAC_LABEL_MIN_ID && label_mode != CAC_LABEL_RANK)
+    return cac::fail(ctx, CAC_EINVAL, "label_mode must be CAC_LABEL_MIN_ID or CAC_LABEL_RANK");
+  if (n > CAC_CC_STATIC_MAX_N)
+    return cac::fail(ctx, CAC_EINVAL, "cac_cc_static_dev: n exceeds CAC_CC_STATIC_MAX_N (2^25 vertices), the size "
+                                      "of its static scratch; use cac_cc_dev");
+  rc = cac::cc_static_init(ctx);
+  if (rc) return rc;
+  uint32_t* tmp = (uint32_t*)cac::g_static_ptr[ctx->device];
+  hipStream_t s = cac::pick_stream(ctx, stream);
+  // per-kernel timing would create events: off for this call
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  rc = cac::cc_launch(ctx, d_edges, m, n, CAC_LABEL_MIN_ID, d_labels, tmp, s);
+  if (rc == CAC_OK && label_mode == CAC_LABEL_RANK && n) {
+    uint64_t* bits = reinterpret_cast<uint64_t*>(tmp + cac::kCcTmpHead);
+    uint32_t* pre = tmp + cac::kCcTmpHead + cac::kStaticBitsWords;
+    uint32_t* partial = pre + cac::kStaticPreWords;
+    const uint64_t words = ((uint64_t)n + 63) / 64;
+    hipLaunchKernelGGL(cac::k_cc_root_bits, dim3(cac::grid_for(words, cac::kCcBlock / 64, 256 * 64)),
+                       dim3(cac::kCcBlock), 0, s, (const uint32_t*)d_labels, n, bits);
+    hipError_t e = cac::device_exclusive_scan<uint32_t>(cac::WordPop{bits}, cac::WordPreStore{pre}, words, partial,
+                                                        (uint32_t*)nullptr, s);
+    if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "root prefix scan");
+    else
+      hipLaunchKernelGGL(cac::k_cc_rank_bits, dim3(cac::grid_for(n, cac::kCcBlock, 256 * 64)), dim3(cac::kCcBlock), 0,
+                         s, d_labels, n, (const uint64_t*)bits, (const uint32_t*)pre);
+    if (rc == CAC_OK) {
+      e = hipGetLastError();
+      if (e != hipSuccess) rc = cac::hip_fail(ctx, e, "rank kernels");
+    }
+  }
+  ctx->profiling = prof;
+  if (rc) return rc;
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, tmp, 4, hipMemcpyDeviceToHost, s));
   CAC_HIP(ctx, hipStreamSynchronize(s));
   *ncomp_out = *(uint32_t*)ctx->pinned;
   return CAC_OK;

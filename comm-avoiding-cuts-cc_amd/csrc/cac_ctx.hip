@@ -155,6 +155,9 @@ int cac_ctx_create(int device, cac_ctx** out) {
     delete ctx;
     return CAC_EDEVICE;
   }
+  // the static scratch of cac_cc_static_dev: its address on this device, taken
+  // now so that entry point never needs to (it retries if this failed)
+  (void)cac::cc_static_init(ctx);
   *out = ctx;
   return CAC_OK;
 }

@@ -100,6 +100,10 @@ struct BufPin {
   BufPin& operator=(const BufPin&) = delete;
 };
 
+// Takes the device address of the static connected-components scratch for the
+// context's device (once per device); cac_ctx_create calls it at set-up.
+int cc_static_init(cac_ctx* ctx);
+
 // Free device memory plus what the context's grow-only buffers already hold.
 int device_memory_available(cac_ctx* ctx, size_t* bytes);
 

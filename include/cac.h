@@ -111,6 +111,18 @@ int cac_cc(cac_ctx* ctx, const cac_uedge* edges, uint64_t m, uint32_t n, int lab
  * validated (the caller guarantees < n).  Blocks until *ncomp_out is known. */
 int cac_cc_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
                uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
+/* cac_cc_dev without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, labels and count as cac_cc_dev; its scratch (counters, the
+ * giant-component bitmap, the rank prefix) is a static device array of the
+ * library, about 6 MiB, which every GPU the library runs on holds until the
+ * process ends.  n <= CAC_CC_STATIC_MAX_N (CAC_EINVAL beyond).  All calls on
+ * one device share that array: calls that overlap on one GPU (other streams,
+ * other threads or contexts) race and must be serialised by the caller. */
+#define CAC_CC_STATIC_MAX_N (1u << 25)
+int cac_cc_static_dev(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, int label_mode,
+                      uint32_t* d_labels, uint32_t* ncomp_out, void* stream);
 
 /* ---- connected components on a symmetric CSR -------------------------------
  * The same labels as cac_cc_dev (CAC_LABEL_MIN_ID / CAC_LABEL_RANK, component
