@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--c3-trials", type=int, default=169, help="dense KS trials (ceil(log2(8192))^2)")
     ap.add_argument("--c4-graphs", type=int, default=4096)
     ap.add_argument("--c4-stride", type=int, default=16, help="one-graph-per-call baseline on every k-th graph")
+    ap.add_argument("--c4-ref-graphs", type=int, default=256,
+                    help="graphs solved under the reference's trial rule (SquareRootCut::numberOfTrials)")
     ap.add_argument("--c5-queries", type=int, default=200)
     ap.add_argument("--c5-rate", type=float, default=20.0, help="offered load, queries per second")
     ap.add_argument("--ks-sizes", default="32000,96000", help="Watts-Strogatz vertex counts of the ks leg")
@@ -120,15 +122,16 @@ def cpu_baseline(edges_host: np.ndarray, n: int, budget_s: float):
     """The oracle's union-find on the same edge list, on the host's cores.
 
     The headline figure is the multithreaded restatement (orc_cc_mt: one edge
-    chunk per thread, CAS hooking) on min(16, cores) threads -- the CPU share of
-    one GPU on the box -- over the whole edge list, repeated until about a third
-    of the budget is spent.  The single-thread union-find is timed beside it on
+    chunk per thread, CAS hooking) on OMP_NUM_THREADS threads when that is set
+    (the host's CPU share per GPU), else on every core nproc reports, over the
+    whole edge list, repeated until about a third of the budget is spent.  The single-thread union-find is timed beside it on
     the longest prefix that fits a third of the budget.
     """
     import oracle  # the CPU restatement: checker and baseline only
 
     m = edges_host.shape[0]
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc = os.cpu_count() or 1
+    threads = max(1, min(nproc, int(os.environ.get("OMP_NUM_THREADS") or nproc)))
     reps, total = 0, 0.0
     while True:
         t0 = time.perf_counter()
@@ -150,6 +153,8 @@ def cpu_baseline(edges_host: np.ndarray, n: int, budget_s: float):
         "value": m * reps / total / 1e9,
         "unit": "GTEPS",
         "cores": threads,
+        "nproc": nproc,
+        "threads_rule": "OMP_NUM_THREADS when set (the host's CPU share per GPU), else nproc",
         "kind": "port",
         "sample": f"the whole {m}-edge R-MAT edge list (sorted by (u,v)), {reps} pass(es) of the oracle's "
                   f"union-find on {threads} threads (CAS hooking, min-id roots), {total:.1f} s; "
@@ -173,6 +178,11 @@ def copy_probe(ctx, mib: int):
 
 
 F32_95 = float(np.float32(0.95))  # the reference parses p with std::stof
+
+
+def _note(msg: str):
+    """A progress line on stderr (a long leg must keep printing)."""
+    print(f"# {msg}", file=sys.stderr, flush=True)
 
 
 def bench_c1(ctx, cac, cpu_budget_s: float):
@@ -204,7 +214,7 @@ def bench_c1(ctx, cac, cpu_budget_s: float):
     return out
 
 
-def bench_c3(ctx, cac, trials: int):
+def bench_c3(ctx, cac, trials: int, cpu_budget_s: float = 0.0):
     """Dense Karger-Stein on the complete graph n = 8192, weights U[1, 42]."""
     n = 8192
     d_adj = ctx.alloc(n * n * 8)
@@ -223,6 +233,29 @@ def bench_c3(ctx, cac, trials: int):
                                           blk.nbytes))
         s = int(blk.sum(axis=1).min())
         deg_min = s if deg_min is None else min(deg_min, s)
+    base = None
+    if cpu_budget_s > 0:
+        import oracle  # the reference's own library (oracle/_ref): baseline only
+        if oracle.ref_available():
+            iu = np.triu_indices(n, 1)
+            w = np.empty(len(iu[0]), dtype=np.uint64)
+            pos = 0
+            for r in range(n - 1):  # the upper triangle of the device matrix, row by row
+                row = np.empty(n, dtype=np.int64)
+                ctx._check(ctx.lib.cac_memcpy_d2h(ctx.handle, row.ctypes.data_as(ctypes.c_void_p),
+                                                  d_adj.ptr + r * n * 8, row.nbytes))
+                w[pos: pos + n - 1 - r] = row[r + 1:]
+                pos += n - 1 - r
+            e3 = np.stack([iu[0].astype(np.uint64), iu[1].astype(np.uint64), w], axis=1)
+            del iu, w
+            t0 = time.perf_counter()
+            ref_cut = oracle.ref_ks_mincut(n, e3, 0.0, 0)  # p = 0: exactly one trial (dense: m >= n^2/32)
+            per_trial = time.perf_counter() - t0
+            del e3
+            base = {"value": 1.0 / per_trial, "unit": "trials/s", "cores": 1, "kind": "reference",
+                    "sample": f"one trial of the reference's comincut::minimum_cut (p=0) on the same complete "
+                              f"graph, {per_trial:.1f} s on one core (cut {ref_cut}); x{trials} trials = "
+                              f"{per_trial * trials:.0f} s per solve"}
     d_adj.free()
     bytes_per_trial = 22.88e9  # sum over recursion children of 8 (s^2 + t'^2) + 8 * 128^2 per leaf
     return {"workload": f"dense Karger-Stein, complete graph n={n}, weights U[1,42], {trials} trials, seed 0",
@@ -230,7 +263,8 @@ def bench_c3(ctx, cac, trials: int):
             "raw_cuts_equal_bound": bool(np.all(raw == deg_min)),
             "roofline": {"bound": "hbm", "algorithmic_bytes_per_trial": bytes_per_trial,
                          "achieved_gbs": bytes_per_trial * trials / dt / 1e9, "peak": HBM_PEAK_GBS,
-                         "frac": bytes_per_trial * trials / dt / 1e9 / HBM_PEAK_GBS}}
+                         "frac": bytes_per_trial * trials / dt / 1e9 / HBM_PEAK_GBS},
+            "cpu_baseline": base}
 
 
 # The reference's karger_stein (p = 0.90, one core) on its Watts-Strogatz inputs:
@@ -393,43 +427,132 @@ def bench_rm(ctx, cac, degrees):
     return out
 
 
-def bench_c4(ctx, cac, count: int, stride: int):
-    """Batched square-root min cut over planted-cut graphs vs one graph per call."""
+def _c4_run(ctx, cac, descs, edges, planted, stride: int, seed: int = 0, warm: bool = True, tag: str = ""):
+    """Batched solve of descs (trials as set in descs) vs every stride-th graph
+    one per call; returns the figures and the batch's cuts."""
+    buf = ctx.alloc(edges.nbytes)
+    buf.upload(edges)
+    if warm:
+        ctx.mincut_batch_dev(buf.ptr, descs, F32_95, seed + 1)  # warm-up at full size (another seed)
+    t0 = time.perf_counter()
+    cuts, trials = ctx.mincut_batch_dev(buf.ptr, descs, F32_95, seed)
+    dt = time.perf_counter() - t0
+    _note(f"c4 {tag}: {len(descs)} graphs, {int(trials.sum())} trials batched in {dt:.2f} s")
+    sub = np.arange(0, len(descs), stride)
+    t0 = time.perf_counter()
+    single = []
+    for g in sub:
+        single.append(ctx.mincut_batch_dev(buf.ptr, descs[g:g + 1], F32_95, seed)[0][0])
+        if time.perf_counter() - t0 > 60:
+            raise RuntimeError("one-graph-per-call sample over 60 s")
+    dt1 = time.perf_counter() - t0
+    _note(f"c4 {tag}: {len(sub)} graphs one per call in {dt1:.2f} s")
+    buf.free()
+    batched_rate, single_rate = len(descs) / dt, len(sub) / dt1
+    # the one-graph-per-call time of the whole batch, estimated from the sample
+    # in proportion to each graph's trials x edges
+    work = descs["trials"].astype(np.float64) * descs["m"].astype(np.float64)
+    single_total_s = dt1 * work.sum() / work[sub].sum()
+    return {"graphs": int(len(descs)), "edges": int(len(edges)), "total_trials": int(trials.sum()),
+            "solves_per_s": batched_rate, "seconds": dt,
+            "one_graph_per_call_solves_per_s": single_rate,
+            "one_graph_per_call_sample": f"every {stride}th graph ({len(sub)} graphs, {dt1:.2f} s)",
+            "speedup": batched_rate / single_rate,
+            "speedup_work_weighted": single_total_s / dt,
+            "success_fraction": float((cuts == planted).mean()),
+            "never_below_planted": bool(np.all(cuts >= planted)),
+            "batched_equals_single": bool(np.array_equal(np.array(single), cuts[sub]))}, cuts
+
+
+def _c4_cpu_baseline(cac, descs, edges, t, budget_s: float):
+    """The oracle's square-root trials (single-threaded port) on a sample of the
+    batch's graphs, scaled to the batch: seconds per trial by graph size class."""
+    import oracle  # the CPU restatement: baseline only
+
+    order = np.argsort(descs["n"])
+    picks = order[np.linspace(0, len(order) - 1, 12).astype(int)]
+    per_trial, done, t0 = [], 0, time.perf_counter()
+    for g in picks:
+        d = descs[g]
+        e = edges[int(d["edge_offset"]): int(d["edge_offset"] + d["m"])]
+        e3 = np.stack([e["u"], e["v"], e["w"]], axis=1).astype(np.uint64)
+        s0 = time.perf_counter()
+        oracle.sqrt_trial(int(d["n"]), e3, int(t[g]), 0, 0)
+        per_trial.append((float(d["m"]), time.perf_counter() - s0))
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    # seconds per trial grow with the edge count: fit a line through the sample
+    ms_, ts_ = np.array([a for a, _ in per_trial]), np.array([b for _, b in per_trial])
+    k = float(np.dot(ms_, ts_) / np.dot(ms_, ms_))
+    total = float(k * (descs["trials"].astype(np.float64) * descs["m"].astype(np.float64)).sum())
+    return {"value": len(descs) / total, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one trial of each of {done} graphs spread over the size range, "
+                      f"{ts_.sum():.1f} s, timed single-threaded; seconds per trial fitted as "
+                      f"{k * 1e9:.1f} ns x edges and scaled to the batch's {int(descs['trials'].sum())} trials"}
+
+
+def bench_c4(ctx, cac, count: int, stride: int, ref_count: int, cpu_budget_s: float):
+    """Batched square-root min cut over planted-cut graphs vs one graph per call,
+    under two trial rules, on the standard and the hard planted family."""
+    out = {}
     t0 = time.perf_counter()
     descs, edges, planted = cac.gen_planted(0, count, 0)
     gen_s = time.perf_counter() - t0
     n = descs["n"].astype(np.int64)
-    m = descs["m"].astype(np.int64)
-    t = np.array([cac.intermediate_size(int(a), int(b)) for a, b in zip(n, m)])
-    descs["trials"] = np.maximum(2, [cac.ks_trials(int(x), F32_95) for x in t])
-    buf = ctx.alloc(edges.nbytes)
-    buf.upload(edges)
-    m_total = len(edges)
-    del edges
-    ctx.mincut_batch_dev(buf.ptr, descs, F32_95, 1)  # warm-up at full size (another seed)
-    t0 = time.perf_counter()
-    cuts, trials = ctx.mincut_batch_dev(buf.ptr, descs, F32_95, 0)
-    dt = time.perf_counter() - t0
-    sub = np.arange(0, count, stride)
-    t0 = time.perf_counter()
-    single = [ctx.mincut_batch_dev(buf.ptr, descs[g:g + 1], F32_95, 0)[0][0] for g in sub]
-    dt1 = time.perf_counter() - t0
-    buf.free()
-    batched_rate, single_rate = count / dt, len(sub) / dt1
-    return {"workload": f"{count} planted-cut graphs, n log-uniform in [256, 65536] (mean {n.mean():.0f}), "
-                        f"halves G(n/2, 16/(n/2-1)) weights U[1,100], c in [1,8] unit cross edges, "
-                        f"{m_total} edges, generated in {gen_s:.1f} s",
-            "trial_rule": "T_g = max(2, ks_trials(t_g, 0.95)), t_g = intermediate_size(n_g, m_g)",
-            "total_trials": int(trials.sum()), "solves_per_s": batched_rate, "seconds": dt,
-            "one_graph_per_call_solves_per_s": single_rate,
-            "one_graph_per_call_sample": f"every {stride}th graph ({len(sub)} graphs)",
-            "speedup": batched_rate / single_rate,
-            "success_fraction": float((cuts == planted).mean()),
-            "batched_equals_single": bool(np.array_equal(np.array(single), cuts[sub])),
-            "launch_shapes": ctx.tuning_file or "built-in rule (no autotuner table)"}
+    t = np.array([cac.intermediate_size(int(a), int(b)) for a, b in zip(n, descs["m"])])
+    light = np.maximum(2, [cac.ks_trials(int(x), F32_95) for x in t])
+    fam = (f"n log-uniform in [256, 65536] (mean {n.mean():.0f}), halves G(n/2, 16/(n/2-1)) weights U[1,100], "
+           f"c in [1,8] unit cross edges, {len(edges)} edges, generated in {gen_s:.1f} s")
+    # 1. the light rule over the full batch (the C4 configuration)
+    descs["trials"] = light
+    _note(f"c4: {count} planted graphs generated in {gen_s:.1f} s")
+    r, _ = _c4_run(ctx, cac, descs, edges, planted, stride, tag="light rule")
+    r["workload"] = f"{count} planted-cut graphs, " + fam
+    r["trial_rule"] = ("light: T_g = max(2, ks_trials(t_g, 0.95)), t_g = intermediate_size(n_g, m_g); "
+                       "no success bound of its own")
+    r["launch_shapes"] = ctx.tuning_file or "built-in rule (no autotuner table)"
+    if cpu_budget_s > 0:
+        r["cpu_baseline"] = _c4_cpu_baseline(cac, descs, edges, t, cpu_budget_s)
+    out.update(r)
+    # the hard family (exact planted cut within 10% of the minimum weighted degree) under the light rule
+    hd, he, hp = cac.gen_planted_hard(0, count, 0)
+    ht = np.array([cac.intermediate_size(int(a), int(b)) for a, b in zip(hd["n"], hd["m"])])
+    hd["trials"] = np.maximum(2, [cac.ks_trials(int(x), F32_95) for x in ht])
+    hr, _ = _c4_run(ctx, cac, hd, he, hp, stride, tag="hard family, light rule")
+    out["hard_family_light_rule"] = {
+        "workload": f"{count} hard planted graphs (same sizes): halves of 8 Hamiltonian cycles with cycle weights "
+                    f"U[1,100], planted cut floor(0.9 x minimum weighted degree) on c in [1,8] cross edges",
+        "trial_rule": out["trial_rule"], **{k: hr[k] for k in ("success_fraction", "never_below_planted",
+                                                                "total_trials", "solves_per_s")}}
+    del he
+    # 2. the reference's rule (SquareRootCut::numberOfTrials) on the first ref_count graphs
+    rd, rp = descs[:ref_count].copy(), planted[:ref_count]
+    re_ = edges[: int(rd["edge_offset"][-1] + rd["m"][-1])]
+    rd["trials"] = [cac.sqrt_trials(int(a), int(b), F32_95) for a, b in zip(rd["n"], rd["m"])]
+    rr, _ = _c4_run(ctx, cac, rd, re_, rp, max(1, stride // 2), warm=False, tag="reference rule")
+    rr["workload"] = f"the first {ref_count} graphs of the same batch"
+    rr["trial_rule"] = "reference: SquareRootCut::numberOfTrials (src/SquareRootCut.cpp:76-83), success >= 0.95 by its bound"
+    hard_count = max(1, ref_count // 4)  # the hard family's trials run several times longer
+    hd2, he2, hp2 = cac.gen_planted_hard(0, hard_count, 0)
+    hd2["trials"] = [cac.sqrt_trials(int(a), int(b), F32_95) for a, b in zip(hd2["n"], hd2["m"])]
+    h2, _ = _c4_run(ctx, cac, hd2, he2, hp2, max(1, stride // 2), warm=False, tag="hard family, reference rule")
+    rr["hard_family"] = {"graphs": hard_count,
+                         **{k: h2[k] for k in ("success_fraction", "never_below_planted", "total_trials",
+                                               "solves_per_s", "seconds", "speedup", "speedup_work_weighted")}}
+    out["reference_rule"] = rr
+    return out
 
 
-def bench_c5(cac, count: int, rate: float):
+def light_trials(n: int, m: int) -> int:
+    """A served min cut's trial budget: the Karger-Stein trial count for p = 0.95
+    of its contracted graph (t = min(intermediate_size, 4096)), at least 2."""
+    import cac
+    t = min(cac.intermediate_size(n, m), 4096)
+    return max(2, cac.ks_trials(t, F32_95))
+
+
+def bench_c5(cac, count: int, rate: float, cpu_budget_s: float = 0.0):
     """Served Poisson stream of CC (70%) and min-cut (30%) queries."""
     from concurrent.futures import ThreadPoolExecutor
 
@@ -440,13 +563,19 @@ def bench_c5(cac, count: int, rate: float):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
     with ThreadPoolExecutor(16) as ex:
         edges = list(ex.map(lambda i: cac.gen_gnp(int(ns[i]), float(ds[i]), 0, i, 100), range(count)))
-    recs = [cac.Server.query(int(kinds[i]), edges[i], int(ns[i]), F32_95, 0) for i in range(count)]
+    # min-cut queries: a trial budget (the Karger-Stein trial count of the
+    # contracted graph, at most 4096 vertices) and a seed of their own
+    budget_T = [light_trials(int(ns[i]), len(edges[i])) for i in range(count)]
+    recs = [cac.Server.query(int(kinds[i]), edges[i], int(ns[i]), F32_95, seed=i, trials=budget_T[i])
+            for i in range(count)]
     budget = 160 << 30
+    _note(f"c5: {count} queries generated")
     with cac.Server(0, workers=4, hbm_budget=budget, max_batch=64) as srv:
         t0 = time.perf_counter()
         res = srv.replay(recs, arrivals)
         wall = time.perf_counter() - t0
         peak, groups = srv.stats()
+    _note(f"c5: stream served in {wall:.1f} s")
     lat = res["done_s"] - res["submit_s"]
     out = {"workload": f"{count} queries, Poisson arrivals at {rate}/s, 70% CC / 30% min cut (p=0.95), "
                        f"G(n, d/n), n log-uniform in [1e3, 1e6], d uniform in [8, 64], weights U[1,100], "
@@ -459,22 +588,46 @@ def bench_c5(cac, count: int, rate: float):
         lk = lat[kinds == k]
         out[f"{name}_latency_ms"] = {"count": int(len(lk)), "p50": float(np.percentile(lk, 50) * 1e3),
                                      "p99": float(np.percentile(lk, 99) * 1e3)}
-    # served min-cut throughput: the stream's min-cut queries all at once, grouped on 4 workers,
-    # vs the same queries one graph per launch group on one worker
-    mc = [recs[i] for i in range(count) if kinds[i] == cac.QUERY_MINCUT][:24]
+    # served min-cut throughput: every min-cut query of the stream submitted at
+    # once (closed loop), grouped on 4 workers, vs the same queries one graph
+    # per launch group on one worker
+    mc = [recs[i] for i in range(count) if kinds[i] == cac.QUERY_MINCUT]
     rates = {}
     for name, w, mb in (("grouped", 4, 64), ("one_graph_per_launch", 1, 1)):
         with cac.Server(0, workers=w, hbm_budget=budget, max_batch=mb) as srv:
             t0 = time.perf_counter()
             r = srv.replay(mc, np.zeros(len(mc)))
             rates[name] = len(mc) / (time.perf_counter() - t0)
+            _note(f"c5: {len(mc)} min-cut queries {name}: {rates[name]:.1f} queries/s")
             assert np.all(r["status"] == 0)
             if name == "grouped":
                 ref = r["cut"].copy()
             else:
                 out["mincut_grouped_equals_single"] = bool(np.array_equal(ref, r["cut"]))
+    out["mincut_closed_loop_queries"] = len(mc)
     out["mincut_closed_loop_qps"] = rates
     out["mincut_speedup"] = rates["grouped"] / rates["one_graph_per_launch"]
+    if cpu_budget_s > 0:
+        import oracle  # the CPU restatement: baseline only
+        t0, done = time.perf_counter(), 0
+        for i in np.argsort(ns):  # smallest queries first, until the budget is spent
+            e = edges[i]
+            if kinds[i] == cac.QUERY_CC:
+                oracle.cc(np.stack([e["u"], e["v"]], 1).astype(np.uint32), int(ns[i]))
+            else:
+                e3 = np.stack([e["u"], e["v"], e["w"]], axis=1).astype(np.uint64)
+                t = min(cac.intermediate_size(int(ns[i]), len(e)), 4096)
+                for k in range(budget_T[i]):
+                    oracle.sqrt_trial(int(ns[i]), e3, t, i, k)
+            done += 1
+            if time.perf_counter() - t0 > cpu_budget_s:
+                break
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": f"the {done} smallest of the {count} queries (n <= {int(np.sort(ns)[done - 1])}) "
+                                         f"answered one after another on one core in {cpu_s:.1f} s: CC by the "
+                                         f"oracle's union-find, min cut by its square-root trials with the same "
+                                         f"budgets; larger queries cost more, so this overstates the CPU rate"}
     return out
 
 
@@ -616,9 +769,10 @@ def main():
     wanted = set() if args.extras == "none" else set(args.extras.split(","))
     extras = {}
     for key, fn in (("c1", lambda: bench_c1(ctx, cac, min(args.cpu_baseline_seconds, 10.0))),
-                    ("c3", lambda: bench_c3(ctx, cac, args.c3_trials)),
-                    ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride)),
-                    ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate)),
+                    ("c3", lambda: bench_c3(ctx, cac, args.c3_trials, args.cpu_baseline_seconds)),
+                    ("c4", lambda: bench_c4(ctx, cac, args.c4_graphs, args.c4_stride, args.c4_ref_graphs,
+                                            min(args.cpu_baseline_seconds, 15.0))),
+                    ("c5", lambda: bench_c5(cac, args.c5_queries, args.c5_rate, min(args.cpu_baseline_seconds, 15.0))),
                     ("ks", lambda: bench_ks(ctx, cac, [int(x) for x in args.ks_sizes.split(",")],
                                                     min(args.cpu_baseline_seconds, 20.0))),
                     ("sq", lambda: bench_sq(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],

@@ -16,9 +16,18 @@ the built-in rule.  The fastest candidate whose raw per-trial cuts equal the
 built-in rule's wins (results do not depend on the block size; a candidate that
 changed one would be a bug, and the tuner stops).
 
-Writes {"device": ..., "ks_contract_block": {"512": B, ...}, "timings_ms": ...}
-to tuning/gfx950.json next to this file (or --out); every cac.Context applies
-that file when it exists.  Run on the target GPU:
+Knob CAC_TUNE_SAMPLE_OCCUPANCY: 1024-thread sampling workgroups per CU (2:
+64 registers per thread with some spilled, 1: the registers the kernel wants)
+for a launch of more square-root trials than CUs, per edge-count class (keys
+16, 20, 24: m < 2^16, 2^20, 2^24).  Each class is timed on many square-root
+trials of one graph of that size (the trials of a batch or of one large solve
+run that way), and the faster setting whose raw cuts equal the built-in
+setting's wins.
+
+Writes {"device": ..., "ks_contract_block": {"512": B, ...},
+"sample_occupancy": {"16": k, ...}, "timings_ms": ...} to tuning/gfx950.json
+next to this file (or --out); every cac.Context applies that file when it
+exists.  Run on the target GPU:
 
     python comm-avoiding-cuts-cc_amd/autotune.py [--out PATH] [--reps 3]
 """
@@ -90,15 +99,69 @@ def tune(device: int = 0, reps: int = 3, log=print):
     return table, timings
 
 
+# (class key, graph, trials): one graph per edge-count class and enough trials
+# to fill the device several times over.
+SAMPLE_CLASSES = ((16, ("er", 2000, 20.0), 4096), (20, ("ws", 32000, 16), 2048), (24, ("er", 100000, 100.0), 512))
+
+
+def _sample_graph(spec):
+    kind, n, d = spec
+    if kind == "ws":
+        return n, cac.gen_ws(n, int(d), 0)
+    return n, cac.gen_gnp(n, d, 0, 0, wmax=100)
+
+
+def tune_sampling(device: int = 0, reps: int = 3, log=print):
+    table, timings = {}, {}
+    with cac.Context(device, tuning=False) as ctx:
+        for key, spec, trials in SAMPLE_CLASSES:
+            n, e = _sample_graph(spec)
+            t = cac.intermediate_size(n, len(e))
+            res, raw0 = {}, None
+            for occ in (2, 1):
+                ctx.tune_set(cac.TUNE_SAMPLE_OCCUPANCY, key, occ)
+                raw = ctx.sqrt_cut_trials(e, n, t, 0, 0, trials)  # warm-up
+                times = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    r = ctx.sqrt_cut_trials(e, n, t, 0, 0, trials)
+                    times.append((time.perf_counter() - t0) * 1e3)
+                    if r.tolist() != raw.tolist():
+                        raise RuntimeError("repeated trials changed a raw cut")
+                if raw0 is None:
+                    raw0 = raw
+                elif raw.tolist() != raw0.tolist():
+                    raise RuntimeError(f"occupancy {occ} for class {key} changed a raw cut")
+                res[occ] = sorted(times)[len(times) // 2]
+                log(f"sampling class 2^{key}: {occ} workgroup(s) per CU  {res[occ]:8.2f} ms  "
+                    f"({trials} trials, n={n}, m={len(e)}, t={t})")
+            best = min(res, key=res.get)
+            table[str(key)] = best
+            timings[str(key)] = {"n": n, "m": len(e), "t": t, "trials": trials,
+                                 "ms": {str(k): v for k, v in res.items()}}
+            ctx.tune_set(cac.TUNE_SAMPLE_OCCUPANCY, key, 0)
+    return table, timings
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=str(cac.TUNING_PATH))
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--only", choices=("all", "sampling"), default="all",
+                    help="'sampling': keep the committed ks_contract_block table, tune only the sampling knob")
     args = ap.parse_args()
-    table, timings = tune(args.device, args.reps, log=lambda s: print(s, flush=True))
-    out = {"device": "gfx950 (MI355X)", "knob": "ks_contract_block", "ks_contract_block": table,
-           "timings_ms": timings}
+    ap_log = lambda s: print(s, flush=True)  # noqa: E731
+    if args.only == "sampling" and Path(args.out).exists():
+        old = json.loads(Path(args.out).read_text())
+        table = old["ks_contract_block"]
+        timings = old["timings_ms"].get("ks_contract_block", old["timings_ms"])
+    else:
+        table, timings = tune(args.device, args.reps, log=ap_log)
+    stable, stimings = tune_sampling(args.device, args.reps, log=ap_log)
+    out = {"device": "gfx950 (MI355X)", "knobs": ["ks_contract_block", "sample_occupancy"],
+           "ks_contract_block": table, "sample_occupancy": stable,
+           "timings_ms": {"ks_contract_block": timings, "sample_occupancy": stimings}}
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     Path(args.out).write_text(json.dumps(out, indent=1) + "\n")
     print(f"wrote {args.out}: {table}", flush=True)

@@ -18,6 +18,8 @@ LIB_PATH = PKG_DIR / "lib" / "libcac.so"
 TUNING_PATH = PKG_DIR / "tuning" / "gfx950.json"
 TUNE_KS_CONTRACT_BLOCK = 1
 TUNE_KS_CONTRACT_KEYS = (512, 1024, 2048, 4096)
+TUNE_SAMPLE_OCCUPANCY = 2
+TUNE_SAMPLE_OCCUPANCY_KEYS = (16, 20, 24, 32)
 
 CAC_OK = 0
 CAC_EINVAL = -1
@@ -29,6 +31,8 @@ CAC_EIO = -6
 LABEL_MIN_ID = 0
 LABEL_RANK = 1
 CC_STATIC_MAX_N = 1 << 25  # CAC_CC_STATIC_MAX_N
+SAMPLE_LC = 0  # |V| draws per round (the reference's low-concurrency rule)
+SAMPLE_HC = 1  # floor(n^1.05) draws per round (its high-concurrency rule)
 
 # Edge records, identical to cac_uedge / cac_edge.
 UEDGE = np.dtype([("u", "<u4"), ("v", "<u4")])
@@ -39,7 +43,7 @@ GRAPH_DESC = np.dtype([("edge_offset", "<u8"), ("m", "<u8"), ("n", "<u4"), ("tri
 QUERY_CC = 0
 QUERY_MINCUT = 1
 QUERY = np.dtype([("kind", "<i4"), ("n", "<u4"), ("m", "<u8"), ("edges", "<u8"), ("p", "<f8"), ("seed", "<u4"),
-                  ("reserved", "<u4"), ("labels_out", "<u8")])
+                  ("trials", "<u4"), ("labels_out", "<u8"), ("graph", "<u8")])
 QUERY_RESULT = np.dtype([("id", "<u8"), ("status", "<i4"), ("ncomp", "<u4"), ("cut", "<i8"), ("t", "<u4"),
                          ("trials", "<u4"), ("submit_s", "<f8"), ("admit_s", "<f8"), ("done_s", "<f8"),
                          ("batch_size", "<u4"), ("reserved", "<u4")])
@@ -87,6 +91,8 @@ SIGNATURES = {
     "cac_ks_trials": (_u32, [_u32, ctypes.c_double]),
     "cac_intermediate_size": (_u32, [_u32, _u64]),
     "cac_sqrt_trials": (_u32, [_u32, _u64, ctypes.c_double]),
+    "cac_set_sampling_rule": (_i32, [_vp, _i32]),
+    "cac_hc_draws": (_u32, [_u32]),
     "cac_sample_contract": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _pu32, _pu32, _vp, _u64, _pu64]),
     "cac_sqrt_cut_trials": (_i32, [_vp, _vp, _u64, _u32, _u32, _u32, _u32, _u32, _pi64]),
     "cac_mincut": (_i32, [_vp, _vp, _u64, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
@@ -97,7 +103,7 @@ SIGNATURES = {
     "cac_gen_er": (_i32, [_u32, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_mincut_batch": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32]),
     "cac_mincut_batch_dev": (_i32, [_vp, _vp, _vp, _u32, ctypes.c_double, _u32, _pi64, _pu32, _vp]),
-    "cac_serve_mincut_plan": (_i32, [_u32, _u64, ctypes.c_double, _pu32, _pu32]),
+    "cac_serve_mincut_plan": (_i32, [_u32, _u64, ctypes.c_double, _u32, _pu32, _pu32]),
     "cac_server_create": (_i32, [_i32, _u32, _u64, _u32, ctypes.POINTER(_vp)]),
     "cac_server_submit": (_i32, [_vp, _vp, _pu64]),
     "cac_server_poll": (_i32, [_vp, _vp, _u32, _pu32, _i32]),
@@ -105,11 +111,15 @@ SIGNATURES = {
     "cac_server_replay": (_i32, [_vp, _vp, _vp, _u32, _vp]),
     "cac_server_stats": (_i32, [_vp, _pu64, _pu64]),
     "cac_server_destroy": (_i32, [_vp]),
+    "cac_server_register_graph": (_i32, [_vp, _vp, _u64, _u32, _pu64]),
+    "cac_server_unregister_graph": (_i32, [_vp, _u64]),
     "cac_gen_gnp": (_i32, [_u32, ctypes.c_double, _u32, _u32, _u32, _vp, _u64, _pu64]),
     "cac_gen_ws": (_i32, [_u32, _u32, ctypes.c_double, _u32, _u64, _vp, _u64, _pu64]),
     "cac_gen_planted_sizes": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _pu32, _pu64]),
     "cac_gen_planted": (_i32, [_u32, _u32, _u32, _u32, _u32, ctypes.c_double, _u32, _u32, _vp, _vp, _u64, _pu64,
                                _pu32]),
+    "cac_gen_planted_hard": (_i32, [_u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, ctypes.c_double, _vp, _vp, _u64,
+                                    _pu64, _pu32]),
 }
 
 _lib = None
@@ -188,6 +198,10 @@ class Context:
             if tuning or path.exists():
                 self.load_tuning(path)
 
+    def set_sampling_rule(self, rule: int):
+        """SAMPLE_LC (default) or SAMPLE_HC for every sampling call on this context."""
+        self._check(self.lib.cac_set_sampling_rule(self.handle, rule))
+
     # ---- tuning -----------------------------------------------------------
     def tune_set(self, knob: int, key: int, value: int):
         self._check(self.lib.cac_tune_set(self.handle, knob, key, value))
@@ -198,12 +212,15 @@ class Context:
         return int(v.value)
 
     def load_tuning(self, path: str | os.PathLike):
-        """Apply a table written by autotune.py: {"ks_contract_block": {"512": B, ...}}."""
+        """Apply a table written by autotune.py: {"ks_contract_block": {"512": B, ...},
+        "sample_occupancy": {"16": k, ...}}."""
         import json
 
         table = json.loads(Path(path).read_text())
         for key, value in table.get("ks_contract_block", {}).items():
             self.tune_set(TUNE_KS_CONTRACT_BLOCK, int(key), int(value))
+        for key, value in table.get("sample_occupancy", {}).items():
+            self.tune_set(TUNE_SAMPLE_OCCUPANCY, int(key), int(value))
         self.tuning_file = str(path)
 
     def close(self):
@@ -452,13 +469,16 @@ class Server:
         self._keep = []
 
     @staticmethod
-    def query(kind: int, edges, n: int, p: float = 0.95, seed: int = 0, labels: np.ndarray | None = None):
-        """One QUERY record; returns (record, arrays to keep alive)."""
-        e = to_edges(edges)
+    def query(kind: int, edges, n: int, p: float = 0.95, seed: int = 0, labels: np.ndarray | None = None,
+              trials: int = 0, graph: int = 0):
+        """One QUERY record; returns (record, arrays to keep alive).  trials: 0 = the
+        reference's rule for p, k > 0 = a budget of k trials (serve_mincut_plan).
+        graph: a handle from register_graph (then edges may be None)."""
+        e = to_edges(edges) if edges is not None else np.zeros(0, dtype=EDGE)
         q = np.zeros(1, dtype=QUERY)
         q["kind"], q["n"], q["m"] = kind, n, len(e)
         q["edges"] = e.ctypes.data if len(e) else 0
-        q["p"], q["seed"] = p, seed
+        q["p"], q["seed"], q["trials"], q["graph"] = p, seed, trials, graph
         q["labels_out"] = labels.ctypes.data if labels is not None else 0
         return q[0], (e, labels)
 
@@ -489,6 +509,20 @@ class Server:
             raise CacError(rc, "cac_server_poll")
         return out[: k.value]
 
+    def register_graph(self, edges, n: int) -> int:
+        """Copies a graph to device memory held by the server; returns its handle."""
+        e = to_edges(edges)
+        h = ctypes.c_uint64()
+        rc = self.lib.cac_server_register_graph(self.handle, _ptr(e), len(e), n, ctypes.byref(h))
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_register_graph")
+        return int(h.value)
+
+    def unregister_graph(self, graph: int):
+        rc = self.lib.cac_server_unregister_graph(self.handle, graph)
+        if rc != CAC_OK:
+            raise CacError(rc, "cac_server_unregister_graph")
+
     def stats(self):
         """(peak admitted footprint in bytes, launch groups run)."""
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
@@ -510,9 +544,10 @@ class Server:
         self.close()
 
 
-def serve_mincut_plan(n: int, m: int, p: float):
+def serve_mincut_plan(n: int, m: int, p: float, trials: int = 0):
+    """(t, T) of a served min-cut query (trials = 0: the reference's rule)."""
     t, T = ctypes.c_uint32(), ctypes.c_uint32()
-    rc = load_library().cac_serve_mincut_plan(n, m, p, ctypes.byref(t), ctypes.byref(T))
+    rc = load_library().cac_serve_mincut_plan(n, m, p, trials, ctypes.byref(t), ctypes.byref(T))
     if rc != CAC_OK:
         raise CacError(rc, "cac_serve_mincut_plan")
     return int(t.value), int(T.value)
@@ -534,6 +569,11 @@ def to_edges(edges) -> np.ndarray:
     out["v"] = a[:, 1]
     out["w"] = a[:, 2]
     return out
+
+
+def hc_draws(n0: int) -> int:
+    """Draws per round of the high-concurrency sampling rule for n0 vertices."""
+    return int(load_library().cac_hc_draws(n0))
 
 
 def ks_trials(n: int, p: float) -> int:
@@ -587,6 +627,29 @@ def gen_planted(first: int, count: int, seed: int, nmin: int = 256, nmax: int = 
                              len(edges), ctypes.byref(m), planted.ctypes.data_as(_pu32))
     if rc != CAC_OK:
         raise CacError(rc, "cac_gen_planted")
+    return descs[:count], edges[: m.value], planted[:count]
+
+
+def gen_planted_hard(first: int, count: int, seed: int, nmin: int = 256, nmax: int = 65536, cycles: int = 8,
+                     wmax: int = 100, cmax: int = 8, cut_frac: float = 0.9):
+    """Hard planted-cut graphs (see cac_gen_planted_hard): halves of `cycles`
+    weighted Hamiltonian cycles, planted cut C = floor(cut_frac * minimum
+    weighted degree).  Returns (descs, edges, planted C per graph)."""
+    lib = load_library()
+    ns = np.empty(max(count, 1), dtype=np.uint32)
+    bound = ctypes.c_uint64()
+    rc = lib.cac_gen_planted_sizes(first, count, seed, nmin, nmax, 2.0 * cycles, cmax, ns.ctypes.data_as(_pu32),
+                                   ctypes.byref(bound))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_planted_sizes")
+    descs = np.empty(max(count, 1), dtype=GRAPH_DESC)
+    edges = np.empty(max(bound.value, 1), dtype=EDGE)
+    planted = np.empty(max(count, 1), dtype=np.uint32)
+    m = ctypes.c_uint64()
+    rc = lib.cac_gen_planted_hard(first, count, seed, nmin, nmax, cycles, wmax, cmax, cut_frac, _ptr(descs),
+                                  _ptr(edges), len(edges), ctypes.byref(m), planted.ctypes.data_as(_pu32))
+    if rc != CAC_OK:
+        raise CacError(rc, "cac_gen_planted_hard")
     return descs[:count], edges[: m.value], planted[:count]
 
 

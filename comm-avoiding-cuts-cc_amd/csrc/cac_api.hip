@@ -118,6 +118,15 @@ extern "C" {
 uint32_t cac_ks_trials(uint32_t n, double p) { return cac::ks_trials_host(n, p); }
 uint32_t cac_sqrt_trials(uint32_t n, uint64_t m, double p) { return cac::sqrt_trials_host(n, m, p); }
 uint32_t cac_intermediate_size(uint32_t n, uint64_t m) { return cac::sqrt_intermediate_size(n, m); }
+uint32_t cac_hc_draws(uint32_t n0) { return cac::hc_draws_host(n0); }
+
+int cac_set_sampling_rule(cac_ctx* ctx, int rule) {
+  if (!ctx) return CAC_EINVAL;
+  if (rule != CAC_SAMPLE_LC && rule != CAC_SAMPLE_HC)
+    return cac::fail(ctx, CAC_EINVAL, "sampling rule must be CAC_SAMPLE_LC or CAC_SAMPLE_HC");
+  ctx->sample_rule = rule;
+  return CAC_OK;
+}
 
 int cac_mincut_dev(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, double p, uint32_t seed,
                    int64_t* cut, uint32_t* trials, void* stream) {

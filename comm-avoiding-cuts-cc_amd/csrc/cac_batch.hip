@@ -107,10 +107,97 @@ bool planted_graph(uint32_t g, uint32_t seed, uint32_t n, double avg_degree, uin
   return false;
 }
 
+// Graph g of the hard family: each half is the union of k Hamiltonian cycles
+// over its h = n/2 vertices.  Cycle i of half `half` visits the vertices in the
+// order of a Fisher-Yates shuffle drawn from Philox(ctr = {g, 2^31 + 2 half,
+// i, j}, key = {seed, 'PLNT'}) word 0, and all its edges weigh w_i = 1 +
+// floor(wmax * word0 / 2^32) of Philox(ctr = {g, 2^31 + 2 half + 1, i, 0}).
+// Every vertex of the half then has weighted degree delta = 2 sum w_i, and a
+// cut inside the half crosses every cycle at least twice, so the half's
+// minimum cut is delta.  c cross edges (c uniform in [1, cmax], distinct pairs)
+// carry C = floor(cut_frac * min(delta_A, delta_B)) split as evenly as
+// integers allow; with C < min delta the minimum cut of the graph is C,
+// the cut between the halves, and it lies within (1 - cut_frac) of the
+// minimum weighted degree.
+bool planted_hard_graph(uint32_t g, uint32_t seed, uint32_t n, uint32_t k, uint32_t wmax, uint32_t cmax,
+                        double cut_frac, std::vector<cac_edge>& out, uint32_t* C_out) {
+  const uint32_t h = n / 2;
+  if (h < 3 || k == 0) return false;
+  out.clear();
+  std::vector<uint32_t> perm(h);
+  uint64_t delta[2] = {0, 0};
+  for (uint32_t half = 0; half < 2; ++half) {
+    const uint32_t base = half * h;
+    for (uint32_t i = 0; i < k; ++i) {
+      const u32x4 rw = philox4x32(g, 0x80000000u + 2 * half + 1, i, 0u, seed, kDomPlanted);
+      const uint64_t w = 1 + (((uint64_t)wmax * rw.x) >> 32);
+      delta[half] += 2 * w;
+      for (uint32_t v = 0; v < h; ++v) perm[v] = v;
+      for (uint32_t j = h - 1; j > 0; --j) {  // Fisher-Yates: position j takes one of [0, j]
+        const u32x4 r = philox4x32(g, 0x80000000u + 2 * half, i, j, seed, kDomPlanted);
+        std::swap(perm[j], perm[(uint32_t)(((uint64_t)(j + 1) * r.x) >> 32)]);
+      }
+      for (uint32_t j = 0; j < h; ++j) {
+        const uint32_t a = base + perm[j], b = base + perm[(j + 1) % h];
+        out.push_back(cac_edge{std::min(a, b), std::max(a, b), w});
+      }
+    }
+  }
+  const u32x4 rc = philox4x32(g, 0xFFFFFFFEu, 0x80000000u, 0u, seed, kDomPlanted);
+  const uint32_t c = 1 + (uint32_t)(((uint64_t)cmax * rc.x) >> 32);
+  const uint64_t C = (uint64_t)std::floor(cut_frac * (double)std::min(delta[0], delta[1]));
+  if (C < c || C >= std::min(delta[0], delta[1])) return false;
+  const size_t cross0 = out.size();
+  for (uint32_t j = 0; out.size() - cross0 < c; ++j) {
+    const u32x4 r = philox4x32(g, 0xFFFFFFFDu, 0x80000000u, j, seed, kDomPlanted);
+    const uint32_t x = (uint32_t)(((uint64_t)h * r.x) >> 32), y = h + (uint32_t)(((uint64_t)h * r.y) >> 32);
+    bool dup = false;
+    for (size_t q = cross0; q < out.size(); ++q) dup = dup || (out[q].u == x && out[q].v == y);
+    if (!dup) out.push_back(cac_edge{x, y, 0});
+  }
+  for (uint32_t j = 0; j < c; ++j) out[cross0 + j].w = C / c + (j < C % c ? 1 : 0);
+  *C_out = (uint32_t)C;
+  return true;
+}
+
 }  // namespace
 }  // namespace cac
 
 extern "C" {
+
+int cac_gen_planted_hard(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
+                         uint32_t cycles, uint32_t wmax, uint32_t cmax, double cut_frac, cac_graph_desc* descs,
+                         cac_edge* edges, uint64_t edge_cap, uint64_t* m_total, uint32_t* planted) {
+  if (!descs || !edges || !m_total || !planted || nmin < 6 || nmax < nmin || cycles == 0 || wmax == 0 ||
+      cmax == 0 || !(cut_frac > 0.0 && cut_frac < 1.0) || (uint64_t)wmax * cycles * 2 >= (1ull << 31))
+    return CAC_EINVAL;
+  std::vector<std::vector<cac_edge>> per(count);
+  std::vector<uint32_t> ns(count), cs(count);
+  std::atomic<uint32_t> next{0};
+  std::atomic<int> failed{0};
+  const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  auto work = [&]() {
+    for (uint32_t i; (i = next.fetch_add(1)) < count;) {
+      ns[i] = cac::planted_n(first + i, seed, nmin, nmax);
+      if (!cac::planted_hard_graph(first + i, seed, ns[i], cycles, wmax, cmax, cut_frac, per[i], &cs[i])) failed = 1;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nth; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (failed) return CAC_EINVAL;
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    descs[i] = cac_graph_desc{off, (uint64_t)per[i].size(), ns[i], 0u};
+    planted[i] = cs[i];
+    off += per[i].size();
+  }
+  *m_total = off;
+  if (off > edge_cap) return CAC_ERANGE;
+  for (uint32_t i = 0; i < count; ++i) std::copy(per[i].begin(), per[i].end(), edges + descs[i].edge_offset);
+  return CAC_OK;
+}
 
 int cac_gen_planted_sizes(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
                           double avg_degree, uint32_t cmax, uint32_t* n_out, uint64_t* edge_bound) {
@@ -203,14 +290,14 @@ int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_
     first_slot[g] = jobs.size();
     if (jobs.size() + T > 0xFFFFFFFFull) return cac::fail(ctx, CAC_EINVAL, "too many trials in one batch");
     for (uint32_t i = 0; i < T; ++i)
-      jobs.push_back({d_edges + d.edge_offset, d.m, d.n, t, i, (uint32_t)jobs.size()});
+      jobs.push_back({d_edges + d.edge_offset, d.m, d.n, t, i, (uint32_t)jobs.size(), seed});
     if (trials_out) trials_out[g] = T;
   }
   first_slot[count] = jobs.size();
   hipStream_t s = cac::pick_stream(ctx, stream);
   int64_t* d_res = nullptr;
   CAC_HIP(ctx, hipMalloc((void**)&d_res, 8 * jobs.size()));
-  rc = cac::sqrt_cut_batch(ctx, jobs, seed, d_res, s);
+  rc = cac::sqrt_cut_batch(ctx, jobs, d_res, s);
   if (rc == CAC_OK) {
     std::vector<int64_t> res(jobs.size());
     hipError_t e = hipMemcpyAsync(res.data(), d_res, 8 * res.size(), hipMemcpyDeviceToHost, s);

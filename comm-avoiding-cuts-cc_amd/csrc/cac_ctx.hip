@@ -267,23 +267,40 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 }
 
 // Tuning table: CAC_TUNE_KS_CONTRACT_BLOCK keys 512, 1024, 2048, 4096 (row-size
-// class upper bounds); a value is 0 (built-in rule) or a power of two from
-// key / 8 (at least 64) to 1024.
+// class upper bounds), a value 0 (built-in rule) or a power of two from key / 8
+// (at least 64) to 1024; CAC_TUNE_SAMPLE_OCCUPANCY keys 16, 20, 24, 32 (log2 of
+// the edge-count class bound), a value 0 (built-in: 2), 1 or 2.
 static int tune_slot(int knob, uint32_t key) {
-  if (knob != CAC_TUNE_KS_CONTRACT_BLOCK) return -1;
-  switch (key) {
-    case 512: return 0;
-    case 1024: return 1;
-    case 2048: return 2;
-    case 4096: return 3;
-    default: return -1;
+  if (knob == CAC_TUNE_KS_CONTRACT_BLOCK) {
+    switch (key) {
+      case 512: return 0;
+      case 1024: return 1;
+      case 2048: return 2;
+      case 4096: return 3;
+      default: return -1;
+    }
   }
+  if (knob == CAC_TUNE_SAMPLE_OCCUPANCY) {
+    switch (key) {
+      case 16: return 0;
+      case 20: return 1;
+      case 24: return 2;
+      case 32: return 3;
+      default: return -1;
+    }
+  }
+  return -1;
 }
 
 int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value) {
   if (!ctx) return CAC_EINVAL;
   const int slot = tune_slot(knob, key);
   if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
+  if (knob == CAC_TUNE_SAMPLE_OCCUPANCY) {
+    if (value < 0 || value > 2) return cac::fail(ctx, CAC_EINVAL, "sampling workgroups per CU must be 0, 1 or 2");
+    ctx->sample_occupancy[slot] = value;
+    return CAC_OK;
+  }
   const int32_t lo = key / 8 < 64 ? 64 : (int32_t)(key / 8);
   if (value != 0 && (value < lo || value > 1024 || (value & (value - 1))))
     return cac::fail(ctx, CAC_EINVAL, "block size must be 0 or a power of two from key/8 (at least 64) to 1024");
@@ -295,7 +312,7 @@ int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value) {
   if (!ctx || !value) return CAC_EINVAL;
   const int slot = tune_slot(knob, key);
   if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
-  *value = ctx->ks_contract_block[slot];
+  *value = knob == CAC_TUNE_SAMPLE_OCCUPANCY ? ctx->sample_occupancy[slot] : ctx->ks_contract_block[slot];
   return CAC_OK;
 }
 

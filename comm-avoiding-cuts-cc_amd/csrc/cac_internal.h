@@ -56,6 +56,11 @@ struct cac_ctx {
   // Tuned launch shapes (cac_tune_set): dense Karger-Stein contraction block
   // size per row-size class, keys 512, 1024, 2048, 4096; 0 = built-in rule.
   int32_t ks_contract_block[4] = {0, 0, 0, 0};
+  // Sampling rule of the square-root cut's shrink phase (cac_set_sampling_rule).
+  int sample_rule = CAC_SAMPLE_LC;
+  // Sampling workgroups per CU for launches of more trials than CUs, by edge
+  // count class m < 2^16, 2^20, 2^24, larger (cac_tune_set); 0 = built-in (2).
+  int32_t sample_occupancy[4] = {0, 0, 0, 0};
   // The captured CSR connected-components step (cac_cc_csr_graph_dev) and the
   // arguments it was captured for: offsets, columns, labels, scratch, stream,
   // n, label mode.

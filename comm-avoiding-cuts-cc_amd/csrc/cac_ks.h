@@ -11,12 +11,13 @@ namespace cac {
 
 // One trial's root: a dense n x n device matrix (row stride n) with its row
 // sums, and where its raw cut goes.  Several roots may share one matrix
-// (independent trials).
+// (independent trials).  Its draws are keyed by (seed, trial, node id).
 struct KsRoot {
   const int64_t* mat;
   const int64_t* rsum;
   uint32_t n, trial, slot;
   int64_t bound;     // the trial's upper bound (minimum weighted degree)
+  uint32_t seed;     // Philox key word 0 of the trial
   uint32_t nid = 1;  // recursion node id of the matrix (1: the trial's root)
 };
 
@@ -24,9 +25,9 @@ struct KsRoot {
 int ks_row_sums(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_rsum, int64_t* d_min, hipStream_t s);
 // Upper estimate of the device bytes one root needs for one level of children.
 size_t ks_level_bytes_per_root(uint32_t n);
-// Runs one trial per root; d_results[slot] (initialised to the root's bound)
-// receives min(bound, leaves).
-int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s);
+// Runs one trial per root (roots of different seeds share launches);
+// d_results[slot] (initialised to the root's bound) receives min(bound, leaves).
+int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, hipStream_t s);
 
 // Host forms of the reference's size and trial-count rules (same floating point).
 uint32_t ks_target_host(uint32_t s);
@@ -34,19 +35,23 @@ uint32_t ks_trials_host(uint32_t n, double p);
 uint32_t sqrt_intermediate_size(uint32_t n, uint64_t m);
 uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p);
 // One sqrt-cut job: trial 'trial' of the graph (edges, m, n) shrunk to t
-// super-vertices, raw cut to d_results[slot].
+// super-vertices with the draws of Philox key `seed`, raw cut to
+// d_results[slot].  Jobs of different graphs and seeds share launches.
 struct SqrtJob {
   const cac_edge* edges;
   uint64_t m;
-  uint32_t n, t, trial, slot;
+  uint32_t n, t, trial, slot, seed;
 };
 constexpr size_t kMaxJobsPerChunk = 8192;
 // Jobs of any mix of graphs, in chunks of shared launches whose estimated
 // device bytes stay within budget (0: 40% of the free device memory).
-int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s,
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
                    size_t budget = 0);
-// Estimated device bytes of one job: sampling scratch, Karger-Stein levels.
-size_t job_sample_bytes(const SqrtJob& j);
+// Estimated device bytes of one job: sampling scratch (dpr: draws per round,
+// 0 = |V|), Karger-Stein levels.
+size_t job_sample_bytes(const SqrtJob& j, uint32_t dpr = 0);
+// Draws per round of the high-concurrency sampling rule for n0 vertices.
+uint32_t hc_draws_host(uint32_t n0);
 size_t job_ks_bytes(const SqrtJob& j);
 // Sqrt-cut trials on a device edge list: raw cut of trial_ids[i] to d_results[i].
 int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
@@ -62,9 +67,9 @@ struct SparseRoot {
   uint64_t m;
   uint32_t n, trial, slot;
   int64_t bound;
+  uint32_t seed;
 };
-int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t seed, int64_t* d_results,
-                    hipStream_t s);
+int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, int64_t* d_results, hipStream_t s);
 int ks_check_matrix(cac_ctx* ctx, const int64_t* adj, uint32_t n);
 
 // One sampling contraction (cac_sample_contract): canonical labels and the merged

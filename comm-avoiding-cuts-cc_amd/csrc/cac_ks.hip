@@ -431,7 +431,7 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
 // per CU for the mid-sized children.
 template <int B>
 __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
-    k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t seed, uint32_t* __restrict__ k_out,
+    k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t* __restrict__ k_out,
                   int64_t* __restrict__ results) {
   extern __shared__ __align__(16) unsigned char smem[];
   const KsChild c = children[blockIdx.x];
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
       status = 1;
       break;
     }
-    const u32x4 rnd = philox4x32(c.trial, c.nid, p, 0u, seed, kDomKs);
+    const u32x4 rnd = philox4x32(c.trial, c.nid, p, 0u, c.seed, kDomKs);
     if (wave == 0) {
       const uint32_t i = wave_select_two_level(L.cap, L.cbs, s, nb, uniform1(lo64(rnd), W));
       if (lane == 0) L.misc[0] = i;
@@ -759,18 +759,18 @@ int run_leaves(cac_ctx* ctx, const std::vector<KsLeaf>& leaves, int buf, int64_t
 }
 
 template <int B>
-void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, KsPool pool, uint32_t seed,
-                     uint32_t* d_kout, int64_t* d_results, hipStream_t s) {
-  hipLaunchKernelGGL(k_ks_contract<B>, dim3(count), dim3(B), contract_lds_bytes(smax, B), s, d_children, pool, seed,
-                     d_kout, d_results);
+void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, KsPool pool, uint32_t* d_kout,
+                     int64_t* d_results, hipStream_t s) {
+  hipLaunchKernelGGL(k_ks_contract<B>, dim3(count), dim3(B), contract_lds_bytes(smax, B), s, d_children, pool, d_kout,
+                     d_results);
 }
 
 // One level's launches for the children ch (already in d_children's order, by
 // block-size class): contraction per class, then compaction; waits and returns
 // the children's k_out.
 int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_children, uint32_t* d_prefix,
-                   uint32_t* d_kout, unsigned long long* d_used, KsPool pool, uint32_t max_tgt, uint32_t seed,
-                   int64_t* d_results, std::vector<uint32_t>& kout, hipStream_t s) {
+                   uint32_t* d_kout, unsigned long long* d_used, KsPool pool, uint32_t max_tgt, int64_t* d_results,
+                   std::vector<uint32_t>& kout, hipStream_t s) {
   const size_t nchild = ch.size();
   std::vector<uint32_t> item_prefix(nchild + 1);
   uint32_t items = 0;
@@ -795,11 +795,11 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
     // row scans (ER(1000, 0.05) square_root: contraction 1.5 -> 1.33 ms per
     // solve).  With many children the narrow block keeps occupancy up.
     if (cnt < kFewChildren && B < 256) B = 256;
-    if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-    else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-    else if (B == 256) launch_contract<256>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-    else if (B == 128) launch_contract<128>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
-    else launch_contract<64>(d_children + c0, cnt, smax, pool, seed, d_kout + c0, d_results, s);
+    if (B == 1024) launch_contract<1024>(d_children + c0, cnt, smax, pool, d_kout + c0, d_results, s);
+    else if (B == 512) launch_contract<512>(d_children + c0, cnt, smax, pool, d_kout + c0, d_results, s);
+    else if (B == 256) launch_contract<256>(d_children + c0, cnt, smax, pool, d_kout + c0, d_results, s);
+    else if (B == 128) launch_contract<128>(d_children + c0, cnt, smax, pool, d_kout + c0, d_results, s);
+    else launch_contract<64>(d_children + c0, cnt, smax, pool, d_kout + c0, d_results, s);
     c0 = c1;
   }
   prof_end(ctx, s);
@@ -840,7 +840,7 @@ size_t ks_level_bytes_per_root(uint32_t n) {
 // Runs one Karger-Stein trial per root.  roots[i]: device matrix (n_i x n_i)
 // with its row sums, trial id, result slot and bound; d_results[slot] must hold
 // the bound already and receives min(bound, every leaf) by atomicMin.
-int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_t* d_results, hipStream_t s) {
+int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, hipStream_t s) {
   if (roots.empty()) return CAC_OK;
   struct Node {
     const int64_t* mat;
@@ -848,6 +848,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     const int64_t* rmax;
     uint32_t s, trial, nid, slot;
     int64_t bound;
+    uint32_t seed;
   };
   static std::once_flag lds_attr;
   hipError_t attr_err = hipSuccess;
@@ -865,7 +866,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
   std::vector<Node> level;
   for (const KsRoot& r : roots) {
     if (r.n > kKsMaxS) return fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
-    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, r.nid, r.slot, r.bound});
+    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, r.nid, r.slot, r.bound, r.seed});
   }
   int buf = 0;  // level buffer for the children of this level; the parents are in the other one
   while (!level.empty()) {
@@ -929,13 +930,13 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
         x.trial = nd.trial;
         x.nid = 2 * nd.nid + k;
         x.slot = nd.slot;
-        x.pad = 0;
+        x.seed = nd.seed;
         x.bound = nd.bound;
       }
     }
     cv.off = (cv.off + 511) & ~size_t(511);
     KsPool pool{(int64_t*)(cv.base + cv.off), d_used, pool_units};
-    rc = contract_level(ctx, ch, d_children, d_prefix, d_kout, d_used, pool, max_tgt, seed, d_results, kout, s);
+    rc = contract_level(ctx, ch, d_children, d_prefix, d_kout, d_used, pool, max_tgt, d_results, kout, s);
     if (rc) return rc;
     // children that ran out of pool rows: again, with s pool rows each (a child
     // merges at most s - 2 times, so this cannot overflow)
@@ -962,7 +963,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
       rv.off = (rv.off + 511) & ~size_t(511);
       KsPool rpool{(int64_t*)(rv.base + rv.off), d_rused, runits};
       std::vector<uint32_t> rkout(no);
-      rc = contract_level(ctx, rch, d_rch, d_rprefix, d_rkout, d_rused, rpool, max_tgt, seed, d_results, rkout, s);
+      rc = contract_level(ctx, rch, d_rch, d_rprefix, d_rkout, d_rused, rpool, max_tgt, d_results, rkout, s);
       if (rc) return rc;
       for (size_t k = 0; k < no; ++k) {
         if (rkout[k] == kKsOverflow) return fail(ctx, CAC_EDEVICE, "Karger-Stein row pool overflow");
@@ -973,7 +974,8 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, uint32_t seed, int64_
     for (size_t i = 0; i < nchild; ++i) {
       if (kout[i] == kKsBad) return fail(ctx, CAC_EDEVICE, "Karger-Stein selection out of range");
       if (kout[i] == 0) continue;  // disconnected child: its 0 is already recorded
-      next.push_back({ch[i].out, ch[i].orsum, ch[i].ormax, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound});
+      next.push_back(
+          {ch[i].out, ch[i].orsum, ch[i].ormax, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound, ch[i].seed});
     }
     level.swap(next);
     buf ^= 1;
@@ -1027,8 +1029,8 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
     for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
       std::vector<KsRoot> roots;
       for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
-        roots.push_back({d_adj, d_rsum, n, first_trial + t, t, bound});
-      rc = ks_run(ctx, roots, seed, d_res, s);
+        roots.push_back({d_adj, d_rsum, n, first_trial + t, t, bound, seed});
+      rc = ks_run(ctx, roots, d_res, s);
     }
   }
   if (rc == CAC_OK) {

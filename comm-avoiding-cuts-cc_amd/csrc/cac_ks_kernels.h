@@ -46,7 +46,7 @@ struct KsChild {
   uint32_t* colmap;       // [s] column -> output index
   uint32_t* live;         // [tgt] output row -> parent row, or kLiveWork | pool unit
   uint32_t s, tgt, trial, nid;
-  uint32_t slot, pad;
+  uint32_t slot, seed;  // seed: Philox key word 0 of the child's trial
   int64_t bound;
 };
 

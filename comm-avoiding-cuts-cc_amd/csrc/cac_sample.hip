@@ -28,19 +28,20 @@
 
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
 
 #include "cac_common.h"
 #include "cac_internal.h"
 #include "cac_ks.h"
 #include "cac_ks_kernels.h"
 #include "cac_scan.h"
+#include "cac_sort.h"
 
 namespace cac {
 namespace {
 
 constexpr int kSmpBlock = 1024;
 constexpr int kSmpWaves = kSmpBlock / 64;
+constexpr int kSmpIndex = 4096;  // LDS segment index of the weight prefix (32 KiB)
 
 // One sampling job = one sqrt-cut trial of one graph; a launch runs one
 // workgroup per job, so trials of different graphs share a launch.
@@ -49,6 +50,8 @@ struct TrialScratch {
   uint64_t m;
   uint32_t n, t;          // vertices, intermediate size
   uint32_t trial, slot;   // Philox trial id, result slot
+  uint32_t seed;          // Philox key word 0 of the trial
+  uint32_t dpr;           // draws per round: 0 = |V| (low-concurrency rule), else that many
   uint32_t* lab;          // [n]
   uint32_t* par;          // [n]
   uint64_t* pre;          // [m]
@@ -152,12 +155,18 @@ __device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t undo, uint32_
 // small launches whose trials each have a CU to themselves.
 template <int kMinWaves>
 __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
-    k_sample_trials(uint32_t seed, const TrialScratch* __restrict__ scratch) {
+    k_sample_trials(const TrialScratch* __restrict__ scratch) {
   __shared__ uint64_t lds[kSmpWaves + 2];
+  // Every G-th prefix value (the last of each segment of G edges): a draw
+  // finds its segment here, then searches only that segment in HBM.
+  __shared__ uint64_t seg_last[kSmpIndex];
   const TrialScratch S = scratch[blockIdx.x];
   const cac_edge* __restrict__ edges = S.edges;
   const uint64_t m = S.m;
-  const uint32_t n = S.n, t = S.t, trial = S.trial;
+  uint32_t gshift = 0;  // segments of 2^gshift edges, at most kSmpIndex of them
+  while (((m + (1ull << gshift) - 1) >> gshift) > (uint64_t)kSmpIndex) ++gshift;
+  const uint32_t nseg = (uint32_t)((m + (1ull << gshift) - 1) >> gshift);
+  const uint32_t n = S.n, t = S.t, trial = S.trial, seed = S.seed;
   for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) {
     S.lab[v] = v;
     S.par[v] = v;
@@ -218,6 +227,7 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
         if (e < m) {
           if (narrow) pre32[e] = (uint32_t)run;
           else S.pre[e] = run;
+          if ((((e + 1) >> gshift) << gshift) == e + 1 || e + 1 == m) seg_last[e >> gshift] = run;
         }
       }
       carry += tot;
@@ -228,11 +238,23 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
     prev_w = W;
     if (W == 0) break;  // only loops left: more than t components
     // 2. draws
-    const uint32_t D = nv;
+    const uint32_t D = S.dpr ? S.dpr : nv;
     for (uint32_t d = threadIdx.x; d < D; d += kSmpBlock) {
       const u32x4 rnd = philox4x32(trial, round, d, 0u, seed, kDomSample);
       const uint64_t r = uniform1(lo64(rnd), W);
-      uint64_t lo = flat ? (r - 1) / wu : 0, hi = flat ? lo : m - 1;
+      uint64_t lo = 0, hi = 0;
+      if (flat) {
+        lo = hi = (r - 1) / wu;
+      } else {  // the first segment whose last prefix reaches r, then inside it
+        uint32_t a = 0, b = nseg - 1;
+        while (a < b) {
+          const uint32_t mid = (a + b) >> 1;
+          if (seg_last[mid] >= r) b = mid;
+          else a = mid + 1;
+        }
+        lo = (uint64_t)a << gshift;
+        hi = min(lo + (1ull << gshift), m) - 1;
+      }
       while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if ((narrow ? (uint64_t)pre32[mid] : S.pre[mid]) >= r) hi = mid;
@@ -394,29 +416,45 @@ struct LabRank {
   }
 };
 
-// key (a << 32) | b of the ranked endpoints (a < b), ~0 for a loop; value w.
+// Sort keys of one job's edges: (job << 2 bk) | (a << bk) | b of the ranked
+// endpoints a < b, the low 2 bk bits all ones for a loop (dropped after the
+// sort); value w.
 __global__ void __launch_bounds__(256) k_lt_keys(const cac_edge* __restrict__ edges, uint64_t m,
                                                  const uint32_t* __restrict__ lab, const uint32_t* __restrict__ rank,
-                                                 uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
+                                                 uint64_t job_hi, uint32_t bk, uint64_t* __restrict__ keys,
+                                                 uint64_t* __restrict__ vals) {
+  const uint64_t loop = (1ull << (2 * bk)) - 1;
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < m; e += (uint64_t)gridDim.x * 256) {
     const cac_edge x = edges[e];
     const uint32_t a = rank[lab[x.u]], b = rank[lab[x.v]];
-    keys[e] = a == b ? ~0ull : ((uint64_t)min(a, b) << 32) | max(a, b);
+    keys[e] = job_hi | (a == b ? loop : ((uint64_t)min(a, b) << bk) | max(a, b));
     vals[e] = x.w;
   }
 }
 
-// Merged keys to edges, and the weighted degrees.
-__global__ void __launch_bounds__(256) k_lt_edges(const uint64_t* __restrict__ ukeys, const uint64_t* __restrict__ sums,
-                                                  uint64_t count, cac_edge* __restrict__ out,
-                                                  unsigned long long* __restrict__ deg) {
+// Merged (key, weight) pairs to cac_edge records in place (the 16-byte layouts
+// coincide: the key's 8 bytes become u, v), and every job's weighted degrees.
+__global__ void __launch_bounds__(256) k_lt_edges(uint64_t* __restrict__ pairs, uint64_t count, uint32_t bk,
+                                                  unsigned long long* const* __restrict__ deg) {
+  const uint64_t low = (1ull << bk) - 1;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) {
-    const uint64_t k = ukeys[i];
-    const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
-    out[i] = cac_edge{a, b, sums[i]};
-    atomicAdd(&deg[a], (unsigned long long)sums[i]);
-    atomicAdd(&deg[b], (unsigned long long)sums[i]);
+    const uint64_t k = pairs[2 * i], w = pairs[2 * i + 1];
+    const uint32_t a = (uint32_t)((k >> bk) & low), b = (uint32_t)(k & low);
+    reinterpret_cast<cac_edge*>(pairs)[i] = cac_edge{a, b, w};
+    unsigned long long* d = deg[k >> (2 * bk)];
+    atomicAdd(&d[a], (unsigned long long)w);
+    atomicAdd(&d[b], (unsigned long long)w);
   }
+}
+
+// firsts[j] = head_idx[offsets[j]]: where each job's merged list starts.
+__global__ void k_lt_firsts(const uint32_t* __restrict__ head_idx, const uint64_t* __restrict__ offsets, uint32_t count,
+                            uint32_t* __restrict__ firsts) {
+  for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) firsts[j] = head_idx[offsets[j]];
+}
+
+__global__ void __launch_bounds__(256) k_fill_u64(uint64_t* p, uint64_t count, uint64_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256) p[i] = v;
 }
 
 __global__ void __launch_bounds__(256) k_lt_zero(unsigned long long* p, uint64_t count) {
@@ -503,17 +541,23 @@ size_t sampling_wave(cac_ctx* ctx) {
 }
 
 
-// One sampling launch of nb trials, in the form suited to its size.
-void launch_sampling(cac_ctx* ctx, size_t nb, uint32_t seed, const TrialScratch* d_scr, hipStream_t s) {
+// One sampling launch of nb trials, in the form suited to its size: a launch
+// of more trials than CUs runs two 1024-thread workgroups per CU (64 VGPRs,
+// some spilled) unless the context's tuning table says one for the largest
+// edge count mmax of the launch; a smaller launch gives each trial a CU of its
+// own with the registers free.  The draws do not depend on the form.
+void launch_sampling(cac_ctx* ctx, size_t nb, uint64_t mmax, const TrialScratch* d_scr, hipStream_t s) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
     (void)hipGetLastError();
     cus = 1;
   }
-  if (nb > (size_t)cus)
-    hipLaunchKernelGGL(k_sample_trials<8>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, d_scr);
+  const int cls = mmax < (1ull << 16) ? 0 : mmax < (1ull << 20) ? 1 : mmax < (1ull << 24) ? 2 : 3;
+  const int per_cu = ctx->sample_occupancy[cls] ? ctx->sample_occupancy[cls] : 2;
+  if (nb > (size_t)cus && per_cu == 2)
+    hipLaunchKernelGGL(k_sample_trials<8>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, d_scr);
   else
-    hipLaunchKernelGGL(k_sample_trials<1>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, seed, d_scr);
+    hipLaunchKernelGGL(k_sample_trials<1>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, d_scr);
 }
 
 }  // namespace
@@ -547,32 +591,46 @@ uint32_t sqrt_trials_host(uint32_t n, uint64_t m, double p) {
   return (uint32_t)std::ceil((n * (n * std::log(cp))) / (2.0 * 2.0 * ((uint32_t)m * pk)));
 }
 
-// Device bytes one sampling job needs beside the Karger-Stein levels.
-size_t job_sample_bytes(const SqrtJob& j) {
-  return carve_size((size_t)j.n * 4) * 2 + carve_size(std::max<uint64_t>(j.m, 1) * 8) + carve_size((size_t)j.n * 8) +
-         carve_size((size_t)j.t * j.t * 8) + carve_size((size_t)j.t * 8) + 256;
+// Draws per round of the high-concurrency rule: (unsigned) pow((float) n0,
+// 1 + epsilon / 2) with epsilon = 0.1f, in float like
+// WeightedIteratedSparseSampling::edgesToSamplePerProcessor
+// (src/WeightedIteratedSparseSampling.hpp:66-68, src/IteratedSparseSampling.hpp:38).
+uint32_t hc_draws_host(uint32_t n0) {
+  const float epsilon = 0.1f;
+  return (uint32_t)std::pow((float)n0, 1 + epsilon / 2);
+}
+
+// Draws per round of a job under the context's sampling rule (0: |V|).
+static uint32_t job_dpr(const cac_ctx* ctx, uint32_t n) {
+  return ctx->sample_rule == CAC_SAMPLE_HC ? hc_draws_host(n) : 0u;
+}
+
+// Device bytes one sampling job needs beside the Karger-Stein levels (dpr:
+// draws per round, 0 = |V|).
+size_t job_sample_bytes(const SqrtJob& j, uint32_t dpr) {
+  return carve_size((size_t)j.n * 4) * 2 + carve_size(std::max<uint64_t>(j.m, 1) * 8) +
+         carve_size((size_t)std::max(j.n, dpr) * 8) + carve_size((size_t)j.t * j.t * 8) + carve_size((size_t)j.t * 8) +
+         256;
 }
 // Upper estimate of the Karger-Stein level arenas of one root of size t: each
 // level holds about t^2 entries in copy-on-write work rows plus the contracted
 // children, and two levels are alive at once.
 size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t); }
 
-// Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
-// one sampling launch, one densify launch and one breadth-first Karger-Stein
-// pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
-// i goes to d_results[jobs[i].slot].
-// Trials with t > 8192, in chunks sized to the free memory: one sampling launch
-// for the chunk, then per trial the merged contracted edge list (rank, keys,
-// radix sort, reduce by key; the sort buffers are shared) and its bound, then
+// Trials with t > 8192, in chunks sized to the free memory: one sampling
+// launch for the chunk; then the contracted edge lists of all its trials at
+// once -- endpoints ranked per trial, keys (trial, a, b), one radix sort of
+// every trial's keys together, one segmented sum of parallel weights, the
+// merged lists and their weighted degrees -- with two host synchronisations
+// per chunk (the super-vertex counts, then the list ranges and bounds); then
 // one sparse Karger-Stein pass over all of the chunk's trials (node 1 each).
 // Raw cut of each job to d_results[slot].
-static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results,
-                            hipStream_t s) {
-  auto job_bytes = [](const SqrtJob& j) {
+static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s) {
+  auto job_bytes = [ctx](const SqrtJob& j) {
     const uint64_t m = std::max<uint64_t>(j.m, 1);
     return carve_size(sizeof(TrialScratch)) + 3 * carve_size((size_t)j.n * 4) + carve_size(m * 8) +
-           carve_size((size_t)j.n * 8) + carve_size(64) + carve_size(m * sizeof(cac_edge)) +
-           carve_size(((size_t)j.t + 2) * 8) + carve_size(sizeof(EdgeSpan)) + carve_size(4);
+           carve_size((size_t)std::max(j.n, job_dpr(ctx, j.n)) * 8) + carve_size(64) + carve_size(((size_t)j.t + 2) * 8) +
+           carve_size(sizeof(EdgeSpan)) + carve_size(4) + 64 * m;  // 64 m: the shared merge buffers
   };
   size_t avail = 0;
   int rc = device_memory_available(ctx, &avail);
@@ -581,52 +639,49 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
   const size_t budget = (size_t)(0.35 * (double)avail);
   size_t b0 = 0;
   while (b0 < jobs.size()) {
-    // the chunk: shared sort buffers for its largest graph, plus per-job scratch
-    size_t nb = 0, per = 0, shared = 0;
-    uint64_t mmax = 1, nmaxs = 1;
-    size_t sort_tmp = 0, red_tmp = 0;
+    size_t nb = 0, per = 0;
+    uint64_t N = 0, nmax = 1;
+    uint32_t tmax = 2;
     while (b0 + nb < jobs.size()) {
       const SqrtJob& j = jobs[b0 + nb];
-      const uint64_t m2 = std::max<uint64_t>(mmax, std::max<uint64_t>(j.m, 1));
-      uint32_t bk = 1;
-      while (bk < 32 && (1ull << bk) < j.t) ++bk;
-      size_t st = 0, rt = 0;
-      rocprim::double_buffer<uint64_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
-      CAC_HIP(ctx, rocprim::radix_sort_pairs(nullptr, st, kb, vb, m2, 0, 32 + bk, s));
-      CAC_HIP(ctx, rocprim::reduce_by_key(nullptr, rt, (const uint64_t*)nullptr, (const uint64_t*)nullptr, m2,
-                                          (uint64_t*)nullptr, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                          rocprim::plus<uint64_t>(), rocprim::equal_to<uint64_t>(), s));
-      const size_t sh = 6 * carve_size(m2 * 8) + carve_size(std::max(std::max(st, rt), std::max(sort_tmp, red_tmp))) +
-                        carve_size((scan_tiles(std::max<uint64_t>(nmaxs, j.n)) + 64) * 4) + 4096;
       const size_t add = job_bytes(j);
-      if (nb > 0 && per + add + sh > budget) break;
+      if (nb > 0 && (per + add > budget || N + j.m >= 0xFFFFFFF0ull || nb + 1 > 4096)) break;
       per += add;
-      shared = sh;
-      mmax = m2;
-      nmaxs = std::max<uint64_t>(nmaxs, j.n);
-      sort_tmp = std::max(sort_tmp, st);
-      red_tmp = std::max(red_tmp, rt);
+      N += j.m;
+      nmax = std::max<uint64_t>(nmax, j.n);
+      tmax = std::max(tmax, j.t);
       ++nb;
     }
+    uint32_t bk = 1;
+    while (bk < 32 && (1ull << bk) < tmax) ++bk;
+    uint32_t jb = 0;
+    while ((1ull << jb) < nb) ++jb;
+    const int key_bits = (int)(2 * bk + jb);
+    if (key_bits > 64) return fail(ctx, CAC_EINVAL, "contracted graph too large for 64-bit merge keys");
+    const uint64_t Np = (std::max<uint64_t>(N, 1) + 31) & ~uint64_t(31);
+    const size_t shared = carve_size(Np * 16) * 2 + carve_size((Np + 1) * 4) + carve_size(radix_sort_temp_bytes(Np)) +
+                          carve_size(segmented_sum_temp_bytes(Np)) + carve_size((scan_tiles(nmax) + 64) * 4) +
+                          carve_size((nb + 1) * 8) + carve_size((nb + 1) * 4) + carve_size(nb * 8) + 4096;
     rc = grow_buffer(ctx, &ctx->lt_buf, &ctx->lt_bytes, per + shared);
     if (rc) return rc;
     BufPin pin(ctx, kBufLt);  // the merged edge lists are the sparse roots
     Carver cv(ctx->lt_buf);
-    uint64_t* k0 = cv.take<uint64_t>(mmax);
-    uint64_t* k1 = cv.take<uint64_t>(mmax);
-    uint64_t* v0 = cv.take<uint64_t>(mmax);
-    uint64_t* v1 = cv.take<uint64_t>(mmax);
-    uint64_t* ukeys = cv.take<uint64_t>(mmax);
-    uint64_t* sums = cv.take<uint64_t>(mmax);
-    void* tmp = cv.take<unsigned char>(std::max(sort_tmp, red_tmp));
-    uint32_t* partial = cv.take<uint32_t>(scan_tiles(nmaxs) + 64);
+    uint64_t* buf_a = cv.take<uint64_t>(2 * Np);  // keys | values, then free for the merged pairs
+    uint64_t* buf_b = cv.take<uint64_t>(2 * Np);
+    uint32_t* head_idx = cv.take<uint32_t>(Np + 1);
+    void* sort_tmp = cv.take<unsigned char>(radix_sort_temp_bytes(Np));
+    void* seg_tmp = cv.take<unsigned char>(segmented_sum_temp_bytes(Np));
+    uint32_t* partial = cv.take<uint32_t>(scan_tiles(nmax) + 64);
+    uint64_t* d_offsets = cv.take<uint64_t>(nb + 1);
+    uint32_t* d_firsts = cv.take<uint32_t>(nb + 1);
+    unsigned long long** d_deg = cv.take<unsigned long long*>(nb);
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
     EdgeSpan* d_spans = cv.take<EdgeSpan>(nb);
     uint32_t* d_flags = cv.take<uint32_t>(nb);
     std::vector<TrialScratch> scr(nb);
     std::vector<uint32_t*> rank(nb);
-    std::vector<cac_edge*> merged(nb);
     std::vector<unsigned long long*> deg(nb);
+    std::vector<uint64_t> offsets(nb + 1, 0);
     for (size_t i = 0; i < nb; ++i) {
       const SqrtJob& j = jobs[b0 + i];
       TrialScratch& x = scr[i];
@@ -637,83 +692,109 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint
       x.t = j.t;
       x.trial = j.trial;
       x.slot = j.slot;
+      x.seed = j.seed;
+      x.dpr = job_dpr(ctx, j.n);
       x.lab = cv.take<uint32_t>(j.n);
       x.par = cv.take<uint32_t>(j.n);
       rank[i] = cv.take<uint32_t>(j.n);
       x.pre = cv.take<uint64_t>(std::max<uint64_t>(j.m, 1));
-      x.draws = cv.take<uint2>(j.n);
+      x.draws = cv.take<uint2>(std::max(j.n, x.dpr));
       x.nv_out = cv.take<uint32_t>(16);
-      merged[i] = cv.take<cac_edge>(std::max<uint64_t>(j.m, 1));
-      deg[i] = cv.take<unsigned long long>((size_t)j.t + 2);  // [t] degrees, [t] count, [t + 1] min
+      deg[i] = cv.take<unsigned long long>((size_t)j.t + 2);  // [t] degrees, [t + 1] their minimum
+      offsets[i + 1] = offsets[i] + j.m;
     }
     rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
     if (rc) return rc;
     CAC_HIP(ctx, hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s));
+    CAC_HIP(ctx, hipMemcpyAsync(d_offsets, offsets.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+    CAC_HIP(ctx, hipMemcpyAsync(d_deg, deg.data(), nb * sizeof(void*), hipMemcpyHostToDevice, s));
     prof_begin(ctx, "sample_trials", s);
-    launch_sampling(ctx, nb, seed, d_scr, s);
+    uint64_t mmax_jobs = 0;
+    for (size_t i = 0; i < nb; ++i) mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
+    launch_sampling(ctx, nb, mmax_jobs, d_scr, s);
     prof_end(ctx, s);
     CAC_LAUNCH_CHECK(ctx, "sample trials");
     std::vector<uint32_t> nv(nb);
     for (size_t i = 0; i < nb; ++i) CAC_HIP(ctx, hipMemcpyAsync(&nv[i], scr[i].nv_out, 4, hipMemcpyDeviceToHost, s));
     CAC_HIP(ctx, hipStreamSynchronize(s));
-    std::vector<SparseRoot> roots;
+    // keys of every job (a disconnected trial's edges all become loops: its
+    // cut of 0 needs no list)
+    prof_begin(ctx, "contracted_edges", s);
+    uint64_t* keys = buf_a;
+    uint64_t* vals = buf_a + Np;
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      const bool disconnected = nv[i] > j.t || nv[i] < 2;
+      if (!disconnected)
+        CAC_HIP(ctx, device_exclusive_scan<uint32_t>(LabLive{scr[i].lab}, LabRank{rank[i]}, j.n, partial,
+                                                     (uint32_t*)nullptr, s));
+      if (j.m == 0) continue;
+      if (disconnected) {  // all loops
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(j.m, 256, 8192)), dim3(256), 0, s, keys + offsets[i], j.m,
+                           ((uint64_t)i << (2 * bk)) | ((1ull << (2 * bk)) - 1));
+        continue;
+      }
+      hipLaunchKernelGGL(k_lt_keys, dim3(grid_for(j.m, 256, 8192)), dim3(256), 0, s, j.edges, j.m,
+                         (const uint32_t*)scr[i].lab, (const uint32_t*)rank[i], (uint64_t)i << (2 * bk), bk,
+                         keys + offsets[i], vals + offsets[i]);
+    }
+    CAC_LAUNCH_CHECK(ctx, "merge keys");
+    bool in_alt = false;
+    rc = radix_sort_pairs(ctx, keys, vals, buf_b, buf_b + Np, N, key_bits, sort_tmp, &in_alt, s);
+    if (rc) return rc;
+    uint64_t* sk = in_alt ? buf_b : buf_a;
+    uint64_t* pairs = in_alt ? buf_a : buf_b;  // the free buffer takes the merged pairs
+    rc = segmented_sum(ctx, sk, sk + Np, N, (1ull << (2 * bk)) - 1, pairs, pairs + 1, 2, head_idx, seg_tmp, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lt_firsts, dim3(grid_for(nb + 1, 256, 64)), dim3(256), 0, s, (const uint32_t*)head_idx,
+                       (const uint64_t*)d_offsets, (uint32_t)(nb + 1), d_firsts);
+    for (size_t i = 0; i < nb; ++i)
+      hipLaunchKernelGGL(k_lt_zero, dim3(grid_for((uint64_t)jobs[b0 + i].t + 2, 256, 1024)), dim3(256), 0, s, deg[i],
+                         (uint64_t)jobs[b0 + i].t + 2);
+    std::vector<uint32_t> firsts(nb + 1);
+    CAC_HIP(ctx, hipMemcpyAsync(firsts.data(), d_firsts, (nb + 1) * 4, hipMemcpyDeviceToHost, s));
+    CAC_HIP(ctx, hipStreamSynchronize(s));
+    const uint64_t merged = firsts[nb];
+    if (merged)
+      hipLaunchKernelGGL(k_lt_edges, dim3(grid_for(merged, 256, 8192)), dim3(256), 0, s, pairs, merged, bk,
+                         (unsigned long long* const*)d_deg);
+    std::vector<unsigned long long> bounds(nb, 0);
     for (size_t i = 0; i < nb; ++i) {
       const SqrtJob& j = jobs[b0 + i];
       if (nv[i] > j.t || nv[i] < 2) {  // disconnected: a cut of weight 0 exists
-        const int64_t zero = 0;
-        CAC_HIP(ctx, hipMemcpyAsync(d_results + j.slot, &zero, 8, hipMemcpyHostToDevice, s));
-        CAC_HIP(ctx, hipStreamSynchronize(s));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)(d_results + j.slot), (uint64_t)1, 0ull);
         continue;
       }
-      const uint32_t k = nv[i];
-      uint32_t bk = 1;
-      while (bk < 32 && (1ull << bk) < j.t) ++bk;
-      prof_begin(ctx, "contracted_edges", s);
-      CAC_HIP(ctx, device_exclusive_scan<uint32_t>(LabLive{scr[i].lab}, LabRank{rank[i]}, j.n, partial,
-                                                   (uint32_t*)nullptr, s));
-      hipLaunchKernelGGL(k_lt_keys, dim3(grid_for(j.m, 256, 8192)), dim3(256), 0, s, j.edges, j.m,
-                         (const uint32_t*)scr[i].lab, (const uint32_t*)rank[i], k0, v0);
-      rocprim::double_buffer<uint64_t> kb(k0, k1), vb(v0, v1);
-      size_t st = sort_tmp, rt = red_tmp;
-      CAC_HIP(ctx, rocprim::radix_sort_pairs(tmp, st, kb, vb, j.m, 0, 32 + bk, s));
-      uint64_t* d_count = (uint64_t*)(deg[i] + j.t);
-      CAC_HIP(ctx, rocprim::reduce_by_key(tmp, rt, (const uint64_t*)kb.current(), (const uint64_t*)vb.current(), j.m,
-                                          ukeys, sums, d_count, rocprim::plus<uint64_t>(),
-                                          rocprim::equal_to<uint64_t>(), s));
-      uint64_t count = 0;
-      CAC_HIP(ctx, hipMemcpyAsync(&count, d_count, 8, hipMemcpyDeviceToHost, s));
-      CAC_HIP(ctx, hipStreamSynchronize(s));
-      uint64_t last = 0;
-      if (count) {
-        CAC_HIP(ctx, hipMemcpyAsync(&last, ukeys + count - 1, 8, hipMemcpyDeviceToHost, s));
-        CAC_HIP(ctx, hipStreamSynchronize(s));
-      }
-      const uint64_t mm = count && last == ~0ull ? count - 1 : count;  // the loop key sorts last
-      hipLaunchKernelGGL(k_lt_zero, dim3(grid_for((uint64_t)j.t + 2, 256, 1024)), dim3(256), 0, s, deg[i],
-                         (uint64_t)j.t + 2);
-      hipLaunchKernelGGL(k_lt_edges, dim3(grid_for(mm, 256, 8192)), dim3(256), 0, s, (const uint64_t*)ukeys,
-                         (const uint64_t*)sums, mm, merged[i], deg[i]);
       unsigned long long* d_min = deg[i] + j.t + 1;
-      const unsigned long long init = ~0ull;
-      CAC_HIP(ctx, hipMemcpyAsync(d_min, &init, 8, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_lt_min, dim3(grid_for(k, 256, 256)), dim3(256), 0, s, (const unsigned long long*)deg[i], k,
-                         d_min);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)d_min, (uint64_t)1, ~0ull);
+      hipLaunchKernelGGL(k_lt_min, dim3(grid_for(nv[i], 256, 256)), dim3(256), 0, s,
+                         (const unsigned long long*)deg[i], nv[i], d_min);
       hipLaunchKernelGGL(k_lt_set, dim3(1), dim3(64), 0, s, d_results + j.slot, (const unsigned long long*)d_min);
-      prof_end(ctx, s);
-      CAC_LAUNCH_CHECK(ctx, "contracted edge list");
-      unsigned long long bound = 0;
-      CAC_HIP(ctx, hipMemcpyAsync(&bound, d_min, 8, hipMemcpyDeviceToHost, s));
-      CAC_HIP(ctx, hipStreamSynchronize(s));
-      if (mm > 0) roots.push_back({merged[i], mm, k, j.trial, j.slot, (int64_t)bound});
+      CAC_HIP(ctx, hipMemcpyAsync(&bounds[i], d_min, 8, hipMemcpyDeviceToHost, s));
     }
-    rc = ks_sparse_roots(ctx, roots, seed, d_results, s);
+    prof_end(ctx, s);
+    CAC_LAUNCH_CHECK(ctx, "contracted edge lists");
+    CAC_HIP(ctx, hipStreamSynchronize(s));
+    std::vector<SparseRoot> roots;
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      const uint64_t mm = firsts[i + 1] - firsts[i];
+      if (nv[i] > j.t || nv[i] < 2 || mm == 0) continue;
+      roots.push_back({reinterpret_cast<cac_edge*>(pairs) + firsts[i], mm, nv[i], j.trial, j.slot,
+                       (int64_t)bounds[i], j.seed});
+    }
+    rc = ks_sparse_roots(ctx, roots, d_results, s);
     if (rc) return rc;
     b0 += nb;
   }
   return CAC_OK;
 }
 
-int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed, int64_t* d_results, hipStream_t s,
+// Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
+// one sampling launch, one densify launch and one breadth-first Karger-Stein
+// pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
+// i goes to d_results[jobs[i].slot].
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
                    size_t budget) {
   if (jobs.empty()) return CAC_OK;
   bool any_large = false;
@@ -721,9 +802,9 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
   if (any_large) {
     std::vector<SqrtJob> dense, large;
     for (const SqrtJob& j : jobs) (j.t <= kKsMaxS ? dense : large).push_back(j);
-    const int rc = sqrt_large_batch(ctx, large, seed, d_results, s);
+    const int rc = sqrt_large_batch(ctx, large, d_results, s);
     if (rc) return rc;
-    return sqrt_cut_batch(ctx, dense, seed, d_results, s, budget);
+    return sqrt_cut_batch(ctx, dense, d_results, s, budget);
   }
   if (budget == 0) {
     size_t avail = 0;
@@ -737,7 +818,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     size_t nb = 0, sample_bytes = 0, ks_bytes = 0;
     while (b0 + nb < jobs.size() && nb < kMaxJobsPerChunk) {
       const SqrtJob& j = jobs[b0 + nb];
-      const size_t a = job_sample_bytes(j), k = job_ks_bytes(j);
+      const size_t a = job_sample_bytes(j, job_dpr(ctx, j.n)), k = job_ks_bytes(j);
       if (nb > 0 && sample_bytes + a + ks_bytes + k > budget) break;
       sample_bytes += a;
       ks_bytes += k;
@@ -769,10 +850,12 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       x.t = j.t;
       x.trial = j.trial;
       x.slot = j.slot;
+      x.seed = j.seed;
+      x.dpr = job_dpr(ctx, j.n);
       x.lab = cv.take<uint32_t>(j.n);
       x.par = cv.take<uint32_t>(j.n);
       x.pre = cv.take<uint64_t>(std::max<uint64_t>(j.m, 1));
-      x.draws = cv.take<uint2>(j.n);
+      x.draws = cv.take<uint2>(std::max(j.n, x.dpr));
       x.mat = cv.take<int64_t>((size_t)j.t * j.t);
       x.rsum = cv.take<int64_t>(j.t);
       x.nv_out = d_nv + i;
@@ -783,7 +866,9 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
     if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
     if (rc == CAC_OK) {
       prof_begin(ctx, "sample_trials", s);
-      launch_sampling(ctx, nb, seed, d_scr, s);
+      uint64_t mmax_jobs = 0;
+      for (size_t i = 0; i < nb; ++i) mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
+      launch_sampling(ctx, nb, mmax_jobs, d_scr, s);
       prof_end(ctx, s);
       prof_begin(ctx, "densify_trials", s);
       rc = densify_attr(ctx);
@@ -806,8 +891,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
       std::vector<KsRoot> roots;
       for (size_t i = 0; i < nb; ++i)
         if (nvs[i] >= 2 && nvs[i] <= scr[i].t)
-          roots.push_back({scr[i].mat, scr[i].rsum, nvs[i], scr[i].trial, scr[i].slot, hb[i]});
-      rc = ks_run(ctx, roots, seed, d_results, s);
+          roots.push_back({scr[i].mat, scr[i].rsum, nvs[i], scr[i].trial, scr[i].slot, hb[i], scr[i].seed});
+      rc = ks_run(ctx, roots, d_results, s);
     }
     (void)hipStreamSynchronize(s);
     b0 += nb;
@@ -819,8 +904,8 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, uint32_t seed
 int sqrt_cut_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
                  const std::vector<uint32_t>& trial_ids, int64_t* d_results, hipStream_t s) {
   std::vector<SqrtJob> jobs(trial_ids.size());
-  for (size_t i = 0; i < trial_ids.size(); ++i) jobs[i] = {d_edges, m, n, t, trial_ids[i], (uint32_t)i};
-  return sqrt_cut_batch(ctx, jobs, seed, d_results, s);
+  for (size_t i = 0; i < trial_ids.size(); ++i) jobs[i] = {d_edges, m, n, t, trial_ids[i], (uint32_t)i, seed};
+  return sqrt_cut_batch(ctx, jobs, d_results, s);
 }
 
 }  // namespace cac
@@ -832,8 +917,9 @@ namespace cac {
 int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t t, uint32_t seed,
                         uint32_t trial, SampleOut* out, hipStream_t s) {
   *out = SampleOut{};
+  const uint32_t dpr = job_dpr(ctx, n);
   const size_t bytes = carve_size((size_t)n * 4) * 2 + carve_size(std::max<uint64_t>(m, 1) * 8) +
-                       carve_size((size_t)n * 8) + 4 * carve_size(64) + carve_size(sizeof(TrialScratch)) +
+                       carve_size((size_t)std::max(n, dpr) * 8) + 4 * carve_size(64) + carve_size(sizeof(TrialScratch)) +
                        carve_size(sizeof(EdgeSpan)) + carve_size(4);
   CAC_HIP(ctx, hipMalloc(&out->arena, bytes));
   Carver cv(out->arena);
@@ -844,11 +930,13 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   h.t = t;
   h.trial = trial;
   h.slot = 0;
+  h.seed = seed;
+  h.dpr = dpr;
   TrialScratch* d_scr = cv.take<TrialScratch>(1);
   h.lab = cv.take<uint32_t>(n);
   h.par = cv.take<uint32_t>(n);
   h.pre = cv.take<uint64_t>(std::max<uint64_t>(m, 1));
-  h.draws = cv.take<uint2>(n);
+  h.draws = cv.take<uint2>(std::max(n, dpr));
   h.nv_out = cv.take<uint32_t>(1);
   int64_t* d_misc = cv.take<int64_t>(2);
   EdgeSpan* d_span = cv.take<EdgeSpan>(1);
@@ -860,7 +948,7 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   if (rc) return rc;
   h = one[0];
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  launch_sampling(ctx, 1, seed, d_scr, s);
+  launch_sampling(ctx, 1, m, d_scr, s);
   CAC_LAUNCH_CHECK(ctx, "sample trial");
   uint32_t k = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&k, h.nv_out, 4, hipMemcpyDeviceToHost, s));

@@ -4,13 +4,14 @@
 //
 // Design: W worker threads, each with its own context (HIP stream + scratch).
 // A free worker takes the oldest pending query, then every other pending query
-// of the same kind (min cut: same seed) that still fits the HBM budget, up to
-// max_batch, and runs them as one launch group:
+// of the same kind that still fits the HBM budget, up to max_batch, and runs
+// them as one launch group:
 //   - CC: the graphs are laid side by side (vertex ids offset) and one CC
 //     pipeline labels all of them; labels are canonical minimum ids, so each
 //     query's labels and component count are exactly its alone result;
 //   - min cut: every (query, trial) job of the group shares the sampling,
-//     densify and Karger-Stein launches (sqrt_cut_batch).
+//     densify and Karger-Stein launches (sqrt_cut_batch); each job carries its
+//     query's seed, so queries of any seeds share launches.
 // Groups on different workers overlap on the device.  A query is admitted only
 // when its footprint estimate fits beside the groups in flight, so requests
 // that arrive while others run are admitted as memory and workers free up.
@@ -20,6 +21,7 @@
 #include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "cac_common.h"
@@ -34,29 +36,57 @@ int cc_launch(cac_ctx* ctx, const cac_uedge* d_edges, uint64_t m, uint32_t n, in
 
 namespace {
 
-constexpr uint32_t kServeMaxT = 4096;              // dense base size cap for served min cuts
+constexpr uint32_t kServeMaxT = 4096;              // dense base size cap for explicit trial budgets
 constexpr uint64_t kServeMaxGroupEdges = 1ull << 27;  // edges per CC launch group
+// Trials of one query whose sampling and Karger-Stein scratch count towards its
+// admitted footprint: the group's trials run in chunks of at most that many
+// per query (two waves of sampling workgroups on 256 CUs).
+constexpr uint32_t kServeChunkTrials = 1024;
 
 struct Pending {
   uint64_t id;
   cac_query q;
   double submit_s;
   size_t bytes;
+  const cac_edge* d_edges;  // a registered graph's device edges (checked once), or null
 };
 
-void mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* T) {
-  *t = std::min<uint32_t>(sqrt_intermediate_size(n, m), kServeMaxT);
-  *T = std::max<uint32_t>(2, ks_trials_host(*t, p));
+// A graph registered with the server: its edges stay in device memory, checked
+// once, and queries name it instead of bringing their edges.
+struct Resident {
+  cac_edge* d_edges;
+  uint64_t m;
+  uint32_t n;
+  size_t bytes;
+  uint32_t users;  // queries submitted and not yet finished
+  bool dropped;    // unregistered: freed when the last user finishes
+};
+
+// trials == 0: the reference's rule, as cac_mincut runs it (SquareRootCut::
+// intermediate_size and numberOfTrials, src/SquareRootCut.cpp:63-83);
+// trials > 0: that many trials on a contracted graph of at most kServeMaxT
+// vertices (the dense engine's fast sizes).
+void mincut_plan(uint32_t n, uint64_t m, double p, uint32_t trials, uint32_t* t, uint32_t* T) {
+  if (trials == 0) {
+    *t = sqrt_intermediate_size(n, m);
+    *T = std::max<uint32_t>(1, sqrt_trials_host(n, m, p));
+  } else {
+    *t = std::min<uint32_t>(sqrt_intermediate_size(n, m), kServeMaxT);
+    *T = trials;
+  }
 }
 
-size_t footprint(const cac_query& q) {
+// Device bytes a query holds while it runs; a registered graph's edges are
+// already resident and not counted again.
+size_t footprint(const cac_query& q, bool resident) {
+  const size_t upload = resident ? 0 : carve_size(q.m * sizeof(cac_edge));
   if (q.kind == CAC_QUERY_CC)
-    return carve_size(q.m * sizeof(cac_edge)) + carve_size(q.m * sizeof(cac_uedge)) + carve_size((size_t)q.n * 4) +
-           cc_tmp_words(q.m, q.n, 0) * 4 + 4096;
+    return upload + carve_size(q.m * sizeof(cac_uedge)) + carve_size((size_t)q.n * 4) + cc_tmp_words(q.m, q.n, 0) * 4 +
+           4096;
   uint32_t t, T;
-  mincut_plan(q.n, q.m, q.p, &t, &T);
-  SqrtJob j{nullptr, q.m, q.n, t, 0, 0};
-  return carve_size(q.m * sizeof(cac_edge)) + (size_t)T * (job_sample_bytes(j) + job_ks_bytes(j)) + 4096;
+  mincut_plan(q.n, q.m, q.p, q.trials, &t, &T);
+  SqrtJob j{nullptr, q.m, q.n, t, 0, 0, 0};
+  return upload + (size_t)std::min(T, kServeChunkTrials) * (job_sample_bytes(j) + job_ks_bytes(j)) + 4096;
 }
 
 }  // namespace
@@ -73,6 +103,8 @@ struct cac_server {
   std::vector<cac_query_result> done;
   uint64_t inflight = 0, next_id = 0;
   uint64_t peak_inflight = 0, groups = 0;
+  std::unordered_map<uint64_t, cac::Resident> graphs;  // registered graphs by handle
+  uint64_t next_graph = 1, resident = 0;               // resident: bytes of registered graphs
   bool stop = false;
   std::vector<std::thread> workers;
   std::vector<cac_ctx*> ctxs;
@@ -91,9 +123,10 @@ cac_query_result blank(const Pending& p) {
   return r;
 }
 
-// Cheap checks at submit time; edge ids and weights are checked on the device.
-int validate(const cac_query& q) {
-  if (q.n < 1 || (q.m && !q.edges)) return CAC_EINVAL;
+// Cheap checks at submit time; edge ids and weights are checked on the device
+// (a registered graph's when it was registered).
+int validate(const cac_query& q, bool resident) {
+  if (q.n < 1 || (q.m && !q.edges && !resident)) return CAC_EINVAL;
   if (q.kind == CAC_QUERY_MINCUT && (q.n < 2 || q.m == 0 || !(q.p >= 0.0 && q.p < 1.0))) return CAC_EINVAL;
   return CAC_OK;
 }
@@ -187,9 +220,13 @@ void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query
     for (size_t i = 0; i < grp.size() && e == hipSuccess; ++i) {
       const cac_query& q = grp[i].q;
       if (!q.m) continue;
-      e = hipMemcpyAsync(d_in + eoff[i], q.edges, q.m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
-      hipLaunchKernelGGL(k_serve_prep_cc, dim3(grid_for(q.m, 256, 2048)), dim3(256), 0, s,
-                         (const cac_edge*)(d_in + eoff[i]), q.m, q.n, (uint32_t)voff[i], d_e + eoff[i], d_bad + i);
+      const cac_edge* src = grp[i].d_edges;  // a registered graph needs no upload
+      if (!src) {
+        e = hipMemcpyAsync(d_in + eoff[i], q.edges, q.m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
+        src = d_in + eoff[i];
+      }
+      hipLaunchKernelGGL(k_serve_prep_cc, dim3(grid_for(q.m, 256, 2048)), dim3(256), 0, s, src, q.m, q.n,
+                         (uint32_t)voff[i], d_e + eoff[i], d_bad + i);
     }
     if (e != hipSuccess) rc = hip_fail(ctx, e, "serve cc upload");
     if (rc == CAC_OK) rc = cc_launch(ctx, d_e, M, (uint32_t)N, CAC_LABEL_MIN_ID, d_l, d_t, s);
@@ -218,8 +255,8 @@ void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query
 }
 
 void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query_result>& res, size_t budget) {
-  uint64_t M = 0;
-  for (auto& p : grp) M += p.q.m;
+  uint64_t M = 0;  // edges to upload: registered graphs are already on the device
+  for (auto& p : grp) M += p.d_edges ? 0 : p.q.m;
   int rc = CAC_OK;
   hipStream_t s = ctx->stream;
   cac_edge* d_e = nullptr;
@@ -236,6 +273,7 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
     for (size_t i = 0; i < grp.size() && e == hipSuccess; ++i) {
       const cac_query& q = grp[i].q;
       eoff[i] = off;
+      if (grp[i].d_edges) continue;  // checked when it was registered
       e = hipMemcpyAsync(d_e + off, q.edges, q.m * sizeof(cac_edge), hipMemcpyHostToDevice, s);
       hipLaunchKernelGGL(k_serve_check_w, dim3(grid_for(q.m, 256, 2048)), dim3(256), 0, s, d_e + off, q.m, q.n,
                          d_chk + 3 * i);
@@ -259,16 +297,17 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
       continue;
     }
     uint32_t t, T;
-    mincut_plan(q.n, q.m, q.p, &t, &T);
+    mincut_plan(q.n, q.m, q.p, q.trials, &t, &T);
     res[i].t = t;
     res[i].trials = T;
-    for (uint32_t k = 0; k < T; ++k) jobs.push_back({d_e + eoff[i], q.m, q.n, t, k, (uint32_t)jobs.size()});
+    const cac_edge* qe = grp[i].d_edges ? grp[i].d_edges : d_e + eoff[i];
+    for (uint32_t k = 0; k < T; ++k) jobs.push_back({qe, q.m, q.n, t, k, (uint32_t)jobs.size(), q.seed});
   }
   first[grp.size()] = jobs.size();
   if (rc == CAC_OK && !jobs.empty()) {
     e = hipMalloc((void**)&d_res, 8 * jobs.size());
     if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
-    if (rc == CAC_OK) rc = sqrt_cut_batch(ctx, jobs, grp[0].q.seed, d_res, s, budget);
+    if (rc == CAC_OK) rc = sqrt_cut_batch(ctx, jobs, d_res, s, budget);
   }
   std::vector<int64_t> raw(jobs.size());
   if (rc == CAC_OK && !jobs.empty()) {
@@ -296,7 +335,7 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
       std::unique_lock<std::mutex> lk(S->mu);
       for (;;) {
         if (S->stop) return;
-        if (!S->pending.empty() && S->inflight + S->pending.front().bytes <= S->budget) break;
+        if (!S->pending.empty() && S->inflight + S->resident + S->pending.front().bytes <= S->budget) break;
         S->cv.wait(lk);
       }
       Pending head = S->pending.front();
@@ -305,8 +344,8 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
       uint64_t edges = head.q.m;
       grp.push_back(head);
       for (auto it = S->pending.begin(); it != S->pending.end() && grp.size() < S->max_batch;) {
-        const bool same = it->q.kind == head.q.kind && (head.q.kind == CAC_QUERY_CC || it->q.seed == head.q.seed);
-        const bool fits = S->inflight + bytes + it->bytes <= S->budget &&
+        const bool same = it->q.kind == head.q.kind;
+        const bool fits = S->inflight + S->resident + bytes + it->bytes <= S->budget &&
                           (head.q.kind != CAC_QUERY_CC || edges + it->q.m <= kServeMaxGroupEdges);
         const bool ids_fit = head.q.kind != CAC_QUERY_CC || [&] {
           uint64_t nsum = 0;
@@ -339,6 +378,7 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
     else
       run_mincut_group(ctx, grp, res, bytes);
     const double fin = S->now();
+    std::vector<cac_edge*> freed;
     {
       std::lock_guard<std::mutex> lk(S->mu);
       S->inflight -= bytes;
@@ -346,7 +386,16 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
         r.done_s = fin;
         S->done.push_back(r);
       }
+      for (auto& p : grp) {  // registered graphs: one user fewer; free unregistered ones now unused
+        if (!p.q.graph) continue;
+        auto it = S->graphs.find(p.q.graph);
+        if (it == S->graphs.end() || --it->second.users > 0 || !it->second.dropped) continue;
+        freed.push_back(it->second.d_edges);
+        S->resident -= it->second.bytes;
+        S->graphs.erase(it);
+      }
     }
+    for (cac_edge* d : freed) (void)hipFree(d);
     S->cv.notify_all();
   }
 }
@@ -356,9 +405,9 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
 
 extern "C" {
 
-int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* trials) {
+int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t trials_req, uint32_t* t, uint32_t* trials) {
   if (!t || !trials || n < 2 || m == 0) return CAC_EINVAL;
-  cac::mincut_plan(n, m, p, t, trials);
+  cac::mincut_plan(n, m, p, trials_req, t, trials);
   return CAC_OK;
 }
 
@@ -401,13 +450,26 @@ int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
   if (q->kind != CAC_QUERY_CC && q->kind != CAC_QUERY_MINCUT) return CAC_EINVAL;
   cac::Pending p;
   p.q = *q;
-  p.bytes = cac::footprint(*q);
-  const int v = cac::validate(*q);
+  p.d_edges = nullptr;
   std::lock_guard<std::mutex> lk(S->mu);
+  int v = CAC_OK;
+  if (q->graph) {  // a registered graph: its size and device edges come from the registration
+    auto it = S->graphs.find(q->graph);
+    if (it == S->graphs.end() || it->second.dropped) {
+      v = CAC_EINVAL;
+    } else {
+      p.q.n = it->second.n;
+      p.q.m = it->second.m;
+      p.q.edges = nullptr;
+      p.d_edges = it->second.d_edges;
+    }
+  }
+  if (v == CAC_OK) v = cac::validate(p.q, p.d_edges != nullptr);
+  p.bytes = v == CAC_OK ? cac::footprint(p.q, p.d_edges != nullptr) : 0;
   p.id = S->next_id++;
   p.submit_s = S->now();
   if (id_out) *id_out = p.id;
-  if (v != CAC_OK || p.bytes > S->budget) {  // rejected now: reported through poll
+  if (v != CAC_OK || p.bytes + S->resident > S->budget) {  // rejected now: reported through poll
     cac_query_result r = cac::blank(p);
     r.status = v != CAC_OK ? v : CAC_ENOMEM;
     r.admit_s = r.done_s = p.submit_s;
@@ -415,9 +477,70 @@ int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
     S->cv.notify_all();
     return CAC_OK;
   }
+  if (p.d_edges) ++S->graphs[q->graph].users;
   S->pending.push_back(p);
   S->cv.notify_all();
   return CAC_OK;
+}
+
+int cac_server_register_graph(cac_server* S, const cac_edge* edges, uint64_t m, uint32_t n, uint64_t* graph_out) {
+  if (!S || !graph_out || n < 1 || (m && !edges)) return CAC_EINVAL;
+  const size_t bytes = cac::carve_size(std::max<uint64_t>(m, 1) * sizeof(cac_edge)) + 256;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->resident + bytes > S->budget) return CAC_ENOMEM;
+  }
+  if (hipSetDevice(S->device) != hipSuccess) return CAC_EDEVICE;
+  void* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return CAC_ENOMEM;
+  }
+  // ids, weights and the weight sum are checked once, here (as a min-cut
+  // group checks an uploaded query)
+  unsigned long long* chk = (unsigned long long*)((char*)d + bytes - 256);
+  unsigned long long h[3] = {0, 0, 0};
+  hipError_t e = m ? hipMemcpy(d, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice) : hipSuccess;
+  if (e == hipSuccess) e = hipMemcpy(chk, h, sizeof h, hipMemcpyHostToDevice);
+  if (e == hipSuccess && m) {
+    hipLaunchKernelGGL(cac::k_serve_check_w, dim3(cac::grid_for(m, 256, 2048)), dim3(256), 0, 0, (cac_edge*)d, m, n,
+                       chk);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(h, chk, sizeof h, hipMemcpyDeviceToHost);
+  int rc = e != hipSuccess ? CAC_EDEVICE
+           : (h[0] & 1)   ? CAC_EINVAL
+           : ((h[0] & 2) || h[1] >= (1ull << 61) || h[2] >= (1ull << 29)) ? CAC_ERANGE
+                                                                          : CAC_OK;
+  if (rc != CAC_OK) {
+    (void)hipFree(d);
+    return rc;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  const uint64_t g = S->next_graph++;
+  S->graphs[g] = cac::Resident{(cac_edge*)d, m, n, bytes, 0u, false};
+  S->resident += bytes;
+  *graph_out = g;
+  return CAC_OK;
+}
+
+int cac_server_unregister_graph(cac_server* S, uint64_t graph) {
+  if (!S) return CAC_EINVAL;
+  cac_edge* d = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->graphs.find(graph);
+    if (it == S->graphs.end() || it->second.dropped) return CAC_EINVAL;
+    if (it->second.users > 0) {  // freed by the worker that finishes its last query
+      it->second.dropped = true;
+      return CAC_OK;
+    }
+    d = it->second.d_edges;
+    S->resident -= it->second.bytes;
+    S->graphs.erase(it);
+  }
+  if (hipSetDevice(S->device) != hipSuccess) return CAC_EDEVICE;
+  return hipFree(d) == hipSuccess ? CAC_OK : CAC_EDEVICE;
 }
 
 int cac_server_poll(cac_server* S, cac_query_result* out, uint32_t cap, uint32_t* count_out, int wait) {
@@ -441,6 +564,8 @@ int cac_server_destroy(cac_server* S) {
   S->cv.notify_all();
   for (auto& t : S->workers) t.join();
   for (auto c : S->ctxs) cac_ctx_destroy(c);
+  if (!S->graphs.empty() && hipSetDevice(S->device) == hipSuccess)
+    for (auto& g : S->graphs) (void)hipFree(g.second.d_edges);
   delete S;
   return CAC_OK;
 }

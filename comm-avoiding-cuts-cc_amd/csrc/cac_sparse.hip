@@ -72,7 +72,7 @@ struct SpChild {
   const cac_edge* edges;  // parent edges [m], u < v
   uint64_t m;
   uint32_t n, tgt, trial, nid;  // nid: this child's node id (keys come from it)
-  uint32_t slot, pad;
+  uint32_t slot, seed;          // seed: Philox key word 0 of the child's trial
   cac_edge* out;              // [m] child edges (kept for the next depth)
   uint64_t* keys;             // [m] scratch: race keys
   uint32_t* list[2];          // [m] scratch: candidate edge indices, ping-pong
@@ -300,11 +300,11 @@ __global__ void k_sp_kmm_init(const SpChild* __restrict__ children, uint32_t cou
 
 // Race keys of every child's parent edges, spread over the whole device (grid
 // y = child), with each child's smallest and largest key.
-__global__ void __launch_bounds__(256) k_sp_keys(const SpChild* __restrict__ children, uint32_t seed) {
+__global__ void __launch_bounds__(256) k_sp_keys(const SpChild* __restrict__ children) {
   const SpChild c = children[blockIdx.y];
   uint64_t kmin = ~0ull, kmax = 0;
   for (uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x; e < c.m; e += (uint64_t)gridDim.x * 256) {
-    const u32x4 r = philox4x32(c.trial, c.nid, (uint32_t)e, (uint32_t)(e >> 32), seed, kDomSparse);
+    const u32x4 r = philox4x32(c.trial, c.nid, (uint32_t)e, (uint32_t)(e >> 32), c.seed, kDomSparse);
     const uint64_t k = race_key(lo64(r), c.edges[e].w);
     c.keys[e] = k;
     kmin = min(kmin, k);
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(256) k_sp_keys(const SpChild* __restrict__ chi
   }
 }
 
-__global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* __restrict__ children, uint32_t seed,
+__global__ void __launch_bounds__(kSpBlock, 8) k_sparse_contract(const SpChild* __restrict__ children,
                                                               int64_t* __restrict__ results) {
   __shared__ uint64_t red[kSpWaves];
   __shared__ uint32_t wcnt[kSpRows * kSpWaves + 1];
@@ -508,14 +508,14 @@ __global__ void __launch_bounds__(kSpBlock) k_sp_tile_write(const SpChild* __res
 // The contraction of nchild children (device descriptors d_children, largest
 // parent edge count mmax): keys over the whole device, then one workgroup per
 // child.
-void launch_sparse_contract(const SpChild* d_children, uint32_t nchild, uint64_t mmax, uint32_t seed,
-                            int64_t* d_results, hipStream_t s) {
+void launch_sparse_contract(const SpChild* d_children, uint32_t nchild, uint64_t mmax, int64_t* d_results,
+                            hipStream_t s) {
   const uint64_t per_child = std::max<uint64_t>(1, 8192 / std::max<uint32_t>(nchild, 1));
   const unsigned gx = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (mmax + 4095) / 4096), per_child);
   hipLaunchKernelGGL(k_sp_kmm_init, dim3(grid_for(nchild, 256, 64)), dim3(256), 0, s, d_children, nchild);
   for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)  // grid y holds at most 65535 children
-    hipLaunchKernelGGL(k_sp_keys, dim3(gx, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0, seed);
-  hipLaunchKernelGGL(k_sparse_contract, dim3(nchild), dim3(kSpBlock), 0, s, d_children, seed, d_results);
+    hipLaunchKernelGGL(k_sp_keys, dim3(gx, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0);
+  hipLaunchKernelGGL(k_sparse_contract, dim3(nchild), dim3(kSpBlock), 0, s, d_children, d_results);
   const unsigned tiles = (unsigned)std::max<uint64_t>(1, (mmax + kSpTile - 1) / kSpTile);
   for (uint32_t c0 = 0; c0 < nchild; c0 += 65535u)
     hipLaunchKernelGGL(k_sp_tile_count, dim3(tiles, std::min(nchild - c0, 65535u)), dim3(256), 0, s, d_children + c0);
@@ -626,6 +626,7 @@ struct SpNode {
   uint64_t m;
   uint32_t n, trial, nid, slot;
   int64_t bound;  // the trial's bound (minimum weighted degree of its root)
+  uint32_t seed;  // Philox key word 0 of the trial
 };
 
 // Device bytes of one child: what stays alive for the next depth (its edge
@@ -641,7 +642,6 @@ size_t dense_bytes(const SpNode& nd) {
 
 struct SpRun {
   cac_ctx* ctx;
-  uint32_t seed;
   int64_t* d_results;
   hipStream_t s;
   size_t depth_budget;  // bytes of children kept alive per depth
@@ -679,7 +679,7 @@ struct SpRun {
       for (size_t i = 0; i < count; ++i) {
         const SpNode& nd = nodes[i0 + i];
         hd[i] = SpDense{nd.edges, nd.m, nd.n, 0, cv.take<int64_t>((size_t)nd.n * nd.n), cv.take<int64_t>(nd.n)};
-        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, nd.bound};
+        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, nd.bound, nd.seed};
         r.nid = nd.nid;
         roots.push_back(r);
         mmax = std::max(mmax, nd.m);
@@ -694,7 +694,7 @@ struct SpRun {
       hipLaunchKernelGGL(k_sp_rowsum, dim3(gr, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
       prof_end(ctx, s);
       CAC_LAUNCH_CHECK(ctx, "sparse densify");
-      rc = ks_run(ctx, roots, seed, d_results, s);
+      rc = ks_run(ctx, roots, d_results, s);
       if (rc) return rc;
       i0 = i1;
     }
@@ -750,7 +750,7 @@ struct SpRun {
           x.trial = nd.trial;
           x.nid = 2 * nd.nid + k;
           x.slot = nd.slot;
-          x.pad = 0;
+          x.seed = nd.seed;
           x.out = cv.take<cac_edge>(nd.m);
           x.keys = ct.take<uint64_t>(nd.m);
           x.list[0] = ct.take<uint32_t>(nd.m);
@@ -765,7 +765,7 @@ struct SpRun {
       }
       CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(SpChild), hipMemcpyHostToDevice, s));
       prof_begin(ctx, "sparse_contract", s);
-      launch_sparse_contract(d_children, (uint32_t)nchild, mmax, seed, d_results, s);
+      launch_sparse_contract(d_children, (uint32_t)nchild, mmax, d_results, s);
       prof_end(ctx, s);
       CAC_LAUNCH_CHECK(ctx, "sparse contract");
       std::vector<unsigned long long> mout(nchild);
@@ -774,7 +774,8 @@ struct SpRun {
       std::vector<SpNode> next;
       for (size_t i = 0; i < nchild; ++i) {
         if (mout[i] == kSpDisconnected) continue;  // its cut of 0 is already recorded
-        next.push_back({ch[i].out, mout[i], ch[i].tgt, ch[i].trial, ch[i].nid, ch[i].slot, sp[i0 + i / 2].bound});
+        next.push_back(
+            {ch[i].out, mout[i], ch[i].tgt, ch[i].trial, ch[i].nid, ch[i].slot, sp[i0 + i / 2].bound, ch[i].seed});
       }
       rc = level(next, depth + 1);
       if (rc) return rc;
@@ -793,8 +794,7 @@ size_t sparse_depth(uint32_t n) {
 
 }  // namespace
 
-int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t seed, int64_t* d_results,
-                    hipStream_t s) {
+int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, int64_t* d_results, hipStream_t s) {
   if (roots.empty()) return CAC_OK;
   size_t avail = 0;
   int rc = device_memory_available(ctx, &avail);
@@ -804,9 +804,9 @@ int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t
   uint32_t nmax = 0;
   for (const SparseRoot& r : roots) nmax = std::max(nmax, r.n);
   const size_t depth = sparse_depth(nmax);
-  SpRun run{ctx, seed, d_results, s, (size_t)(0.4 * (double)avail) / (depth + 1), (size_t)(0.15 * (double)avail)};
+  SpRun run{ctx, d_results, s, (size_t)(0.4 * (double)avail) / (depth + 1), (size_t)(0.15 * (double)avail)};
   std::vector<SpNode> level;
-  for (const SparseRoot& r : roots) level.push_back({r.edges, r.m, r.n, r.trial, 1u, r.slot, r.bound});
+  for (const SparseRoot& r : roots) level.push_back({r.edges, r.m, r.n, r.trial, 1u, r.slot, r.bound, r.seed});
   rc = run.level(level, 0);
   if (rc) return rc;
   CAC_HIP(ctx, hipStreamSynchronize(s));
@@ -816,8 +816,9 @@ int ks_sparse_roots(cac_ctx* ctx, const std::vector<SparseRoot>& roots, uint32_t
 int ks_sparse_run(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint32_t n, uint32_t seed,
                   const std::vector<uint32_t>& trial_ids, int64_t bound, int64_t* d_results, hipStream_t s) {
   std::vector<SparseRoot> roots;
-  for (size_t i = 0; i < trial_ids.size(); ++i) roots.push_back({d_edges, m, n, trial_ids[i], (uint32_t)i, bound});
-  return ks_sparse_roots(ctx, roots, seed, d_results, s);
+  for (size_t i = 0; i < trial_ids.size(); ++i)
+    roots.push_back({d_edges, m, n, trial_ids[i], (uint32_t)i, bound, seed});
+  return ks_sparse_roots(ctx, roots, d_results, s);
 }
 
 }  // namespace cac
@@ -923,6 +924,7 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   c.tgt = tgt;
   c.trial = trial;
   c.nid = nid;
+  c.seed = seed;
   c.out = cv.take<cac_edge>(mm);
   c.keys = cv.take<uint64_t>(mm);
   c.list[0] = cv.take<uint32_t>(mm);
@@ -936,7 +938,7 @@ int cac_sparse_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
   c.tiles = cv.take<uint32_t>((mm + cac::kSpTile - 1) / cac::kSpTile);
   if (m) CAC_HIP(ctx, hipMemcpyAsync(d_in, edges, m * sizeof(cac_edge), hipMemcpyHostToDevice, s));
   CAC_HIP(ctx, hipMemcpyAsync(d_child, &c, sizeof(c), hipMemcpyHostToDevice, s));
-  cac::launch_sparse_contract(d_child, 1, m, seed, d_res, s);
+  cac::launch_sparse_contract(d_child, 1, m, d_res, s);
   CAC_LAUNCH_CHECK(ctx, "sparse contract");
   unsigned long long mo = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&mo, d_mout, 8, hipMemcpyDeviceToHost, s));

@@ -84,6 +84,13 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
  * of two >= s/8, at least 64); otherwise a power of two from max(64, key/8) to
  * 1024, CAC_EINVAL else. */
 #define CAC_TUNE_KS_CONTRACT_BLOCK 1
+/* CAC_TUNE_SAMPLE_OCCUPANCY: 1024-thread sampling workgroups per CU for a
+ * launch of more square-root trials than CUs, keyed by the class of the
+ * launch's largest edge count m: key 16 holds m < 2^16, 20 m < 2^20, 24
+ * m < 2^24, 32 the rest.  value 0 restores the built-in 2 (64 registers per
+ * thread, some spilled); 1 gives the workgroup the registers it wants;
+ * CAC_EINVAL else. */
+#define CAC_TUNE_SAMPLE_OCCUPANCY 2
 int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value);
 int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value);
 
@@ -238,6 +245,24 @@ int cac_sample_contract(cac_ctx* ctx, const cac_edge* edges, uint64_t m, uint32_
                         uint32_t trial, uint32_t* vmap_out, uint32_t* n_out, cac_edge* merged_out,
                         uint64_t merged_cap, uint64_t* m_out);
 
+/* Sampling rule of the shrink phase for every sampling call on this context
+ * (cac_sample_contract, cac_sqrt_cut_trials, cac_mincut, cac_mincut_batch):
+ *   CAC_SAMPLE_LC (the default): |V| draws per round, the low-concurrency rule
+ *     of SequentialSquareRootCut::iteratedSampling (src/SequentialSquareRootCut.cpp:37-48);
+ *   CAC_SAMPLE_HC: cac_hc_draws(n) draws per round, the high-concurrency rule of
+ *     WeightedIteratedSparseSampling::shrink / edgesToSamplePerProcessor
+ *     (src/WeightedIteratedSparseSampling.cpp:4-24, 236-279, .hpp:66-84).
+ * The reference takes the high-concurrency path when its ranks number at least
+ * twice the trials (SquareRootCut::lowConcurrency, src/SquareRootCut.cpp:11-13).
+ * Draw d of round r is the same Philox word under both rules; CAC_EINVAL for
+ * another value. */
+#define CAC_SAMPLE_LC 0
+#define CAC_SAMPLE_HC 1
+int cac_set_sampling_rule(cac_ctx* ctx, int rule);
+/* (unsigned) powf(n0, 1.05f): the reference's draws per round for n0 vertices
+ * (src/WeightedIteratedSparseSampling.hpp:66-68, epsilon = 0.1f). */
+uint32_t cac_hc_draws(uint32_t n0);
+
 /* ---- square-root min cut ----------------------------------------------------
  * One low-concurrency trial = SequentialSquareRootCut(graph, engine, t).compute()
  * (src/SequentialSquareRootCut.cpp:4-19): the sampling contraction above, then
@@ -284,14 +309,16 @@ int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_
  * solve one graph per MPI job).  W worker threads, each with its own context
  * and stream, admit queries first come first served when their device-memory
  * footprint fits beside the work in flight (hbm_budget bytes in total), and
- * run up to max_batch pending queries of one kind (min cut: same seed) as one
- * launch group: CC graphs side by side in one CC pipeline, min-cut trials of
- * all queries in shared sampling / Karger-Stein launches.  Groups on different
- * workers overlap on the device.  Results do not depend on batching:
+ * run up to max_batch pending queries of one kind as one launch group: CC
+ * graphs side by side in one CC pipeline, the min-cut trials of all queries
+ * (each with its own seed) in shared sampling / Karger-Stein launches.  Groups
+ * on different workers overlap on the device.  Results do not depend on
+ * batching:
  *   CC:      ncomp and labels (canonical minimum ids) equal cac_cc's;
  *   min cut: cut = min over trials 0 .. T-1 of the square-root cut with
  *            (t, T) from cac_serve_mincut_plan, i.e. the minimum of
- *            cac_sqrt_cut_trials(edges, n, t, seed, 0, T).
+ *            cac_sqrt_cut_trials(edges, n, t, seed, 0, T); with trials = 0
+ *            (the reference's rule) that is cac_mincut(edges, n, p, seed).
  * A query's edges (and labels_out) are host memory that must stay valid until
  * its result is polled.  Invalid queries (ids >= n, weights >= 2^40, weight sum
  * >= 2^61, footprint > budget) come back with a non-zero status. */
@@ -304,8 +331,9 @@ typedef struct cac_query {
   const cac_edge* edges;   /* m records; CC ignores w */
   double p;                /* min cut: success probability */
   uint32_t seed;           /* min cut: Philox seed */
-  uint32_t reserved;
+  uint32_t trials;         /* min cut: 0 = the reference's rule for p, k > 0 = k trials (cac_serve_mincut_plan) */
   uint32_t* labels_out;    /* CC: optional, n labels (minimum id of the component) */
+  uint64_t graph;          /* 0: edges/n/m above; else a cac_server_register_graph handle (edges, n, m ignored) */
 } cac_query;
 typedef struct cac_query_result {
   uint64_t id;             /* as returned by cac_server_submit */
@@ -318,9 +346,13 @@ typedef struct cac_query_result {
   uint32_t reserved;
 } cac_query_result;
 typedef struct cac_server cac_server;
-/* Served min cut: t = min(cac_intermediate_size(n, m), 4096) (the dense base
- * stays within one GPU's fast path), T = max(2, cac_ks_trials(t, p)). */
-int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t* t, uint32_t* trials);
+/* Served min cut: the intermediate size t and trial count T of a query.
+ * trials_req == 0: the reference's rule for success probability p, as cac_mincut
+ *   runs it: t = cac_intermediate_size(n, m), T = cac_sqrt_trials(n, m, p)
+ *   (SquareRootCut::intermediate_size / numberOfTrials, src/SquareRootCut.cpp:63-83);
+ * trials_req > 0: a trial budget with no success guarantee: T = trials_req,
+ *   t = min(cac_intermediate_size(n, m), 4096) (the dense engine's fast sizes). */
+int cac_serve_mincut_plan(uint32_t n, uint64_t m, double p, uint32_t trials_req, uint32_t* t, uint32_t* trials);
 int cac_server_create(int device, uint32_t workers, uint64_t hbm_budget, uint32_t max_batch, cac_server** out);
 int cac_server_submit(cac_server* srv, const cac_query* q, uint64_t* id_out);
 /* Moves up to cap finished results to out; with wait != 0 blocks until at least
@@ -336,6 +368,15 @@ int cac_server_stats(cac_server* srv, uint64_t* peak_inflight_bytes, uint64_t* g
 int cac_server_replay(cac_server* srv, const cac_query* q, const double* arrival_s, uint32_t count,
                       cac_query_result* out);
 int cac_server_destroy(cac_server* srv);
+/* Device-resident graphs: the edges (cac_edge records, ids < n) are copied to
+ * device memory the server holds until unregistered, counted against
+ * hbm_budget, and checked once (CAC_EINVAL for ids >= n, CAC_ERANGE for weights
+ * >= 2^40 or a sum >= 2^61).  A query with cac_query.graph = *graph_out runs on
+ * them with no upload and no per-query check; its results equal those of the
+ * same query bringing the edges.  Unregistering a graph that queries still use
+ * frees it when the last of them finishes. */
+int cac_server_register_graph(cac_server* srv, const cac_edge* edges, uint64_t m, uint32_t n, uint64_t* graph_out);
+int cac_server_unregister_graph(cac_server* srv, uint64_t graph);
 
 /* ---- approximate minimum cut ----------------------------------------------
  * Replaces ApproximateCut::run (src/ApproximateCut.cpp:10-79) and numberOfTrials
@@ -411,6 +452,20 @@ int cac_gen_planted_sizes(uint32_t first, uint32_t count, uint32_t seed, uint32_
 int cac_gen_planted(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax, double avg_degree,
                     uint32_t wmax, uint32_t cmax, cac_graph_desc* descs, cac_edge* edges, uint64_t edge_cap,
                     uint64_t* m_total, uint32_t* planted);
+/* The hard planted family (no counterpart in the reference): the sizes n_g of
+ * cac_gen_planted, but each half is the union of `cycles` Hamiltonian cycles
+ * over its n/2 vertices, cycle i in a Philox-shuffled vertex order with every
+ * edge of weight w_i uniform in [1, wmax].  Every vertex of a half has
+ * weighted degree delta = 2 sum w_i and every cut inside a half crosses each
+ * cycle twice, so a half's minimum cut is delta.  c uniform in [1, cmax]
+ * distinct cross edges carry C = floor(cut_frac * min(delta_A, delta_B)) in
+ * all (split as evenly as integers allow): the minimum cut is C, within a
+ * factor cut_frac of the minimum weighted degree -- many single-vertex cuts
+ * only slightly heavier than the planted one.  planted[i] = C; m_g =
+ * cycles * n_g + c.  Host-side, multithreaded. */
+int cac_gen_planted_hard(uint32_t first, uint32_t count, uint32_t seed, uint32_t nmin, uint32_t nmax,
+                         uint32_t cycles, uint32_t wmax, uint32_t cmax, double cut_frac, cac_graph_desc* descs,
+                         cac_edge* edges, uint64_t edge_cap, uint64_t* m_total, uint32_t* planted);
 
 #ifdef __cplusplus
 }

@@ -566,8 +566,31 @@ static uint32_t root_min(uint32_t* p, uint32_t x) {
     return r;
 }
 
+/* Draws per round of the high-concurrency weighted sampling:
+ * WeightedIteratedSparseSampling::edgesToSamplePerProcessor
+ * (src/WeightedIteratedSparseSampling.hpp:66-84) draws (unsigned)
+ * pow((float) n0, 1 + epsilon / 2) edges per round, epsilon = 0.1f
+ * (src/IteratedSparseSampling.hpp:38), n0 the initial vertex count, all in float. */
+uint32_t orc_hc_draws(uint32_t n0) {
+    const float epsilon = 0.1f;
+    return (uint32_t)powf((float)n0, 1 + epsilon / 2);
+}
+
 uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
                              uint32_t trial, uint32_t* labels) {
+    return orc_sample_contract_rule(n, m, edges, t, seed, trial, 0, labels);
+}
+
+/* The round loop of both sampling rules: a round draws draws_per_round edges,
+ * or |V| when draws_per_round == 0 (the low-concurrency rule,
+ * src/SequentialSquareRootCut.cpp:37-48); the high-concurrency rule draws
+ * orc_hc_draws(n) per round over the current graph
+ * (src/WeightedIteratedSparseSampling.cpp:236-279) and contracts a draw prefix
+ * to the target (src/IteratedSparseSampling.hpp:182-224); its gathered draws
+ * are shuffled (src/IteratedSparseSampling.hpp:278), which leaves i.i.d. draws
+ * i.i.d., so they are used in draw order. */
+uint32_t orc_sample_contract_rule(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
+                                  uint32_t trial, uint32_t draws_per_round, uint32_t* labels) {
     uint64_t* P = (uint64_t*)malloc(sizeof(uint64_t) * (m ? m : 1));
     uint32_t* par = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
     uint32_t key[2] = {seed, DOM_SAMPLE};
@@ -582,7 +605,7 @@ uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint
         }
         if (W == 0) break;
         for (uint32_t v = 0; v < n; ++v) par[v] = v;
-        uint32_t draws = nv;
+        uint32_t draws = draws_per_round ? draws_per_round : nv;
         for (uint32_t d = 0; d < draws; ++d) {
             uint32_t ctr[4] = {trial, round, d, 0}, o[4];
             orc_philox4x32_10(ctr, key, o);
@@ -671,8 +694,13 @@ static uint64_t* merged_contracted(uint32_t n, uint64_t m, const uint64_t* edges
 }
 
 int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial) {
+    return orc_sqrt_trial_rule(n, m, edges, t, seed, trial, 0);
+}
+
+int64_t orc_sqrt_trial_rule(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial,
+                            uint32_t draws_per_round) {
     uint32_t* lab = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
-    orc_sample_contract(n, m, edges, t, seed, trial, lab);
+    orc_sample_contract_rule(n, m, edges, t, seed, trial, draws_per_round, lab);
     uint32_t k = 0;
     for (uint32_t x = 0; x < n; ++x) k += lab[x] == x;
     if (t > SQRT_DENSE_MAX) {

@@ -107,6 +107,13 @@ void orc_ks_first_pick(uint32_t n, const int64_t* adj, uint32_t seed, uint32_t t
  * (== t unless the graph has more than t components). edges: m x (u, v, w). */
 uint32_t orc_sample_contract(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
                              uint32_t trial, uint32_t* labels);
+/* The same with draws_per_round draws in every round (0: |V|, the rule above).
+ * The high-concurrency rule (WeightedIteratedSparseSampling::shrink,
+ * src/WeightedIteratedSparseSampling.cpp:4-24, 236-279) draws orc_hc_draws(n)
+ * = (unsigned) powf(n, 1.05f) per round (.hpp:66-84). */
+uint32_t orc_sample_contract_rule(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed,
+                                  uint32_t trial, uint32_t draws_per_round, uint32_t* labels);
+uint32_t orc_hc_draws(uint32_t n0);
 /* Contracted multigraph -> dense matrix over the ranks of the canonical labels
  * (AdjacencyListGraph::compact, src/AdjacencyListGraph.cpp:34-52, renamed by rank
  * of the minimum id instead of first appearance).  out: k x k, k = distinct labels. */
@@ -118,6 +125,8 @@ uint32_t orc_densify(uint32_t n, uint64_t m, const uint64_t* edges, const uint32
  * loops dropped, parallel edges summed and sorted by (u, v), through one sparse
  * Karger-Stein trial (orc_ks_sparse_trial, node 1); a disconnected graph gives 0. */
 int64_t orc_sqrt_trial(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial);
+int64_t orc_sqrt_trial_rule(uint32_t n, uint64_t m, const uint64_t* edges, uint32_t t, uint32_t seed, uint32_t trial,
+                            uint32_t draws_per_round);
 
 /* ---- approximate minimum cut ------------------------------------------------
  * Restates ApproximateCut::run (src/ApproximateCut.cpp:10-79): for i = 1 ..

@@ -38,8 +38,11 @@ SIGS = {
     "orc_ks_trial": (_i64, [_u32, _vp, _u32, _u32]),
     "orc_min_degree": (_i64, [_u32, _vp]),
     "orc_sample_contract": (_u32, [_u32, _u64, _vp, _u32, _u32, _u32, _vp]),
+    "orc_sample_contract_rule": (_u32, [_u32, _u64, _vp, _u32, _u32, _u32, _u32, _vp]),
+    "orc_hc_draws": (_u32, [_u32]),
     "orc_densify": (_u32, [_u32, _u64, _vp, _vp, _vp]),
     "orc_sqrt_trial": (_i64, [_u32, _u64, _vp, _u32, _u32, _u32]),
+    "orc_sqrt_trial_rule": (_i64, [_u32, _u64, _vp, _u32, _u32, _u32, _u32]),
     "orc_ks_first_pick": (None, [_u32, _vp, _u32, _u32, _u32, _vp, _vp]),
     "orc_keep_threshold": (_u64, [_u32, _u64]),
     "orc_approx_cut": (None, [_u32, _u64, _vp, ctypes.c_double, _u32, _vp, _vp]),
@@ -172,11 +175,17 @@ def _edges3(edges):
     return np.ascontiguousarray(np.asarray(edges, dtype=np.uint64).reshape(-1, 3))
 
 
-def sample_contract(n: int, edges, t: int, seed: int, trial: int):
+def sample_contract(n: int, edges, t: int, seed: int, trial: int, draws_per_round: int = 0):
+    """draws_per_round: 0 = |V| per round (low-concurrency rule), else that many."""
     e = _edges3(edges)
     lab = np.empty(n, dtype=np.uint32)
-    nv = lib().orc_sample_contract(n, e.shape[0], _p(e), t, seed, trial, _p(lab))
+    nv = lib().orc_sample_contract_rule(n, e.shape[0], _p(e), t, seed, trial, draws_per_round, _p(lab))
     return lab, int(nv)
+
+
+def hc_draws(n0: int) -> int:
+    """Draws per round of the high-concurrency rule: (unsigned) powf(n0, 1.05f)."""
+    return int(lib().orc_hc_draws(n0))
 
 
 def densify(n: int, edges, labels):
@@ -188,9 +197,9 @@ def densify(n: int, edges, labels):
     return out
 
 
-def sqrt_trial(n: int, edges, t: int, seed: int, trial: int) -> int:
+def sqrt_trial(n: int, edges, t: int, seed: int, trial: int, draws_per_round: int = 0) -> int:
     e = _edges3(edges)
-    return int(lib().orc_sqrt_trial(n, e.shape[0], _p(e), t, seed, trial))
+    return int(lib().orc_sqrt_trial_rule(n, e.shape[0], _p(e), t, seed, trial, draws_per_round))
 
 
 def race_key(x: int, w: int) -> int:

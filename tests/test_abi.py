@@ -44,6 +44,9 @@ def test_tuning_file_shape():
     for key, b in t.items():
         assert int(key) in cac.TUNE_KS_CONTRACT_KEYS
         assert b == 0 or (max(64, int(key) // 8) <= b <= 1024 and b & (b - 1) == 0)
+    occ = json.loads(cac.TUNING_PATH.read_text()).get("sample_occupancy", {})
+    for key, k in occ.items():
+        assert int(key) in cac.TUNE_SAMPLE_OCCUPANCY_KEYS and k in (0, 1, 2)
 
 
 def test_record_layouts():

@@ -173,3 +173,50 @@ def test_sqrt_trial_beyond_dense_engine(ctx, orc):
     print(f"sqrt trial beyond dense: n={n} m={m} t={t}", flush=True)
     raw = ctx.sqrt_cut_trials(E, n, t, 3, 0, 1)
     assert raw[0] == orc.sqrt_trial(n, E, t, 3, 0)
+
+
+def test_sqrt_trials_beyond_dense_engine_merge(ctx, orc):
+    # n = 10^5 contracted to t = 8248: most edges become loops or parallel
+    # edges, so the merge (radix sort of every trial's keys together, sum of
+    # parallel weights) does real work; four trials of one call, bit-exact
+    rng = np.random.default_rng(4)
+    n, m = 100_000, 17_000_000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    v = np.where(u == v, (v + 1) % n, v)
+    E = np.stack([np.minimum(u, v), np.maximum(u, v), rng.integers(1, 10, m)], 1).astype(np.uint64)
+    t = orc.intermediate_size(n, m)
+    assert t > 8192
+    raw = ctx.sqrt_cut_trials(E, n, t, 5, 0, 4)
+    for k in range(4):
+        want = orc.sqrt_trial(n, E, t, 5, k)
+        print(f"sqrt trial {k} beyond dense (merge): n={n} m={m} t={t} device {raw[k]} oracle {want}", flush=True)
+        assert raw[k] == want
+
+
+@pytest.mark.parametrize("n,t", [(1000, 318), (3000, 400)])
+def test_hc_rule_bit_exact(orc, n, t):
+    """The high-concurrency rule (floor(n^1.05) draws per round) on the device:
+    labels, merged graph and raw sqrt-cut trials equal the oracle's."""
+    rng = np.random.default_rng(n)
+    u = rng.integers(0, n, 12 * n)
+    v = rng.integers(0, n, 12 * n)
+    keep = u != v
+    E = np.stack([u[keep], v[keep], rng.integers(1, 30, keep.sum())], 1).astype(np.uint64)
+    dpr = orc.hc_draws(n)
+    assert cac.hc_draws(n) == dpr
+    with cac.Context(0) as c:
+        c.set_sampling_rule(cac.SAMPLE_HC)
+        for trial in range(3):
+            vmap, k, merged = c.sample_contract(E, n, t, 9, trial)
+            lab, nv = orc.sample_contract(n, E, t, 9, trial, dpr)
+            assert k == nv
+            _, rank = np.unique(lab, return_inverse=True)
+            assert np.array_equal(vmap, rank.astype(np.uint32))
+        raw = c.sqrt_cut_trials(E, n, t, 9, 0, 4)
+        want = [orc.sqrt_trial(n, E, t, 9, k, dpr) for k in range(4)]
+        _say(f"hc rule n {n} t {t}: device {raw.tolist()} oracle {want}")
+        assert raw.tolist() == want
+        c.set_sampling_rule(cac.SAMPLE_LC)
+        lc = c.sqrt_cut_trials(E, n, t, 9, 0, 4)
+        assert lc.tolist() == [orc.sqrt_trial(n, E, t, 9, k) for k in range(4)]

@@ -107,3 +107,56 @@ def test_ws_generator():
     assert abs(moved.mean() - 0.3 * (1 - 1 / (n - 1))) < 4.5 * np.sqrt(0.21 / N)
     k, _ = connected_components(coo_matrix((np.ones(N), (u, v)), shape=(n, n)), directed=False)
     assert k == 1
+
+
+def test_hard_family_structure():
+    """Hard family: halves of k weighted Hamiltonian cycles (every vertex of a
+    half has the same weighted degree delta), c cross edges of total
+    C = floor(0.9 min delta), the same sizes as the standard family."""
+    k = 8
+    descs, edges, planted = cac.gen_planted_hard(0, 24, 5, 256, 4096, cycles=k)
+    std = cac.gen_planted(0, 24, 5, 256, 4096)[0]
+    assert np.array_equal(descs["n"], std["n"])
+    for i in range(len(descs)):
+        nv, e = _graph(descs, edges, i)
+        h = nv // 2
+        u, v, w = e["u"].astype(np.int64), e["v"].astype(np.int64), e["w"].astype(np.int64)
+        assert np.all(u < v) and np.all(v < nv)
+        cross = (u < h) & (v >= h)
+        c = int(cross.sum())
+        assert 1 <= c <= 8 and len(e) == k * nv + c
+        assert w[cross].sum() == planted[i] and w[cross].max() - w[cross].min() <= 1
+        inner = ~cross
+        assert np.all((w[inner] >= 1) & (w[inner] <= 100))
+        deg = np.bincount(u[inner], w[inner], nv) + np.bincount(v[inner], w[inner], nv)
+        da, db = deg[:h], deg[h:]
+        assert np.all(da == da[0]) and np.all(db == db[0])  # regular within each half
+        assert planted[i] == int(np.floor(0.9 * min(da[0], db[0])))
+        # each half is k cycles: every vertex has 2k edge ends inside its half
+        ends = np.bincount(u[inner], minlength=nv) + np.bincount(v[inner], minlength=nv)
+        assert np.all(ends == 2 * k)
+        g = coo_matrix((np.ones(inner.sum()), (u[inner], v[inner])), shape=(nv, nv))
+        nc, lab = connected_components(g, directed=False)
+        assert nc == 2
+
+
+def test_hard_family_planted_cut_is_the_min_cut(orc):
+    """C is the exact minimum cut (Stoer-Wagner, the oracle) and lies within
+    10% of the minimum weighted degree."""
+    descs, edges, planted = cac.gen_planted_hard(0, 10, 13, 64, 300, cycles=4)
+    for i in range(len(descs)):
+        nv, e = _graph(descs, edges, i)
+        adj = np.zeros((nv, nv), dtype=np.int64)
+        np.add.at(adj, (e["u"], e["v"]), e["w"].astype(np.int64))
+        adj += adj.T
+        assert orc.stoer_wagner(adj) == planted[i]
+        assert planted[i] >= 0.85 * adj.sum(axis=1).min()
+
+
+def test_hard_family_deterministic():
+    a = cac.gen_planted_hard(0, 6, 7, 256, 2048)
+    b = cac.gen_planted_hard(3, 3, 7, 256, 2048)
+    for j in range(3):
+        na, ea = _graph(a[0], a[1], 3 + j)
+        nb, eb = _graph(b[0], b[1], j)
+        assert na == nb and np.array_equal(ea, eb) and a[2][3 + j] == b[2][j]
