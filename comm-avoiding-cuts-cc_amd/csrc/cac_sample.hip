@@ -153,22 +153,40 @@ __device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t undo, uint32_
 // workgroups share a CU, at the price of a few spilled registers: for launches
 // of more workgroups than CUs.  kMinWaves = 1 leaves the registers free, for
 // small launches whose trials each have a CU to themselves.
-template <int kMinWaves>
+//
+// kLdsLab keeps the trial's labels in LDS as u16 (every vertex id below 2^16)
+// and writes them to S.lab once at the end.  The prefix scan reads two labels
+// per edge per round at random; with hundreds of trials in flight their label
+// arrays do not fit in L2, so from global memory each such read fetches a whole
+// line from HBM, many times the edge's own 16 bytes.
+//
+// Dynamic LDS: idx_cap u64 segment-index entries, then (kLdsLab) n u16 labels.
+template <int kMinWaves, bool kLdsLab>
 __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
-    k_sample_trials(const TrialScratch* __restrict__ scratch) {
+    k_sample_trials(const TrialScratch* __restrict__ scratch, uint32_t idx_cap) {
   __shared__ uint64_t lds[kSmpWaves + 2];
+  extern __shared__ uint64_t smp_dyn[];
   // Every G-th prefix value (the last of each segment of G edges): a draw
   // finds its segment here, then searches only that segment in HBM.
-  __shared__ uint64_t seg_last[kSmpIndex];
+  uint64_t* seg_last = smp_dyn;
+  uint16_t* slab = reinterpret_cast<uint16_t*>(smp_dyn + idx_cap);
   const TrialScratch S = scratch[blockIdx.x];
   const cac_edge* __restrict__ edges = S.edges;
   const uint64_t m = S.m;
-  uint32_t gshift = 0;  // segments of 2^gshift edges, at most kSmpIndex of them
-  while (((m + (1ull << gshift) - 1) >> gshift) > (uint64_t)kSmpIndex) ++gshift;
+  uint32_t gshift = 0;  // segments of 2^gshift edges, at most idx_cap of them
+  while (((m + (1ull << gshift) - 1) >> gshift) > (uint64_t)idx_cap) ++gshift;
   const uint32_t nseg = (uint32_t)((m + (1ull << gshift) - 1) >> gshift);
   const uint32_t n = S.n, t = S.t, trial = S.trial, seed = S.seed;
+  auto lab_get = [&](uint32_t v) -> uint32_t {
+    if constexpr (kLdsLab) return slab[v];
+    else return S.lab[v];
+  };
+  auto lab_set = [&](uint32_t v, uint32_t x) {
+    if constexpr (kLdsLab) slab[v] = (uint16_t)x;
+    else S.lab[v] = x;
+  };
   for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) {
-    S.lab[v] = v;
+    lab_set(v, v);
     S.par[v] = v;
   }
   block_sync_global();
@@ -198,7 +216,7 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
         w[k] = 0;
         if (e < m) {
           const cac_edge x = edges[e];
-          if (S.lab[x.u] != S.lab[x.v]) w[k] = x.w;
+          if (lab_get(x.u) != lab_get(x.v)) w[k] = x.w;
         }
         s += w[k];
       }
@@ -261,7 +279,7 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
         else lo = mid + 1;
       }
       const cac_edge x = edges[lo];
-      S.draws[d] = make_uint2(S.lab[x.u], S.lab[x.v]);
+      S.draws[d] = make_uint2(lab_get(x.u), lab_get(x.v));
     }
     block_sync_global();
     // 3. shortest draw prefix that leaves t super-vertices
@@ -278,9 +296,12 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
       if (last != lo) hooks = probe(S.draws, lo, last, S.par, lds);
     }  // else all D draws are kept, and the forest of the first probe stays
     // 4. canonical relabel
-    for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = uf_root(S.par, S.lab[v]);
+    for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) lab_set(v, uf_root(S.par, lab_get(v)));
     block_sync_global();
     nv -= hooks;
+  }
+  if constexpr (kLdsLab) {  // each thread copies the labels it relabelled last
+    for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) S.lab[v] = slab[v];
   }
   if (threadIdx.x == 0) *S.nv_out = nv;
 }
@@ -524,12 +545,48 @@ int densify_attr(cac_ctx* ctx) {
 }
 
 
+// Dynamic LDS a sampling workgroup may take: the CU's 160 KiB less the
+// kernel's static arrays, with room to spare.
+constexpr size_t kSmpLdsMax = 160 * 1024 - 1024;
+
+// Launch shape of k_sample_trials for trials of at most nmax vertices.
+struct SmpShape {
+  bool lds_lab;      // labels in LDS as u16 (nmax <= 2^16)
+  uint32_t idx_cap;  // segment-index entries
+  size_t dyn;        // dynamic LDS bytes
+};
+
+SmpShape sampling_shape(uint64_t nmax) {
+  SmpShape sh{};
+  sh.lds_lab = nmax <= 65536u;
+  const size_t lab_bytes = sh.lds_lab ? (((size_t)nmax * 2 + 15) & ~size_t(15)) : 0;
+  sh.idx_cap = (uint32_t)std::min<size_t>(kSmpIndex, ((kSmpLdsMax - lab_bytes) / 8) & ~size_t(63));
+  sh.dyn = (size_t)sh.idx_cap * 8 + lab_bytes;
+  return sh;
+}
+
+// The sampling kernels' dynamic LDS exceeds the default limit: raise it once.
+int sampling_attr(cac_ctx* ctx) {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    const void* fns[] = {(const void*)k_sample_trials<8, false>, (const void*)k_sample_trials<1, false>,
+                         (const void*)k_sample_trials<8, true>, (const void*)k_sample_trials<1, true>};
+    for (const void* f : fns) {
+      const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSmpLdsMax);
+      if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+  });
+  CAC_HIP(ctx, err);
+  return CAC_OK;
+}
+
 // Sampling workgroups the device runs at once (workgroups per CU times CUs).
 size_t sampling_wave(cac_ctx* ctx) {
   if (ctx->smp_wave == 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_trials<8>, kSmpBlock, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sample_trials<8, false>, kSmpBlock,
+                                                     (size_t)kSmpIndex * 8) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
       (void)hipGetLastError();
       per_cu = 1;
@@ -541,23 +598,34 @@ size_t sampling_wave(cac_ctx* ctx) {
 }
 
 
-// One sampling launch of nb trials, in the form suited to its size: a launch
-// of more trials than CUs runs two 1024-thread workgroups per CU (64 VGPRs,
-// some spilled) unless the context's tuning table says one for the largest
-// edge count mmax of the launch; a smaller launch gives each trial a CU of its
-// own with the registers free.  The draws do not depend on the form.
-void launch_sampling(cac_ctx* ctx, size_t nb, uint64_t mmax, const TrialScratch* d_scr, hipStream_t s) {
+// One sampling launch of nb trials of at most nmax vertices and mmax edges, in
+// the form suited to its size: a launch of more trials than CUs runs two
+// 1024-thread workgroups per CU (64 VGPRs, some spilled) when their LDS fits
+// twice in a CU and the context's tuning table does not say one for mmax; a
+// smaller launch gives each trial a CU of its own with the registers free.
+// The draws do not depend on the form.
+int launch_sampling(cac_ctx* ctx, size_t nb, uint64_t mmax, uint64_t nmax, const TrialScratch* d_scr,
+                    hipStream_t s) {
+  int rc = sampling_attr(ctx);
+  if (rc) return rc;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
     (void)hipGetLastError();
     cus = 1;
   }
+  const SmpShape sh = sampling_shape(nmax);
   const int cls = mmax < (1ull << 16) ? 0 : mmax < (1ull << 20) ? 1 : mmax < (1ull << 24) ? 2 : 3;
   const int per_cu = ctx->sample_occupancy[cls] ? ctx->sample_occupancy[cls] : 2;
-  if (nb > (size_t)cus && per_cu == 2)
-    hipLaunchKernelGGL(k_sample_trials<8>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, d_scr);
-  else
-    hipLaunchKernelGGL(k_sample_trials<1>, dim3((unsigned)nb), dim3(kSmpBlock), 0, s, d_scr);
+  const bool two = nb > (size_t)cus && per_cu == 2 && 2 * (sh.dyn + 1024) <= 160 * 1024;
+  const dim3 grid((unsigned)nb), block(kSmpBlock);
+  if (sh.lds_lab) {
+    if (two) hipLaunchKernelGGL((k_sample_trials<8, true>), grid, block, sh.dyn, s, d_scr, sh.idx_cap);
+    else hipLaunchKernelGGL((k_sample_trials<1, true>), grid, block, sh.dyn, s, d_scr, sh.idx_cap);
+  } else {
+    if (two) hipLaunchKernelGGL((k_sample_trials<8, false>), grid, block, sh.dyn, s, d_scr, sh.idx_cap);
+    else hipLaunchKernelGGL((k_sample_trials<1, false>), grid, block, sh.dyn, s, d_scr, sh.idx_cap);
+  }
+  return CAC_OK;
 }
 
 }  // namespace
@@ -711,7 +779,8 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int6
     prof_begin(ctx, "sample_trials", s);
     uint64_t mmax_jobs = 0;
     for (size_t i = 0; i < nb; ++i) mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
-    launch_sampling(ctx, nb, mmax_jobs, d_scr, s);
+    rc = launch_sampling(ctx, nb, mmax_jobs, nmax, d_scr, s);
+    if (rc) return rc;
     prof_end(ctx, s);
     CAC_LAUNCH_CHECK(ctx, "sample trials");
     std::vector<uint32_t> nv(nb);
@@ -866,12 +935,15 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
     if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
     if (rc == CAC_OK) {
       prof_begin(ctx, "sample_trials", s);
-      uint64_t mmax_jobs = 0;
-      for (size_t i = 0; i < nb; ++i) mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
-      launch_sampling(ctx, nb, mmax_jobs, d_scr, s);
+      uint64_t mmax_jobs = 0, nmax_jobs = 0;
+      for (size_t i = 0; i < nb; ++i) {
+        mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
+        nmax_jobs = std::max<uint64_t>(nmax_jobs, jobs[b0 + i].n);
+      }
+      rc = launch_sampling(ctx, nb, mmax_jobs, nmax_jobs, d_scr, s);
       prof_end(ctx, s);
       prof_begin(ctx, "densify_trials", s);
-      rc = densify_attr(ctx);
+      if (rc == CAC_OK) rc = densify_attr(ctx);
       if (rc == CAC_OK)
         hipLaunchKernelGGL(k_densify_trials, dim3((unsigned)nb), dim3(kSmpBlock), kDensifyLds, s,
                            (const TrialScratch*)d_scr, d_bounds, d_results);
@@ -948,7 +1020,8 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   if (rc) return rc;
   h = one[0];
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
-  launch_sampling(ctx, 1, m, d_scr, s);
+  rc = launch_sampling(ctx, 1, m, n, d_scr, s);
+  if (rc) return rc;
   CAC_LAUNCH_CHECK(ctx, "sample trial");
   uint32_t k = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&k, h.nv_out, 4, hipMemcpyDeviceToHost, s));

@@ -307,6 +307,10 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
   if (rc == CAC_OK && !jobs.empty()) {
     e = hipMalloc((void**)&d_res, 8 * jobs.size());
     if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
+    // the largest graphs' trials first: they are the launch's longest
+    // workgroups, so they start while the small ones fill the other CUs
+    // (results go by slot, so the order changes no cut)
+    std::stable_sort(jobs.begin(), jobs.end(), [](const SqrtJob& x, const SqrtJob& y) { return x.m > y.m; });
     if (rc == CAC_OK) rc = sqrt_cut_batch(ctx, jobs, d_res, s, budget);
   }
   std::vector<int64_t> raw(jobs.size());
