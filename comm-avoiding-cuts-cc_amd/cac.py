@@ -20,6 +20,7 @@ TUNE_KS_CONTRACT_BLOCK = 1
 TUNE_KS_CONTRACT_KEYS = (512, 1024, 2048, 4096)
 TUNE_SAMPLE_OCCUPANCY = 2
 TUNE_SAMPLE_OCCUPANCY_KEYS = (16, 20, 24, 32)
+TUNE_WIDE_SAMPLING = 3  # key 0: log2 of the smallest edge count run in the wide form (0 built-in, 63 never)
 
 CAC_OK = 0
 CAC_EINVAL = -1
@@ -213,7 +214,7 @@ class Context:
 
     def load_tuning(self, path: str | os.PathLike):
         """Apply a table written by autotune.py: {"ks_contract_block": {"512": B, ...},
-        "sample_occupancy": {"16": k, ...}}."""
+        "sample_occupancy": {"16": k, ...}, "wide_sampling_log2": v}."""
         import json
 
         table = json.loads(Path(path).read_text())
@@ -221,6 +222,8 @@ class Context:
             self.tune_set(TUNE_KS_CONTRACT_BLOCK, int(key), int(value))
         for key, value in table.get("sample_occupancy", {}).items():
             self.tune_set(TUNE_SAMPLE_OCCUPANCY, int(key), int(value))
+        if "wide_sampling_log2" in table:
+            self.tune_set(TUNE_WIDE_SAMPLING, 0, int(table["wide_sampling_log2"]))
         self.tuning_file = str(path)
 
     def close(self):

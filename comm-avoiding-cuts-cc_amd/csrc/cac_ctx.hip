@@ -269,7 +269,8 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 // Tuning table: CAC_TUNE_KS_CONTRACT_BLOCK keys 512, 1024, 2048, 4096 (row-size
 // class upper bounds), a value 0 (built-in rule) or a power of two from key / 8
 // (at least 64) to 1024; CAC_TUNE_SAMPLE_OCCUPANCY keys 16, 20, 24, 32 (log2 of
-// the edge-count class bound), a value 0 (built-in: 2), 1 or 2.
+// the edge-count class bound), a value 0 (built-in: 2), 1 or 2;
+// CAC_TUNE_WIDE_SAMPLING key 0, a value 0 (built-in: 2^20 edges) to 63 (never).
 static int tune_slot(int knob, uint32_t key) {
   if (knob == CAC_TUNE_KS_CONTRACT_BLOCK) {
     switch (key) {
@@ -289,6 +290,7 @@ static int tune_slot(int knob, uint32_t key) {
       default: return -1;
     }
   }
+  if (knob == CAC_TUNE_WIDE_SAMPLING) return key == 0 ? 0 : -1;
   return -1;
 }
 
@@ -299,6 +301,11 @@ int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value) {
   if (knob == CAC_TUNE_SAMPLE_OCCUPANCY) {
     if (value < 0 || value > 2) return cac::fail(ctx, CAC_EINVAL, "sampling workgroups per CU must be 0, 1 or 2");
     ctx->sample_occupancy[slot] = value;
+    return CAC_OK;
+  }
+  if (knob == CAC_TUNE_WIDE_SAMPLING) {
+    if (value < 0 || value > 63) return cac::fail(ctx, CAC_EINVAL, "wide sampling threshold must be 0 to 63");
+    ctx->wide_log2 = value;
     return CAC_OK;
   }
   const int32_t lo = key / 8 < 64 ? 64 : (int32_t)(key / 8);
@@ -312,7 +319,9 @@ int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value) {
   if (!ctx || !value) return CAC_EINVAL;
   const int slot = tune_slot(knob, key);
   if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
-  *value = knob == CAC_TUNE_SAMPLE_OCCUPANCY ? ctx->sample_occupancy[slot] : ctx->ks_contract_block[slot];
+  *value = knob == CAC_TUNE_SAMPLE_OCCUPANCY ? ctx->sample_occupancy[slot]
+           : knob == CAC_TUNE_WIDE_SAMPLING  ? ctx->wide_log2
+                                             : ctx->ks_contract_block[slot];
   return CAC_OK;
 }
 

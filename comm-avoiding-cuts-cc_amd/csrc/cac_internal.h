@@ -61,6 +61,9 @@ struct cac_ctx {
   // Sampling workgroups per CU for launches of more trials than CUs, by edge
   // count class m < 2^16, 2^20, 2^24, larger (cac_tune_set); 0 = built-in (2).
   int32_t sample_occupancy[4] = {0, 0, 0, 0};
+  // log2 of the smallest edge count whose sqrt-cut trials run in the wide
+  // (grid-per-trial) form: 0 = built-in (2^20), 63 = never
+  int32_t wide_log2 = 0;
   // The captured CSR connected-components step (cac_cc_csr_graph_dev) and the
   // arguments it was captured for: offsets, columns, labels, scratch, stream,
   // n, label mode.

@@ -35,6 +35,7 @@
 #include "cac_ks_kernels.h"
 #include "cac_scan.h"
 #include "cac_sort.h"
+#include "cac_trial.h"
 
 namespace cac {
 namespace {
@@ -42,27 +43,6 @@ namespace {
 constexpr int kSmpBlock = 1024;
 constexpr int kSmpWaves = kSmpBlock / 64;
 constexpr int kSmpIndex = 4096;  // LDS segment index of the weight prefix (32 KiB)
-
-// One sampling job = one sqrt-cut trial of one graph; a launch runs one
-// workgroup per job, so trials of different graphs share a launch.
-struct TrialScratch {
-  const cac_edge* edges;  // the job's graph (ids local to it)
-  uint64_t m;
-  uint32_t n, t;          // vertices, intermediate size
-  uint32_t trial, slot;   // Philox trial id, result slot
-  uint32_t seed;          // Philox key word 0 of the trial
-  uint32_t dpr;           // draws per round: 0 = |V| (low-concurrency rule), else that many
-  uint32_t* lab;          // [n]
-  uint32_t* par;          // [n]
-  uint64_t* pre;          // [m]
-  uint2* draws;           // [n]
-  int64_t* mat;           // [t x t]
-  int64_t* rsum;          // [t] row sums of mat (may be null)
-  uint32_t* nv_out;       // [1]
-  // [1] of the job's graph, nonzero when an edge is a loop or two weights
-  // differ (may be null: then every round builds the prefix)
-  const uint32_t* mixed;
-};
 
 // A distinct graph of a sampling launch and its flag word.
 struct EdgeSpan {
@@ -859,46 +839,37 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int6
   return CAC_OK;
 }
 
-// Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory:
-// one sampling launch, one densify launch and one breadth-first Karger-Stein
-// pass per chunk, whatever graphs the chunk's jobs belong to.  Raw cut of job
-// i goes to d_results[jobs[i].slot].
-int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
-                   size_t budget) {
-  if (jobs.empty()) return CAC_OK;
-  bool any_large = false;
-  for (const SqrtJob& j : jobs) any_large = any_large || j.t > kKsMaxS;
-  if (any_large) {
-    std::vector<SqrtJob> dense, large;
-    for (const SqrtJob& j : jobs) (j.t <= kKsMaxS ? dense : large).push_back(j);
-    const int rc = sqrt_large_batch(ctx, large, d_results, s);
-    if (rc) return rc;
-    return sqrt_cut_batch(ctx, dense, d_results, s, budget);
-  }
-  if (budget == 0) {
-    size_t avail = 0;
-    int rc0 = device_memory_available(ctx, &avail);
-    if (rc0) return rc0;
-    budget = std::max<size_t>((size_t)(0.45 * (double)avail), (size_t)1 << 28);
-  }
+// Dense-engine jobs (t <= 8192) in chunks sized to the budget: sampling,
+// densification, then one breadth-first Karger-Stein pass per chunk, whatever
+// graphs the chunk's jobs belong to.  wide: every job runs in the wide form
+// (its rounds as grid-wide kernels), else one workgroup per job.
+static int sqrt_dense_chunks(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
+                             size_t budget, bool wide) {
   int rc = CAC_OK;
   size_t b0 = 0;
   while (b0 < jobs.size() && rc == CAC_OK) {
-    size_t nb = 0, sample_bytes = 0, ks_bytes = 0;
-    while (b0 + nb < jobs.size() && nb < kMaxJobsPerChunk) {
+    size_t nb = 0, sample_bytes = 0, ks_bytes = 0, wide_bytes = 0;
+    const size_t cap = wide ? 4096 : kMaxJobsPerChunk;
+    while (b0 + nb < jobs.size() && nb < cap) {
       const SqrtJob& j = jobs[b0 + nb];
       const size_t a = job_sample_bytes(j, job_dpr(ctx, j.n)), k = job_ks_bytes(j);
-      if (nb > 0 && sample_bytes + a + ks_bytes + k > budget) break;
+      const size_t wb = wide ? wide_trial_bytes(j.m, j.n) : 0;
+      if (nb > 0 && sample_bytes + a + ks_bytes + k + wide_bytes + wb > budget) break;
       sample_bytes += a;
       ks_bytes += k;
+      wide_bytes += wb;
       ++nb;
     }
     // whole waves of sampling workgroups: a last partial wave would leave most
     // of the GPU idle while it runs
     const size_t wave = sampling_wave(ctx);
-    if (b0 + nb < jobs.size() && nb > wave) nb -= nb % wave;
+    if (!wide && b0 + nb < jobs.size() && nb > wave) nb -= nb % wave;
+    if (wide) {
+      wide_bytes = 0;
+      for (size_t i = 0; i < nb; ++i) wide_bytes += wide_trial_bytes(jobs[b0 + i].m, jobs[b0 + i].n);
+    }
     const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) +
-                         carve_size(nb * sizeof(EdgeSpan)) + carve_size(nb * 4) + 1024;
+                         carve_size(nb * sizeof(EdgeSpan)) + carve_size(nb * 4) + carve_size(wide_bytes) + 1024;
     rc = grow_buffer(ctx, &ctx->smp_buf, &ctx->smp_bytes, bytes);
     if (rc) return rc;
     BufPin pin(ctx, kBufSmp);  // the sampled matrices are the dense roots
@@ -909,6 +880,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
     int64_t* d_bounds = cv.take<int64_t>(nb);
     EdgeSpan* d_spans = cv.take<EdgeSpan>(nb);
     uint32_t* d_flags = cv.take<uint32_t>(nb);
+    void* wide_work = wide ? (void*)cv.take<unsigned char>(wide_bytes) : nullptr;
     std::vector<TrialScratch> scr(nb);
     for (size_t i = 0; i < nb; ++i) {
       const SqrtJob& j = jobs[b0 + i];
@@ -929,11 +901,20 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
       x.rsum = cv.take<int64_t>(j.t);
       x.nv_out = d_nv + i;
     }
-    rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
-    if (rc) return rc;
+    if (!wide) {
+      rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
+      if (rc) return rc;
+    }
     e = hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(ctx, e, "job descriptors");
-    if (rc == CAC_OK) {
+    if (rc == CAC_OK && wide) {
+      prof_begin(ctx, "wide_sample", s);
+      rc = wide_sample(ctx, scr, d_scr, wide_work, s);
+      prof_end(ctx, s);
+      prof_begin(ctx, "wide_densify", s);
+      if (rc == CAC_OK) rc = wide_densify(ctx, scr, d_scr, wide_work, d_bounds, d_results, s);
+      prof_end(ctx, s);
+    } else if (rc == CAC_OK) {
       prof_begin(ctx, "sample_trials", s);
       uint64_t mmax_jobs = 0, nmax_jobs = 0;
       for (size_t i = 0; i < nb; ++i) {
@@ -970,6 +951,35 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
     b0 += nb;
   }
   return rc;
+}
+
+// Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory.
+// Raw cut of job i goes to d_results[jobs[i].slot].
+int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
+                   size_t budget) {
+  if (jobs.empty()) return CAC_OK;
+  bool any_large = false;
+  for (const SqrtJob& j : jobs) any_large = any_large || j.t > kKsMaxS;
+  if (any_large) {
+    std::vector<SqrtJob> dense, large;
+    for (const SqrtJob& j : jobs) (j.t <= kKsMaxS ? dense : large).push_back(j);
+    const int rc = sqrt_large_batch(ctx, large, d_results, s);
+    if (rc) return rc;
+    return sqrt_cut_batch(ctx, dense, d_results, s, budget);
+  }
+  if (budget == 0) {
+    size_t avail = 0;
+    int rc0 = device_memory_available(ctx, &avail);
+    if (rc0) return rc0;
+    budget = std::max<size_t>((size_t)(0.45 * (double)avail), (size_t)1 << 28);
+  }
+  // trials of large edge lists run in the wide form, the rest one workgroup each
+  const uint64_t wmin = wide_min_edges(ctx);
+  std::vector<SqrtJob> narrow, wide;
+  for (const SqrtJob& j : jobs) (j.m >= wmin ? wide : narrow).push_back(j);
+  int rc = sqrt_dense_chunks(ctx, wide, d_results, s, budget, true);
+  if (rc) return rc;
+  return sqrt_dense_chunks(ctx, narrow, d_results, s, budget, false);
 }
 
 // Trials trial_ids of one graph; raw cut of trial_ids[i] to d_results[i].

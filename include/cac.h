@@ -91,6 +91,12 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
  * thread, some spilled); 1 gives the workgroup the registers it wants;
  * CAC_EINVAL else. */
 #define CAC_TUNE_SAMPLE_OCCUPANCY 2
+/* CAC_TUNE_WIDE_SAMPLING: key 0; value v = log2 of the smallest edge count
+ * whose square-root trials run in the wide form (each trial's sampling rounds
+ * and densification as grid-wide kernels instead of one workgroup per trial).
+ * v = 0 restores the built-in 20 (2^20 edges), v = 63 never uses the wide form,
+ * 1 <= v <= 62 sets 2^v; CAC_EINVAL else.  Cuts do not depend on it. */
+#define CAC_TUNE_WIDE_SAMPLING 3
 int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value);
 int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value);
 

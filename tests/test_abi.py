@@ -47,6 +47,8 @@ def test_tuning_file_shape():
     occ = json.loads(cac.TUNING_PATH.read_text()).get("sample_occupancy", {})
     for key, k in occ.items():
         assert int(key) in cac.TUNE_SAMPLE_OCCUPANCY_KEYS and k in (0, 1, 2)
+    wide = json.loads(cac.TUNING_PATH.read_text()).get("wide_sampling_log2", 0)
+    assert 0 <= int(wide) <= 63
 
 
 def test_record_layouts():
