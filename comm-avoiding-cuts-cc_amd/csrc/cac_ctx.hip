@@ -61,12 +61,14 @@ void prof_begin(cac_ctx* ctx, const char* name, hipStream_t s) {
   if (!ctx->profiling) return;
   cac_prof_rec r{name, take_event(ctx), take_event(ctx)};
   if (r.start) (void)hipEventRecord(r.start, s);
+  ctx->prof_open.push_back(ctx->prof.size());
   ctx->prof.push_back(r);
 }
 
 void prof_end(cac_ctx* ctx, hipStream_t s) {
-  if (!ctx->profiling || ctx->prof.empty()) return;
-  cac_prof_rec& r = ctx->prof.back();
+  if (!ctx->profiling || ctx->prof_open.empty()) return;
+  cac_prof_rec& r = ctx->prof[ctx->prof_open.back()];
+  ctx->prof_open.pop_back();
   if (r.stop) (void)hipEventRecord(r.stop, s);
 }
 
@@ -102,6 +104,7 @@ static void release_unpinned(cac_ctx* ctx, void** keep) {
   drop(kBufLt, &ctx->lt_buf, &ctx->lt_bytes);
   drop(kBufSpDense, &ctx->sp_dense_buf, &ctx->sp_dense_bytes);
   drop(kBufSpTmp, &ctx->sp_tmp, &ctx->sp_tmp_bytes);
+  drop(kBufServe, &ctx->serve_buf, &ctx->serve_bytes);
   for (size_t d = 0; d < ctx->sp_bufs.size(); ++d) drop(kBufSpDepth + (int)d, &ctx->sp_bufs[d], &ctx->sp_bytes[d]);
 }
 
@@ -182,6 +185,7 @@ int cac_ctx_destroy(cac_ctx* ctx) {
   if (ctx->sp_dense_buf) (void)hipFree(ctx->sp_dense_buf);
   if (ctx->sp_tmp) (void)hipFree(ctx->sp_tmp);
   if (ctx->lt_buf) (void)hipFree(ctx->lt_buf);
+  if (ctx->serve_buf) (void)hipFree(ctx->serve_buf);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -340,6 +344,7 @@ int cac_prof_reset(cac_ctx* ctx) {
     if (r.stop) ctx->event_pool.push_back(r.stop);
   }
   ctx->prof.clear();
+  ctx->prof_open.clear();
   return CAC_OK;
 }
 

@@ -25,6 +25,7 @@ struct cac_ctx {
   std::string last_error;
   bool profiling = false;
   std::vector<cac_prof_rec> prof;
+  std::vector<size_t> prof_open;  // records begun and not yet ended (ranges nest)
   std::vector<hipEvent_t> event_pool;
   // Two grow-only buffers for the levels of the Karger-Stein recursion (parents
   // in one, children in the other).
@@ -49,6 +50,9 @@ struct cac_ctx {
   // merged contracted edge list of one trial.
   void* lt_buf = nullptr;
   size_t lt_bytes = 0;
+  // a served launch group's uploads, checks and results
+  void* serve_buf = nullptr;
+  size_t serve_bytes = 0;
   // Grow-only buffers whose contents are live (ids below); the others may be
   // released when an allocation fails.
   std::vector<int> pins;
@@ -90,7 +94,7 @@ int scratch_reserve(cac_ctx* ctx, size_t bytes, void** base);
 int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes);
 
 // Ids of the context's grow-only buffers, for pinning.
-enum : int { kBufKs0 = 0, kBufKs1, kBufKsRetry, kBufSmp, kBufLt, kBufSpDense, kBufSpTmp, kBufSpDepth = 16 };
+enum : int { kBufKs0 = 0, kBufKs1, kBufKsRetry, kBufSmp, kBufLt, kBufSpDense, kBufSpTmp, kBufServe, kBufSpDepth = 16 };
 
 // Keeps a grow-only buffer from being released while its contents are live.
 struct BufPin {
@@ -134,7 +138,8 @@ struct Carver {
 };
 inline size_t carve_size(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
-// Kernel timing: brackets one launch with events when profiling is enabled.
+// Kernel timing: brackets one launch (or a nested range of them) with events
+// when profiling is enabled; prof_end closes the innermost open range.
 void prof_begin(cac_ctx* ctx, const char* name, hipStream_t s);
 void prof_end(cac_ctx* ctx, hipStream_t s);
 

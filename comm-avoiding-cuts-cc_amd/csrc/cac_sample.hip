@@ -673,12 +673,22 @@ size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t);
 // per chunk (the super-vertex counts, then the list ranges and bounds); then
 // one sparse Karger-Stein pass over all of the chunk's trials (node 1 each).
 // Raw cut of each job to d_results[slot].
-static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s) {
-  auto job_bytes = [ctx](const SqrtJob& j) {
+static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs_in, int64_t* d_results,
+                            hipStream_t s) {
+  // trials of large edge lists sample in the wide form: they go last in every
+  // chunk, after the workgroup-per-trial ones
+  const uint64_t wmin = wide_min_edges(ctx);
+  std::vector<SqrtJob> jobs;
+  for (const SqrtJob& j : jobs_in)
+    if (j.m < wmin) jobs.push_back(j);
+  for (const SqrtJob& j : jobs_in)
+    if (j.m >= wmin) jobs.push_back(j);
+  auto job_bytes = [ctx, wmin](const SqrtJob& j) {
     const uint64_t m = std::max<uint64_t>(j.m, 1);
     return carve_size(sizeof(TrialScratch)) + 3 * carve_size((size_t)j.n * 4) + carve_size(m * 8) +
            carve_size((size_t)std::max(j.n, job_dpr(ctx, j.n)) * 8) + carve_size(64) + carve_size(((size_t)j.t + 2) * 8) +
-           carve_size(sizeof(EdgeSpan)) + carve_size(4) + 64 * m;  // 64 m: the shared merge buffers
+           carve_size(sizeof(EdgeSpan)) + carve_size(4) + 64 * m +  // 64 m: the shared merge buffers
+           (j.m >= wmin ? wide_trial_bytes(j.m, j.n) : 0);
   };
   size_t avail = 0;
   int rc = device_memory_available(ctx, &avail);
@@ -707,9 +717,16 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int6
     const int key_bits = (int)(2 * bk + jb);
     if (key_bits > 64) return fail(ctx, CAC_EINVAL, "contracted graph too large for 64-bit merge keys");
     const uint64_t Np = (std::max<uint64_t>(N, 1) + 31) & ~uint64_t(31);
+    size_t w0 = 0, wide_bytes = 0;  // jobs [b0, b0 + w0) run one workgroup each, the rest wide
+    for (size_t i = 0; i < nb; ++i) {
+      const SqrtJob& j = jobs[b0 + i];
+      if (j.m < wmin) ++w0;
+      else wide_bytes += wide_trial_bytes(j.m, j.n);
+    }
     const size_t shared = carve_size(Np * 16) * 2 + carve_size((Np + 1) * 4) + carve_size(radix_sort_temp_bytes(Np)) +
                           carve_size(segmented_sum_temp_bytes(Np)) + carve_size((scan_tiles(nmax) + 64) * 4) +
-                          carve_size((nb + 1) * 8) + carve_size((nb + 1) * 4) + carve_size(nb * 8) + 4096;
+                          carve_size((nb + 1) * 8) + carve_size((nb + 1) * 4) + carve_size(nb * 8) +
+                          carve_size(wide_bytes) + 4096;
     rc = grow_buffer(ctx, &ctx->lt_buf, &ctx->lt_bytes, per + shared);
     if (rc) return rc;
     BufPin pin(ctx, kBufLt);  // the merged edge lists are the sparse roots
@@ -726,6 +743,7 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int6
     TrialScratch* d_scr = cv.take<TrialScratch>(nb);
     EdgeSpan* d_spans = cv.take<EdgeSpan>(nb);
     uint32_t* d_flags = cv.take<uint32_t>(nb);
+    void* wide_work = cv.take<unsigned char>(std::max<size_t>(wide_bytes, 1));
     std::vector<TrialScratch> scr(nb);
     std::vector<uint32_t*> rank(nb);
     std::vector<unsigned long long*> deg(nb);
@@ -751,16 +769,27 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int6
       deg[i] = cv.take<unsigned long long>((size_t)j.t + 2);  // [t] degrees, [t + 1] their minimum
       offsets[i + 1] = offsets[i] + j.m;
     }
-    rc = launch_mixed_flags(ctx, scr, d_spans, d_flags, s);
+    std::vector<TrialScratch> nar(scr.begin(), scr.begin() + w0), wid(scr.begin() + w0, scr.end());
+    rc = launch_mixed_flags(ctx, nar, d_spans, d_flags, s);
     if (rc) return rc;
+    for (size_t i = 0; i < w0; ++i) scr[i] = nar[i];
     CAC_HIP(ctx, hipMemcpyAsync(d_scr, scr.data(), nb * sizeof(TrialScratch), hipMemcpyHostToDevice, s));
     CAC_HIP(ctx, hipMemcpyAsync(d_offsets, offsets.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
     CAC_HIP(ctx, hipMemcpyAsync(d_deg, deg.data(), nb * sizeof(void*), hipMemcpyHostToDevice, s));
     prof_begin(ctx, "sample_trials", s);
-    uint64_t mmax_jobs = 0;
-    for (size_t i = 0; i < nb; ++i) mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
-    rc = launch_sampling(ctx, nb, mmax_jobs, nmax, d_scr, s);
-    if (rc) return rc;
+    if (w0 > 0) {
+      uint64_t mmax_jobs = 0, nmax_jobs = 0;
+      for (size_t i = 0; i < w0; ++i) {
+        mmax_jobs = std::max<uint64_t>(mmax_jobs, jobs[b0 + i].m);
+        nmax_jobs = std::max<uint64_t>(nmax_jobs, jobs[b0 + i].n);
+      }
+      rc = launch_sampling(ctx, w0, mmax_jobs, nmax_jobs, d_scr, s);
+      if (rc) return rc;
+    }
+    if (w0 < nb) {
+      rc = wide_sample(ctx, wid, d_scr + w0, wide_work, s);
+      if (rc) return rc;
+    }
     prof_end(ctx, s);
     CAC_LAUNCH_CHECK(ctx, "sample trials");
     std::vector<uint32_t> nv(nb);
@@ -866,7 +895,8 @@ static int sqrt_dense_chunks(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int
     if (!wide && b0 + nb < jobs.size() && nb > wave) nb -= nb % wave;
     if (wide) {
       wide_bytes = 0;
-      for (size_t i = 0; i < nb; ++i) wide_bytes += wide_trial_bytes(jobs[b0 + i].m, jobs[b0 + i].n);
+      for (size_t i = 0; i < nb; ++i)
+        wide_bytes += wide_trial_bytes(jobs[b0 + i].m, jobs[b0 + i].n);
     }
     const size_t bytes = sample_bytes + carve_size(nb * sizeof(TrialScratch)) + 2 * carve_size(nb * 8) +
                          carve_size(nb * sizeof(EdgeSpan)) + carve_size(nb * 4) + carve_size(wide_bytes) + 1024;

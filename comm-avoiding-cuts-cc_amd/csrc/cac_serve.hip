@@ -14,7 +14,10 @@
 //     query's seed, so queries of any seeds share launches.
 // Groups on different workers overlap on the device.  A query is admitted only
 // when its footprint estimate fits beside the groups in flight, so requests
-// that arrive while others run are admitted as memory and workers free up.
+// that arrive while others run are admitted as memory and workers free up.  A
+// min-cut group runs its trials in chunks, so it is charged its uploads plus
+// at most an equal per-worker share of the budget for scratch (or its largest
+// query's scratch, if more); that share is the chunk budget of its batch.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -27,6 +30,7 @@
 #include "cac_common.h"
 #include "cac_internal.h"
 #include "cac_ks.h"
+#include "cac_trial.h"
 
 namespace cac {
 
@@ -47,7 +51,9 @@ struct Pending {
   uint64_t id;
   cac_query q;
   double submit_s;
-  size_t bytes;
+  size_t bytes;             // upload + scratch
+  size_t upload;            // its edges, when it brings them
+  size_t scratch;           // device scratch of its work (for a min cut: its trials' chunk)
   const cac_edge* d_edges;  // a registered graph's device edges (checked once), or null
 };
 
@@ -76,17 +82,30 @@ void mincut_plan(uint32_t n, uint64_t m, double p, uint32_t trials, uint32_t* t,
   }
 }
 
-// Device bytes a query holds while it runs; a registered graph's edges are
-// already resident and not counted again.
-size_t footprint(const cac_query& q, bool resident) {
-  const size_t upload = resident ? 0 : carve_size(q.m * sizeof(cac_edge));
-  if (q.kind == CAC_QUERY_CC)
-    return upload + carve_size(q.m * sizeof(cac_uedge)) + carve_size((size_t)q.n * 4) + cc_tmp_words(q.m, q.n, 0) * 4 +
-           4096;
+// Device bytes a query holds while it runs: its upload (a registered graph's
+// edges are already resident and not counted again) and its scratch.  A min
+// cut's scratch covers a chunk of up to kServeChunkTrials trials, with the
+// wide form's buffers for edge lists of 2^20 edges or more.
+void footprint(const cac_query& q, bool resident, size_t* upload, size_t* scratch) {
+  *upload = resident ? 0 : carve_size(q.m * sizeof(cac_edge));
+  if (q.kind == CAC_QUERY_CC) {
+    *scratch = carve_size(q.m * sizeof(cac_uedge)) + carve_size((size_t)q.n * 4) + cc_tmp_words(q.m, q.n, 0) * 4 + 4096;
+    return;
+  }
   uint32_t t, T;
   mincut_plan(q.n, q.m, q.p, q.trials, &t, &T);
   SqrtJob j{nullptr, q.m, q.n, t, 0, 0, 0};
-  return upload + (size_t)std::min(T, kServeChunkTrials) * (job_sample_bytes(j) + job_ks_bytes(j)) + 4096;
+  const size_t wide = q.m >= (1ull << 20) ? wide_trial_bytes(q.m, q.n) : 0;
+  *scratch = (size_t)std::min(T, kServeChunkTrials) * (job_sample_bytes(j) + job_ks_bytes(j) + wide) + 4096;
+}
+
+// Bytes a launch group holds at once.  CC graphs are laid side by side, so a
+// CC group holds all of its scratch.  A min-cut group runs its trials in
+// chunks sized to the bytes it was admitted with, so it holds at most
+// max(cap, its largest query's scratch) beside its uploads.
+size_t group_bytes(int kind, size_t upload, size_t scratch, size_t scratch_max, size_t cap) {
+  if (kind == CAC_QUERY_CC) return upload + scratch;
+  return upload + std::min(scratch, std::max(cap, scratch_max));
 }
 
 }  // namespace
@@ -202,15 +221,17 @@ void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query
     M += grp[i].q.m;
   }
   const size_t tw = cc_tmp_words(M, (uint32_t)N, CAC_LABEL_MIN_ID);
-  void* base = nullptr;
-  int rc = CAC_OK;
-  hipError_t e = hipMalloc(&base, carve_size(M * sizeof(cac_edge)) + carve_size(M * sizeof(cac_uedge)) +
-                                      carve_size(N * 4) + carve_size(tw * 4) + carve_size(grp.size() * 4) + 2048);
-  if (e != hipSuccess) rc = hip_fail(ctx, e, "serve cc buffers");
+  // the worker's grow-only buffer: no allocation (and no device-wide wait
+  // of a free) per group once it has grown
+  int rc = grow_buffer(ctx, &ctx->serve_buf, &ctx->serve_bytes,
+                       carve_size(M * sizeof(cac_edge)) + carve_size(M * sizeof(cac_uedge)) + carve_size(N * 4) +
+                           carve_size(tw * 4) + carve_size(grp.size() * 4) + 2048);
+  BufPin pin(ctx, kBufServe);
+  hipError_t e = hipSuccess;
   std::vector<uint32_t> lab(N), bad(grp.size(), 0);
   hipStream_t s = ctx->stream;
   if (rc == CAC_OK) {
-    Carver cv(base);
+    Carver cv(ctx->serve_buf);
     cac_edge* d_in = cv.take<cac_edge>(M);
     cac_uedge* d_e = cv.take<cac_uedge>(M);
     uint32_t* d_l = cv.take<uint32_t>(N);
@@ -238,7 +259,6 @@ void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query
     }
   }
   (void)hipStreamSynchronize(s);
-  if (base) (void)hipFree(base);
   for (size_t i = 0; i < grp.size(); ++i) {
     cac_query_result& r = res[i];
     r.status = rc != CAC_OK ? rc : (bad[i] ? CAC_EINVAL : CAC_OK);
@@ -256,15 +276,30 @@ void run_cc_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query
 
 void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_query_result>& res, size_t budget) {
   uint64_t M = 0;  // edges to upload: registered graphs are already on the device
-  for (auto& p : grp) M += p.d_edges ? 0 : p.q.m;
-  int rc = CAC_OK;
+  size_t jobs_max = 0;
+  for (auto& p : grp) {
+    M += p.d_edges ? 0 : p.q.m;
+    uint32_t t, T;
+    mincut_plan(p.q.n, p.q.m, p.q.p, p.q.trials, &t, &T);
+    jobs_max += T;
+  }
+  // the worker's grow-only buffer: no allocation (and no device-wide wait of
+  // a free) per group once it has grown
+  int rc = grow_buffer(ctx, &ctx->serve_buf, &ctx->serve_bytes,
+                       carve_size(std::max<uint64_t>(M, 1) * sizeof(cac_edge)) + carve_size(24 * grp.size()) +
+                           carve_size(8 * std::max<size_t>(jobs_max, 1)) + 1024);
+  BufPin pin(ctx, kBufServe);
   hipStream_t s = ctx->stream;
   cac_edge* d_e = nullptr;
   int64_t* d_res = nullptr;
   unsigned long long* d_chk = nullptr;
-  hipError_t e = hipMalloc((void**)&d_e, std::max<uint64_t>(M, 1) * sizeof(cac_edge));
-  if (e == hipSuccess) e = hipMalloc((void**)&d_chk, 24 * grp.size());
-  if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut buffers");
+  hipError_t e = hipSuccess;
+  if (rc == CAC_OK) {
+    Carver cv(ctx->serve_buf);
+    d_e = cv.take<cac_edge>(std::max<uint64_t>(M, 1));
+    d_chk = cv.take<unsigned long long>(3 * grp.size());
+    d_res = cv.take<int64_t>(std::max<size_t>(jobs_max, 1));
+  }
   std::vector<uint64_t> eoff(grp.size());
   std::vector<unsigned long long> chk(3 * grp.size());
   if (rc == CAC_OK) {
@@ -305,8 +340,6 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
   }
   first[grp.size()] = jobs.size();
   if (rc == CAC_OK && !jobs.empty()) {
-    e = hipMalloc((void**)&d_res, 8 * jobs.size());
-    if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
     // the largest graphs' trials first: they are the launch's longest
     // workgroups, so they start while the small ones fill the other CUs
     // (results go by slot, so the order changes no cut)
@@ -320,9 +353,6 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
     if (e != hipSuccess) rc = hip_fail(ctx, e, "serve mincut results");
   }
   (void)hipStreamSynchronize(s);
-  if (d_res) (void)hipFree(d_res);
-  if (d_chk) (void)hipFree(d_chk);
-  if (d_e) (void)hipFree(d_e);
   for (size_t i = 0; i < grp.size(); ++i) {
     if (rc != CAC_OK) res[i].status = rc;
     if (res[i].status == CAC_OK)
@@ -333,7 +363,7 @@ void run_mincut_group(cac_ctx* ctx, std::vector<Pending>& grp, std::vector<cac_q
 void worker_loop(cac_server* S, cac_ctx* ctx) {
   for (;;) {
     std::vector<Pending> grp;
-    size_t bytes = 0;
+    size_t bytes = 0, chunk_budget = 0;
     double admit = 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -344,12 +374,18 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
       }
       Pending head = S->pending.front();
       S->pending.pop_front();
-      bytes = head.bytes;
+      // a min-cut group's scratch is capped at an equal share of the budget
+      // per worker: its trials then run in chunks of that size
+      const size_t cap = S->budget / std::max<size_t>(S->ctxs.size(), 1);
+      size_t upload = head.upload, scratch = head.scratch, scratch_max = head.scratch;
+      bytes = group_bytes(head.q.kind, upload, scratch, scratch_max, cap);
       uint64_t edges = head.q.m;
       grp.push_back(head);
       for (auto it = S->pending.begin(); it != S->pending.end() && grp.size() < S->max_batch;) {
         const bool same = it->q.kind == head.q.kind;
-        const bool fits = S->inflight + S->resident + bytes + it->bytes <= S->budget &&
+        const size_t with = group_bytes(head.q.kind, upload + it->upload, scratch + it->scratch,
+                                        std::max(scratch_max, it->scratch), cap);
+        const bool fits = S->inflight + S->resident + with <= S->budget &&
                           (head.q.kind != CAC_QUERY_CC || edges + it->q.m <= kServeMaxGroupEdges);
         const bool ids_fit = head.q.kind != CAC_QUERY_CC || [&] {
           uint64_t nsum = 0;
@@ -357,7 +393,10 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
           return nsum + it->q.n < (1ull << 32);
         }();
         if (same && fits && ids_fit) {
-          bytes += it->bytes;
+          upload += it->upload;
+          scratch += it->scratch;
+          scratch_max = std::max(scratch_max, it->scratch);
+          bytes = with;
           edges += it->q.m;
           grp.push_back(*it);
           it = S->pending.erase(it);
@@ -369,6 +408,7 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
       S->peak_inflight = std::max(S->peak_inflight, S->inflight);
       ++S->groups;
       admit = S->now();
+      chunk_budget = bytes - upload;
     }
     std::vector<cac_query_result> res(grp.size());
     for (size_t i = 0; i < grp.size(); ++i) {
@@ -380,7 +420,7 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
     if (grp[0].q.kind == CAC_QUERY_CC)
       run_cc_group(ctx, grp, res);
     else
-      run_mincut_group(ctx, grp, res, bytes);
+      run_mincut_group(ctx, grp, res, chunk_budget);
     const double fin = S->now();
     std::vector<cac_edge*> freed;
     {
@@ -469,7 +509,9 @@ int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
     }
   }
   if (v == CAC_OK) v = cac::validate(p.q, p.d_edges != nullptr);
-  p.bytes = v == CAC_OK ? cac::footprint(p.q, p.d_edges != nullptr) : 0;
+  p.upload = p.scratch = 0;
+  if (v == CAC_OK) cac::footprint(p.q, p.d_edges != nullptr, &p.upload, &p.scratch);
+  p.bytes = p.upload + p.scratch;
   p.id = S->next_id++;
   p.submit_s = S->now();
   if (id_out) *id_out = p.id;

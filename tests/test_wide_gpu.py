@@ -163,3 +163,47 @@ def test_wide_equals_workgroup_form_large(narrow):
     raw_n = narrow.sqrt_cut_trials(E, n, t, 6, 0, 4)
     _say(f"wide vs workgroup form n {n} m {m} t {t}: {raw_w.tolist()} {raw_n.tolist()}")
     assert raw_w.tolist() == raw_n.tolist()
+
+
+def test_wide_beyond_dense_engine(narrow):
+    # t > 8192 (m above 16.7M): the contracted graphs become merged edge lists
+    # for the sparse Karger-Stein; the sampling before it runs wide by default
+    rng = np.random.default_rng(23)
+    n, m = 50_000, 17_000_000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    v = np.where(u == v, (v + 1) % n, v)
+    E = np.stack([np.minimum(u, v), np.maximum(u, v), rng.integers(1, 10, m)], 1).astype(np.uint64)
+    t = cac.intermediate_size(n, m)
+    assert t > 8192
+    with cac.Context(0) as c:
+        c.tune_set(cac.TUNE_WIDE_SAMPLING, 0, 0)
+        raw_w = c.sqrt_cut_trials(E, n, t, 8, 0, 3)
+    raw_n = narrow.sqrt_cut_trials(E, n, t, 8, 0, 3)
+    _say(f"beyond dense engine n {n} m {m} t {t}: wide {raw_w.tolist()} workgroup {raw_n.tolist()}")
+    assert raw_w.tolist() == raw_n.tolist()
+
+
+def test_wide_profiled_solve():
+    # with profiling on, a wide solve records its timed ranges; every record
+    # closes and reads back, and the cuts do not change
+    rng = np.random.default_rng(29)
+    n, m = 1 << 17, 1 << 21
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    v = np.where(u == v, (v + 1) % n, v)
+    E = np.stack([np.minimum(u, v), np.maximum(u, v), rng.integers(1, 100, m)], 1).astype(np.uint64)
+    t = min(cac.intermediate_size(n, m), 4096)
+    with cac.Context(0) as c:
+        c.tune_set(cac.TUNE_WIDE_SAMPLING, 0, 20)
+        want = c.sqrt_cut_trials(E, n, t, 2, 0, 3)
+        c.prof_reset()
+        c.prof_enable(True)
+        got = c.sqrt_cut_trials(E, n, t, 2, 0, 3)
+        recs = c.prof_records()
+        c.prof_enable(False)
+    assert got.tolist() == want.tolist()
+    names = [k for k, _ in recs]
+    _say(f"profiled wide solve: {len(recs)} records, names {sorted(set(names))}")
+    assert "wide_sample" in names and "wide_densify" in names
+    assert all(ms >= 0 for _, ms in recs)
