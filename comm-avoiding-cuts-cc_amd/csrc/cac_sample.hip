@@ -22,6 +22,7 @@
 // which feeds one dense Karger-Stein trial per sqrt-cut trial.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -665,6 +666,37 @@ size_t job_sample_bytes(const SqrtJob& j, uint32_t dpr) {
 // children, and two levels are alive at once.
 size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t); }
 
+// Edge-count bound from which a job of the batch runs in the wide form.  One
+// workgroup scans a trial's edges in time proportional to m on one CU, while
+// the workgroup-form jobs keep every CU busy for about (their sum of m) / CUs;
+// a trial above that is the launch's straggler.  From the largest m down, a job
+// goes wide (if m reaches the context's threshold) while it would still be a
+// straggler beside the jobs left to the workgroup form, which spend fewer
+// CU-seconds per edge and so take the rest.
+static uint64_t wide_cut(cac_ctx* ctx, const std::vector<SqrtJob>& jobs) {
+  const uint64_t wmin = wide_min_edges(ctx);
+  if (wmin == ~0ull) return wmin;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1) {
+    (void)hipGetLastError();
+    cus = 1;
+  }
+  std::vector<uint64_t> ms;
+  uint64_t narrow_total = 0;
+  for (const SqrtJob& j : jobs) {
+    ms.push_back(j.m);
+    narrow_total += j.m;
+  }
+  std::sort(ms.begin(), ms.end(), std::greater<uint64_t>());
+  uint64_t cut = ~0ull;
+  for (const uint64_t m : ms) {
+    if (m < wmin || (long double)m * cus <= (long double)narrow_total) break;
+    cut = m;
+    narrow_total -= m;
+  }
+  return cut;
+}
+
 // Trials with t > 8192, in chunks sized to the free memory: one sampling
 // launch for the chunk; then the contracted edge lists of all its trials at
 // once -- endpoints ranked per trial, keys (trial, a, b), one radix sort of
@@ -677,7 +709,7 @@ static int sqrt_large_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs_in, i
                             hipStream_t s) {
   // trials of large edge lists sample in the wide form: they go last in every
   // chunk, after the workgroup-per-trial ones
-  const uint64_t wmin = wide_min_edges(ctx);
+  const uint64_t wmin = wide_cut(ctx, jobs_in);
   std::vector<SqrtJob> jobs;
   for (const SqrtJob& j : jobs_in)
     if (j.m < wmin) jobs.push_back(j);
@@ -1004,7 +1036,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
     budget = std::max<size_t>((size_t)(0.45 * (double)avail), (size_t)1 << 28);
   }
   // trials of large edge lists run in the wide form, the rest one workgroup each
-  const uint64_t wmin = wide_min_edges(ctx);
+  const uint64_t wmin = wide_cut(ctx, jobs);
   std::vector<SqrtJob> narrow, wide;
   for (const SqrtJob& j : jobs) (j.m >= wmin ? wide : narrow).push_back(j);
   int rc = sqrt_dense_chunks(ctx, wide, d_results, s, budget, true);
