@@ -24,8 +24,17 @@ trials of one graph of that size (the trials of a batch or of one large solve
 run that way), and the faster setting whose raw cuts equal the built-in
 setting's wins.
 
+Knob CAC_TUNE_KS_COMPACT_BLOCK: threads per workgroup (256, 512, 1024) of the
+dense Karger-Stein compaction, per child-size class (keys 512, 1024, 2048,
+4096, 8192: t' <= 512, then t' in (key/2, key]).  One solve of 57 trials on the
+complete graph n = 8192 holds children of every class; each class is swept,
+largest first, with the classes above at their tuned values and the ones below
+at the built-in 512, and the fastest setting whose raw cuts equal the built-in
+rule's wins.
+
 Writes {"device": ..., "ks_contract_block": {"512": B, ...},
-"sample_occupancy": {"16": k, ...}, "timings_ms": ...} to tuning/gfx950.json
+"sample_occupancy": {"16": k, ...}, "ks_compact_block": {"512": B, ...},
+"timings_ms": ...} to tuning/gfx950.json
 next to this file (or --out); every cac.Context applies that file when it
 exists.  Run on the target GPU:
 
@@ -143,25 +152,61 @@ def tune_sampling(device: int = 0, reps: int = 3, log=print):
     return table, timings
 
 
+COMPACT_BLOCKS = (256, 512, 1024)
+
+
+def tune_compact(device: int = 0, reps: int = 3, log=print):
+    n, trials = 8192, 57
+    table, timings = {}, {}
+    with cac.Context(device, tuning=False) as ctx:
+        d_adj = ctx.alloc(n * n * 8)
+        ctx.gen_complete_dev(n, 0, 42, d_adj.ptr)
+        base_ms, base_raw = time_solve(ctx, d_adj.ptr, n, trials, reps)
+        for key in reversed(cac.TUNE_KS_COMPACT_KEYS):
+            res = {}
+            for b in COMPACT_BLOCKS:
+                ctx.tune_set(cac.TUNE_KS_COMPACT_BLOCK, key, b)
+                ms, raw = time_solve(ctx, d_adj.ptr, n, trials, reps)
+                if raw.tolist() != base_raw.tolist():
+                    raise RuntimeError(f"compaction block {b} for class {key} changed a raw cut")
+                res[b] = ms
+                log(f"compaction class {key}: block {b:4d}  {ms:8.2f} ms  ({trials} trials, n={n})")
+            best = min(res, key=res.get)
+            table[str(key)] = best
+            ctx.tune_set(cac.TUNE_KS_COMPACT_BLOCK, key, best)
+            timings[str(key)] = {"n": n, "trials": trials, "built_in_ms": base_ms,
+                                 "ms": {str(b): v for b, v in res.items()}}
+            log(f"compaction class {key}: picked {best}")
+        d_adj.free()
+    return table, timings
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=str(cac.TUNING_PATH))
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--only", choices=("all", "sampling"), default="all",
-                    help="'sampling': keep the committed ks_contract_block table, tune only the sampling knob")
+    ap.add_argument("--only", choices=("all", "sampling", "compact"), default="all",
+                    help="tune only that knob and keep the committed tables of the others")
     args = ap.parse_args()
     ap_log = lambda s: print(s, flush=True)  # noqa: E731
-    if args.only == "sampling" and Path(args.out).exists():
-        old = json.loads(Path(args.out).read_text())
-        table = old["ks_contract_block"]
-        timings = old["timings_ms"].get("ks_contract_block", old["timings_ms"])
-    else:
+    old = json.loads(Path(args.out).read_text()) if args.only != "all" and Path(args.out).exists() else {}
+    old_t = old.get("timings_ms", {})
+    if args.only in ("all",) or "ks_contract_block" not in old:
         table, timings = tune(args.device, args.reps, log=ap_log)
-    stable, stimings = tune_sampling(args.device, args.reps, log=ap_log)
-    out = {"device": "gfx950 (MI355X)", "knobs": ["ks_contract_block", "sample_occupancy"],
-           "ks_contract_block": table, "sample_occupancy": stable,
-           "timings_ms": {"ks_contract_block": timings, "sample_occupancy": stimings}}
+    else:
+        table, timings = old["ks_contract_block"], old_t.get("ks_contract_block", {})
+    if args.only in ("all", "sampling") or "sample_occupancy" not in old:
+        stable, stimings = tune_sampling(args.device, args.reps, log=ap_log)
+    else:
+        stable, stimings = old["sample_occupancy"], old_t.get("sample_occupancy", {})
+    if args.only in ("all", "compact") or "ks_compact_block" not in old:
+        ctable, ctimings = tune_compact(args.device, args.reps, log=ap_log)
+    else:
+        ctable, ctimings = old["ks_compact_block"], old_t.get("ks_compact_block", {})
+    out = {"device": "gfx950 (MI355X)", "knobs": ["ks_contract_block", "sample_occupancy", "ks_compact_block"],
+           "ks_contract_block": table, "sample_occupancy": stable, "ks_compact_block": ctable,
+           "timings_ms": {"ks_contract_block": timings, "sample_occupancy": stimings, "ks_compact_block": ctimings}}
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     Path(args.out).write_text(json.dumps(out, indent=1) + "\n")
     print(f"wrote {args.out}: {table}", flush=True)

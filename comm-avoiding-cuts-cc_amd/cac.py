@@ -21,6 +21,8 @@ TUNE_KS_CONTRACT_KEYS = (512, 1024, 2048, 4096)
 TUNE_SAMPLE_OCCUPANCY = 2
 TUNE_SAMPLE_OCCUPANCY_KEYS = (16, 20, 24, 32)
 TUNE_WIDE_SAMPLING = 3  # key 0: log2 of the smallest edge count run in the wide form (0 built-in, 63 never)
+TUNE_KS_COMPACT_BLOCK = 4
+TUNE_KS_COMPACT_KEYS = (512, 1024, 2048, 4096, 8192)
 
 CAC_OK = 0
 CAC_EINVAL = -1
@@ -222,6 +224,8 @@ class Context:
             self.tune_set(TUNE_KS_CONTRACT_BLOCK, int(key), int(value))
         for key, value in table.get("sample_occupancy", {}).items():
             self.tune_set(TUNE_SAMPLE_OCCUPANCY, int(key), int(value))
+        for key, value in table.get("ks_compact_block", {}).items():
+            self.tune_set(TUNE_KS_COMPACT_BLOCK, int(key), int(value))
         if "wide_sampling_log2" in table:
             self.tune_set(TUNE_WIDE_SAMPLING, 0, int(table["wide_sampling_log2"]))
         self.tuning_file = str(path)

@@ -274,7 +274,9 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
 // class upper bounds), a value 0 (built-in rule) or a power of two from key / 8
 // (at least 64) to 1024; CAC_TUNE_SAMPLE_OCCUPANCY keys 16, 20, 24, 32 (log2 of
 // the edge-count class bound), a value 0 (built-in: 2), 1 or 2;
-// CAC_TUNE_WIDE_SAMPLING key 0, a value 0 (built-in: 2^20 edges) to 63 (never).
+// CAC_TUNE_WIDE_SAMPLING key 0, a value 0 (built-in: 2^20 edges) to 63 (never);
+// CAC_TUNE_KS_COMPACT_BLOCK keys 512, 1024, 2048, 4096, 8192 (child-size class
+// upper bounds), a value 0 (built-in: 512), 256, 512 or 1024.
 static int tune_slot(int knob, uint32_t key) {
   if (knob == CAC_TUNE_KS_CONTRACT_BLOCK) {
     switch (key) {
@@ -295,6 +297,16 @@ static int tune_slot(int knob, uint32_t key) {
     }
   }
   if (knob == CAC_TUNE_WIDE_SAMPLING) return key == 0 ? 0 : -1;
+  if (knob == CAC_TUNE_KS_COMPACT_BLOCK) {
+    switch (key) {
+      case 512: return 0;
+      case 1024: return 1;
+      case 2048: return 2;
+      case 4096: return 3;
+      case 8192: return 4;
+      default: return -1;
+    }
+  }
   return -1;
 }
 
@@ -312,6 +324,12 @@ int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value) {
     ctx->wide_log2 = value;
     return CAC_OK;
   }
+  if (knob == CAC_TUNE_KS_COMPACT_BLOCK) {
+    if (value != 0 && value != 256 && value != 512 && value != 1024)
+      return cac::fail(ctx, CAC_EINVAL, "compaction block size must be 0, 256, 512 or 1024");
+    ctx->ks_compact_block[slot] = value;
+    return CAC_OK;
+  }
   const int32_t lo = key / 8 < 64 ? 64 : (int32_t)(key / 8);
   if (value != 0 && (value < lo || value > 1024 || (value & (value - 1))))
     return cac::fail(ctx, CAC_EINVAL, "block size must be 0 or a power of two from key/8 (at least 64) to 1024");
@@ -325,7 +343,8 @@ int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value) {
   if (slot < 0) return cac::fail(ctx, CAC_EINVAL, "unknown tuning knob or key");
   *value = knob == CAC_TUNE_SAMPLE_OCCUPANCY ? ctx->sample_occupancy[slot]
            : knob == CAC_TUNE_WIDE_SAMPLING  ? ctx->wide_log2
-                                             : ctx->ks_contract_block[slot];
+           : knob == CAC_TUNE_KS_COMPACT_BLOCK ? ctx->ks_compact_block[slot]
+                                               : ctx->ks_contract_block[slot];
   return CAC_OK;
 }
 

@@ -65,6 +65,9 @@ struct cac_ctx {
   // Sampling workgroups per CU for launches of more trials than CUs, by edge
   // count class m < 2^16, 2^20, 2^24, larger (cac_tune_set); 0 = built-in (2).
   int32_t sample_occupancy[4] = {0, 0, 0, 0};
+  // threads per k_ks_compact workgroup by child-size class (<= 512, 1024, 2048,
+  // 4096, larger); 0 = built-in (512)
+  int32_t ks_compact_block[5] = {0, 0, 0, 0, 0};
   // log2 of the smallest edge count whose sqrt-cut trials run in the wide
   // (grid-per-trial) form: 0 = built-in (2^20), 63 = never
   int32_t wide_log2 = 0;

@@ -563,6 +563,16 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
 // accumulator holds rows x t' int64, at most 32 KiB unless t' alone is larger).
 constexpr uint32_t kCompactUnroll = 8;  // row elements per thread in flight in k_ks_compact
 
+// Threads per compaction workgroup for children of tgt output vertices: the
+// context's CAC_TUNE_KS_COMPACT_BLOCK setting for tgt's class, or 512 (on
+// MI355X 17% less compaction time than 256 on the n = 8192 complete graph,
+// and 1024 was slower than either).
+inline int compact_block(const cac_ctx* ctx, uint32_t tgt) {
+  const int cls = tgt <= 512 ? 0 : tgt <= 1024 ? 1 : tgt <= 2048 ? 2 : tgt <= 4096 ? 3 : 4;
+  const int v = ctx->ks_compact_block[cls];
+  return v ? v : 512;
+}
+
 __host__ __device__ inline uint32_t compact_rows(uint32_t tgt) {
   const uint32_t r = 4096 / (tgt ? tgt : 1);
   return r < 1 ? 1 : (r > 64 ? 64 : r);
@@ -570,11 +580,14 @@ __host__ __device__ inline uint32_t compact_rows(uint32_t tgt) {
 
 // One workgroup per (child, group of output rows): sums every live row by column
 // class into LDS, then writes the rows with their sums and maxima.  The work
-// item's child comes from a binary search of the per-child item prefix.
-__global__ void __launch_bounds__(256) k_ks_compact(const KsChild* __restrict__ children,
-                                                    const uint32_t* __restrict__ k_out,
-                                                    const uint32_t* __restrict__ item_prefix, uint32_t nchild,
-                                                    KsPool pool) {
+// item's child comes from a binary search of the per-child item prefix.  CB
+// threads per workgroup: large children, whose rows go one per workgroup, take
+// more threads, so more of a row's loads are in flight per CU.
+template <int CB>
+__global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ children,
+                                                   const uint32_t* __restrict__ k_out,
+                                                   const uint32_t* __restrict__ item_prefix, uint32_t nchild,
+                                                   KsPool pool) {
   extern __shared__ __align__(16) unsigned long long acc[];
   const uint32_t item = blockIdx.x;
   uint32_t lo = 0, hi = nchild;  // last child with item_prefix[child] <= item
@@ -591,7 +604,7 @@ __global__ void __launch_bounds__(256) k_ks_compact(const KsChild* __restrict__ 
   if (r0 >= kk) return;
   const uint32_t nr = min(R, kk - r0);
   const uint32_t s = c.s;
-  for (uint32_t x = threadIdx.x; x < nr * kk; x += 256) acc[x] = 0ull;
+  for (uint32_t x = threadIdx.x; x < nr * kk; x += CB) acc[x] = 0ull;
   __syncthreads();
   // Rows and the column map are global memory (the struct's pointers are
   // generic, which would give flat loads); kCompactUnroll elements per thread
@@ -606,26 +619,26 @@ __global__ void __launch_bounds__(256) k_ks_compact(const KsChild* __restrict__ 
     g_i64* src = (g_i64*)((e & kLiveWork) ? pool.base + (size_t)(e & ~kLiveWork) * kKsUnit : c.parent + (size_t)e * s);
     unsigned long long* a = acc + (size_t)rr * kk;
     uint32_t y = threadIdx.x;
-    for (; y + (U - 1) * 256 < s; y += U * 256) {
+    for (; y + (U - 1) * CB < s; y += U * CB) {
       int64_t v[U];
       uint32_t m[U];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
-        v[u] = src[y + u * 256];
-        m[u] = cm[y + u * 256];
+        v[u] = src[y + u * CB];
+        m[u] = cm[y + u * CB];
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u)
         if (v[u]) atomicAdd(&a[m[u]], (unsigned long long)v[u]);
     }
-    for (; y < s; y += 256) {
+    for (; y < s; y += CB) {
       const int64_t v = src[y];
       if (v) atomicAdd(&a[cm[y]], (unsigned long long)v);
     }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t rr = wave; rr < nr; rr += 4) {
+  for (uint32_t rr = wave; rr < nr; rr += CB / 64) {
     const uint32_t r = r0 + rr;
     int64_t* dst = c.out + (size_t)r * kk;
     int64_t sm = 0, mx = 0;
@@ -806,8 +819,16 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
   CAC_LAUNCH_CHECK(ctx, "ks contract");
   prof_begin(ctx, "ks_compact", s);
   const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
-  hipLaunchKernelGGL(k_ks_compact, dim3(items), dim3(256), lds, s, (const KsChild*)d_children, (const uint32_t*)d_kout,
-                     (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+  const int cb = compact_block(ctx, max_tgt);
+  if (cb == 1024)
+    hipLaunchKernelGGL(k_ks_compact<1024>, dim3(items), dim3(1024), lds, s, (const KsChild*)d_children,
+                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+  else if (cb == 512)
+    hipLaunchKernelGGL(k_ks_compact<512>, dim3(items), dim3(512), lds, s, (const KsChild*)d_children,
+                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+  else
+    hipLaunchKernelGGL(k_ks_compact<256>, dim3(items), dim3(256), lds, s, (const KsChild*)d_children,
+                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
   prof_end(ctx, s);
   CAC_LAUNCH_CHECK(ctx, "ks compact");
   CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));

@@ -97,6 +97,12 @@ int cac_memcpy_d2d(cac_ctx* ctx, void* dst, const void* src, size_t bytes, void*
  * v = 0 restores the built-in 20 (2^20 edges), v = 63 never uses the wide form,
  * 1 <= v <= 62 sets 2^v; CAC_EINVAL else.  Cuts do not depend on it. */
 #define CAC_TUNE_WIDE_SAMPLING 3
+/* CAC_TUNE_KS_COMPACT_BLOCK: threads per workgroup of the dense Karger-Stein
+ * compaction for children of t' vertices, keyed by class: key 512 holds
+ * t' <= 512, keys 1024, 2048, 4096 and 8192 hold t' in (key/2, key].  value 0
+ * restores the built-in 512; otherwise 256, 512 or 1024, CAC_EINVAL else.
+ * Results never depend on it. */
+#define CAC_TUNE_KS_COMPACT_BLOCK 4
 int cac_tune_set(cac_ctx* ctx, int knob, uint32_t key, int32_t value);
 int cac_tune_get(cac_ctx* ctx, int knob, uint32_t key, int32_t* value);
 

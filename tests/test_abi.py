@@ -49,6 +49,9 @@ def test_tuning_file_shape():
         assert int(key) in cac.TUNE_SAMPLE_OCCUPANCY_KEYS and k in (0, 1, 2)
     wide = json.loads(cac.TUNING_PATH.read_text()).get("wide_sampling_log2", 0)
     assert 0 <= int(wide) <= 63
+    comp = json.loads(cac.TUNING_PATH.read_text()).get("ks_compact_block", {})
+    for key, b in comp.items():
+        assert int(key) in cac.TUNE_KS_COMPACT_KEYS and b in (0, 256, 512, 1024)
 
 
 def test_record_layouts():

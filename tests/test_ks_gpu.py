@@ -128,3 +128,32 @@ def test_ks_tune_rejects_bad_values():
         c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, 512, 256)
         c.tune_set(cac.TUNE_KS_CONTRACT_BLOCK, 512, 0)
         assert c.tune_get(cac.TUNE_KS_CONTRACT_BLOCK, 512) == 0
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_ks_compaction_block_sizes_bit_exact(orc, block):
+    # the compaction block size of every child-size class (cac_tune_set)
+    # changes the launch shape only: the oracle's per-trial cuts at n = 1500
+    n = 1500
+    rng = np.random.default_rng(78)
+    e = np.array([[i, (i + 1) % n, int(rng.integers(10**6, 2 * 10**6))] for i in range(n)], dtype=np.int64)
+    A = dense(n, e)
+    T = 6
+    _say(f"ks compaction block {block} n {n}")
+    with cac.Context(0, tuning=False) as c:
+        for key in cac.TUNE_KS_COMPACT_KEYS:
+            c.tune_set(cac.TUNE_KS_COMPACT_BLOCK, key, block)
+        assert all(c.tune_get(cac.TUNE_KS_COMPACT_BLOCK, k) == block for k in cac.TUNE_KS_COMPACT_KEYS)
+        raw, _ = c.ks_dense(A, 12, 0, T)
+    assert raw.tolist() == [orc.ks_trial(A, 12, t) for t in range(T)]
+
+
+def test_ks_compaction_tune_rejects_bad_values():
+    with cac.Context(0, tuning=False) as c:
+        for key, b in [(512, 128), (1024, 2048), (3000, 512), (16384, 512), (8192, 300)]:
+            with pytest.raises(cac.CacError):
+                c.tune_set(cac.TUNE_KS_COMPACT_BLOCK, key, b)
+        c.tune_set(cac.TUNE_KS_COMPACT_BLOCK, 8192, 1024)
+        assert c.tune_get(cac.TUNE_KS_COMPACT_BLOCK, 8192) == 1024
+        c.tune_set(cac.TUNE_KS_COMPACT_BLOCK, 8192, 0)
+        assert c.tune_get(cac.TUNE_KS_COMPACT_BLOCK, 8192) == 0
