@@ -130,6 +130,95 @@ __device__ uint32_t probe(const uint2* draws, uint32_t L, uint32_t undo, uint32_
   return (uint32_t)bsum64(mine, lds);
 }
 
+// Root of x without path splitting: no entry but a hooked root's changes.
+__device__ inline uint32_t uf_root_fixed(const uint32_t* par, uint32_t x) {
+  uint32_t p = par[x];
+  while (p != x) {
+    x = p;
+    p = par[x];
+  }
+  return x;
+}
+
+// Union without path splitting; on a hook, *hooked is the root that got a
+// parent, so resetting it to itself undoes the hook exactly.
+__device__ inline bool uf_union_fixed(uint32_t* par, uint32_t a, uint32_t b, uint32_t* hooked) {
+  uint32_t ra = uf_root_fixed(par, a), rb = uf_root_fixed(par, b);
+  while (ra != rb) {
+    const uint32_t lo = min(ra, rb), hi = max(ra, rb);
+    const uint32_t old = atomicCAS(par + hi, hi, lo);
+    if (old == hi) {
+      *hooked = hi;
+      return true;
+    }
+    ra = uf_root_fixed(par, old);
+    rb = uf_root_fixed(par, lo);
+  }
+  return false;
+}
+
+// Shortest prefix of the D draws whose unions make `need` hooks (all D make at
+// least that many); leaves par holding exactly that prefix's forest and
+// returns its hooks (== need).  From the round's starting forest the draws are
+// united a chunk of kSmpBlock at a time (one per thread), with no path
+// splitting, and a chunk that stays short of `need` is kept.  The chunk that
+// reaches it is undone (every thread resets the root its hook changed) and
+// the prefix length inside it is binary-searched the same way.  A prefix's
+// hook count does not depend on the order of its unions, and each draw adds at
+// most one hook, so this is the length the reference's draw-by-draw
+// contraction stops at.
+__device__ uint32_t prefix_search(const uint2* draws, uint32_t D, uint32_t need, uint32_t* par, uint64_t* lds) {
+  for (uint32_t d = threadIdx.x; d < D; d += kSmpBlock) {  // back to the round's starting forest
+    const uint2 x = draws[d];
+    par[x.x] = x.x;
+    par[x.y] = x.y;
+  }
+  block_sync_global();
+  uint32_t total = 0;
+  for (uint32_t c0 = 0; c0 < D; c0 += kSmpBlock) {
+    const uint32_t len = min((uint32_t)kSmpBlock, D - c0);
+    uint32_t mine = 0;
+    bool hooked = false;
+    if (threadIdx.x < len) {
+      const uint2 x = draws[c0 + threadIdx.x];
+      hooked = uf_union_fixed(par, x.x, x.y, &mine);
+    }
+    block_sync_global();
+    const uint32_t ch = (uint32_t)bsum64(hooked ? 1 : 0, lds);
+    if (total + ch < need) {  // the whole chunk stays
+      total += ch;
+      continue;
+    }
+    if (hooked) par[mine] = mine;  // undo the chunk
+    block_sync_global();
+    uint32_t lo = 1, hi = len;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      uint32_t m2 = 0;
+      bool h2 = false;
+      if (threadIdx.x < mid) {
+        const uint2 x = draws[c0 + threadIdx.x];
+        h2 = uf_union_fixed(par, x.x, x.y, &m2);
+      }
+      block_sync_global();
+      const uint32_t c2 = (uint32_t)bsum64(h2 ? 1 : 0, lds);
+      if (h2) par[m2] = m2;
+      block_sync_global();
+      if (total + c2 >= need) hi = mid;
+      else lo = mid + 1;
+    }
+    uint32_t m3 = 0;
+    bool h3 = false;
+    if (threadIdx.x < lo) {
+      const uint2 x = draws[c0 + threadIdx.x];
+      h3 = uf_union_fixed(par, x.x, x.y, &m3);
+    }
+    block_sync_global();
+    return total + (uint32_t)bsum64(h3 ? 1 : 0, lds);
+  }
+  return total;
+}
+
 // kMinWaves = 8 holds the kernel to 64 VGPRs (8 waves per SIMD), so two 16-wave
 // workgroups share a CU, at the price of a few spilled registers: for launches
 // of more workgroups than CUs.  kMinWaves = 1 leaves the registers free, for
@@ -265,17 +354,8 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
     block_sync_global();
     // 3. shortest draw prefix that leaves t super-vertices
     uint32_t hooks = probe(S.draws, D, 0, S.par, lds);
-    if (nv - hooks <= t) {
-      uint32_t lo = 1, hi = D, last = D;
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        hooks = probe(S.draws, mid, last, S.par, lds);
-        last = mid;
-        if (nv - hooks <= t) hi = mid;
-        else lo = mid + 1;
-      }
-      if (last != lo) hooks = probe(S.draws, lo, last, S.par, lds);
-    }  // else all D draws are kept, and the forest of the first probe stays
+    if (nv - hooks <= t) hooks = prefix_search(S.draws, D, nv - t, S.par, lds);
+    // else all D draws are kept, and the forest of the first probe stays
     // 4. canonical relabel
     for (uint32_t v = threadIdx.x; v < n; v += kSmpBlock) lab_set(v, uf_root(S.par, lab_get(v)));
     block_sync_global();
