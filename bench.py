@@ -593,7 +593,7 @@ def bench_c5(cac, count: int, rate: float, cpu_budget_s: float = 0.0):
     # per launch group on one worker
     mci = [i for i in range(count) if kinds[i] == cac.QUERY_MINCUT]
     mc = [recs[i] for i in mci]
-    rates, rates_up = {}, {}
+    rates, rates_up, all_rates, all_rates_up = {}, {}, {}, {}
     for name, w, mb in (("grouped", 4, 64), ("one_graph_per_launch", 1, 1)):
         with cac.Server(0, workers=w, hbm_budget=budget, max_batch=mb) as srv:
             # device-resident graphs (registered before the clock starts), as the
@@ -602,15 +602,24 @@ def bench_c5(cac, count: int, rate: float, cpu_budget_s: float = 0.0):
             resident = [cac.Server.query(cac.QUERY_MINCUT, None, 0, F32_95, seed=i, trials=budget_T[i], graph=h)
                         for i, h in zip(mci, hs)]
             srv.replay(resident, np.zeros(len(resident)))  # warm-up: buffers reach their size
-            t0 = time.perf_counter()
-            r = srv.replay(resident, np.zeros(len(resident)))
-            rates[name] = len(mc) / (time.perf_counter() - t0)
-            assert np.all(r["status"] == 0)
+            # groups form by timing, so a replay can need larger buffers than
+            # the last one grew: three timed replays, the median reported
+            reps, reps_up = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r = srv.replay(resident, np.zeros(len(resident)))
+                reps.append(len(mc) / (time.perf_counter() - t0))
+                assert np.all(r["status"] == 0)
+            rates[name] = float(np.median(reps))
+            all_rates[name] = reps
             # the same queries bringing their edges (host upload inside the clock)
-            t0 = time.perf_counter()
-            ru = srv.replay(mc, np.zeros(len(mc)))
-            rates_up[name] = len(mc) / (time.perf_counter() - t0)
-            assert np.all(ru["status"] == 0) and np.array_equal(r["cut"], ru["cut"])
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ru = srv.replay(mc, np.zeros(len(mc)))
+                reps_up.append(len(mc) / (time.perf_counter() - t0))
+                assert np.all(ru["status"] == 0) and np.array_equal(r["cut"], ru["cut"])
+            rates_up[name] = float(np.median(reps_up))
+            all_rates_up[name] = reps_up
             _note(f"c5: {len(mc)} min-cut queries {name}: {rates[name]:.1f} queries/s resident, "
                   f"{rates_up[name]:.1f} with upload")
             if name == "grouped":
@@ -619,6 +628,7 @@ def bench_c5(cac, count: int, rate: float, cpu_budget_s: float = 0.0):
                 out["mincut_grouped_equals_single"] = bool(np.array_equal(ref, r["cut"]))
     out["mincut_closed_loop_queries"] = len(mc)
     out["mincut_closed_loop_qps"] = rates
+    out["mincut_closed_loop_qps_each_replay"] = {"resident": all_rates, "with_upload": all_rates_up}
     out["mincut_speedup"] = rates["grouped"] / rates["one_graph_per_launch"]
     out["mincut_closed_loop_qps_with_upload"] = rates_up
     out["mincut_speedup_with_upload"] = rates_up["grouped"] / rates_up["one_graph_per_launch"]
