@@ -335,7 +335,7 @@ struct KsLds {
   uint32_t* rep;
   uint32_t* wslot;
   uint64_t* red;
-  uint32_t* misc;  // [0] i  [1] k  [2] slot of the other row  [3] destination  [4] overflow
+  uint32_t* misc;  // [0] i  [1] k  [2] rep[k]  [5] spare pool row
 };
 
 __device__ inline const int64_t* ks_row(const KsChild& c, const KsPool& pool, uint32_t slot, uint32_t r) {
@@ -360,47 +360,26 @@ __device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int
 
 // Merge row a into row b (a dies, b keeps its index).  The calling threads hold
 // their chunk of row `held` (a or b) in v; the other row is loaded here.  The
-// merged row goes to b's pool row; if b has none it takes a's (a dies, and every
-// thread rewrites exactly the entries it read), and only when neither has one a
-// new pool row.  Returns the contracted weight (cap[a] + cap[b] - cap_new) / 2
-// in w; false when the pool is exhausted.  Called by the whole block.
+// merged row goes to b's pool row, else a's (a dies, and every thread rewrites
+// exactly the entries it read), else the child's spare pool row (misc[5]),
+// which thread 0 then replaces with a fresh one.  Every thread derives that
+// destination from the same LDS words; thread 0 changes them only after the
+// block sum's barrier, which every thread reaches after its reads, and the
+// merge ends with one more barrier.  Each thread relabels only the
+// representatives of the columns it owns.  Returns the contracted weight
+// (cap[a] + cap[b] - cap_new) / 2 in w; false when the pool is exhausted.
+// Called by the whole block.
 template <int B>
 __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, uint32_t a, uint32_t b, uint32_t held,
                          const int64_t (&v)[kKsChunk], uint32_t cc, int64_t& w) {
   const uint32_t s = c.s;
   const uint32_t other = held == a ? b : a;
-  // The other row's address is fixed before the slots change, so its load is
-  // issued first and lands while the slots and representatives are updated
-  // (the barriers below wait for LDS operations only).
+  const uint32_t slot_a = L.wslot[a], slot_b = L.wslot[b];
   int64_t u[kKsChunk];
-  load_row<B>(ks_row(c, pool, L.wslot[other], other), cc, s, u);
-  if (B > 64) __syncthreads();  // every wave has read the slot before thread 0 changes it
-  if (threadIdx.x == 0) {
-    uint32_t db = L.wslot[b];
-    uint32_t overflow = 0;
-    if (db == kNoSlot && L.wslot[a] != kNoSlot) {
-      db = L.wslot[a];
-      L.wslot[b] = db;
-      L.wslot[a] = kNoSlot;
-    } else if (db == kNoSlot) {
-      const unsigned long long units = (s + kKsUnit - 1) / kKsUnit;
-      const unsigned long long off = atomicAdd(pool.used_units, units);
-      if (off + units > pool.cap_units) {
-        overflow = 1;
-      } else {
-        db = (uint32_t)off;
-        L.wslot[b] = db;
-      }
-    }
-    L.misc[3] = db;
-    L.misc[4] = overflow;
-  }
-  __syncthreads();  // every thread has read rep[k] and the slots before they change
-  if (L.misc[4]) return false;
-  const uint32_t db = L.misc[3];
-  for (uint32_t x = threadIdx.x; x < s; x += B)
-    if (L.rep[x] == a) L.rep[x] = b;
-  __syncthreads();
+  load_row<B>(ks_row(c, pool, other == a ? slot_a : slot_b, other), cc, s, u);
+  const bool spare = slot_b == kNoSlot && slot_a == kNoSlot;
+  const uint32_t db = slot_b != kNoSlot ? slot_b : slot_a != kNoSlot ? slot_a : L.misc[5];
+  if (db == kNoSlot) return false;  // only a spare can be missing: the pool is exhausted
   int64_t* dst = pool.base + (size_t)db * kKsUnit;
   uint64_t sum = 0;
 #pragma unroll
@@ -408,16 +387,27 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
     const uint32_t x = k * B + threadIdx.x;
     if (k < cc && x < s) {
       int64_t t = v[k] + u[k];
-      if (L.rep[x] == b) t = 0;  // loop of the merged vertex
-      dst[x] = t;                // each thread reads back only its own chunk later
+      const uint32_t rx = L.rep[x];
+      if (rx == a || rx == b) t = 0;  // loop of the merged vertex
+      if (rx == a) L.rep[x] = b;
+      dst[x] = t;  // each thread reads back only its own chunk later
       sum += (uint64_t)t;
     }
   }
   const int64_t ca = L.cap[a], cb = L.cap[b];
-  const int64_t newcap = (int64_t)block_sum_u64<B>(sum, L.red);
+  const int64_t newcap = (int64_t)block_sum_u64_open<B>(sum, L.red);
   w = (ca + cb - newcap) / 2;
-  if (B == 64) __syncthreads();  // every lane has read cap[a], cap[b]
+  if (B == 64) __syncthreads();  // every lane has read the slots and capacities
   if (threadIdx.x == 0) {
+    if (slot_b == kNoSlot) {
+      L.wslot[b] = db;
+      L.wslot[a] = kNoSlot;
+    }
+    if (spare) {
+      const unsigned long long units = (s + kKsUnit - 1) / kKsUnit;
+      const unsigned long long off = atomicAdd(pool.used_units, units);
+      L.misc[5] = off + units <= pool.cap_units ? (uint32_t)off : kNoSlot;
+    }
     L.cbs[a / 64] -= (uint64_t)ca;
     L.cap[a] = 0;
     L.cbs[b / 64] += (uint64_t)(newcap - cb);
@@ -449,6 +439,11 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
     L.cap[x] = c.prsum[x];
     L.rep[x] = x;
     L.wslot[x] = kNoSlot;
+  }
+  if (threadIdx.x == 0) {  // the spare pool row of the first merge of two original rows
+    const unsigned long long units = (s + kKsUnit - 1) / kKsUnit;
+    const unsigned long long off = atomicAdd(pool.used_units, units);
+    L.misc[5] = off + units <= pool.cap_units ? (uint32_t)off : kNoSlot;
   }
   __syncthreads();
   for (uint32_t blk = wave; blk < nb; blk += B / 64) {
@@ -482,12 +477,13 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
       break;
     }
     load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
-    const uint32_t k = block_select_row<B>(v, cc, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1);
+    const uint32_t k =
+        block_select_row<B>(v, cc, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1, L.rep);
     if (k >= s) {
       status = 2;
       break;
     }
-    const uint32_t j = L.rep[k];
+    const uint32_t j = L.misc[2];  // rep[k], read by the selecting thread before the merge relabels
     const uint32_t a = i > j ? i : j, b = i > j ? j : i;
     int64_t w = 0;
     if (!ks_merge<B>(c, pool, L, a, b, i, v, cc, w)) {
@@ -509,6 +505,8 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
         if (first == kNoSlot && k < cc && k * B + threadIdx.x < s && v[k] >= c.bound) first = k * B + threadIdx.x;
       first = block_min_u32<B>(first, (uint32_t*)L.red);
       if (first == kNoSlot) continue;
+      // rep[first] is not i: row i's entries of its own class are 0, below the
+      // bound, so this merge's relabelling (of class i) leaves it unchanged
       int64_t w = 0;
       if (!ks_merge<B>(c, pool, L, i, L.rep[first], i, v, cc, w)) {
         status = 3;

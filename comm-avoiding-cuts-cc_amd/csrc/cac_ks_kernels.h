@@ -102,6 +102,19 @@ __device__ inline uint64_t block_sum_u64(uint64_t x, uint64_t* lds) {
   __syncthreads();
   return t;
 }
+// Block sum without the trailing barrier: the caller reaches a barrier before
+// lds is written again.
+template <int B>
+__device__ inline uint64_t block_sum_u64_open(uint64_t x, uint64_t* lds) {
+  x = wave_sum_u64(x);
+  if (B == 64) return x;
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint64_t t = 0;
+#pragma unroll
+  for (int w = 0; w < B / 64; ++w) t += lds[w];
+  return t;
+}
 template <int B>
 __device__ inline uint32_t block_min_u32(uint32_t x, uint32_t* lds) {
   x = wave_min_u32(x);
@@ -146,15 +159,18 @@ __device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint6
 // First index (natural column order) of a row whose inclusive prefix reaches r
 // (1 <= r), with the row held in registers interleaved: thread t owns entries
 // x = k * B + t, k < cc, and holds 0 for x >= s.  Natural order is segment k
-// by segment k, so: the eight segment totals (wave sums, one barrier), the
-// segment where the prefix reaches r, then one block scan inside it.  lds:
-// (kKsChunk + 1) * B/64 u64.  kNoSlot if the row total is below r.
+// by segment k, so every wave's segment totals go to LDS (one barrier); then
+// each thread finds the segment where the prefix reaches r and its wave's
+// offset inside it, and the one thread whose entry holds r publishes its index
+// (a second barrier), with rep[index] in lds_idx[1]: it reads that entry before
+// the barrier, so a merge that relabels representatives right after cannot race
+// with it.  lds: kKsChunk * B/64 u64.  kNoSlot if the row total is below r.
+// The caller reaches a barrier before lds or lds_idx is written again.
 template <int B>
 __device__ inline uint32_t block_select_row(const int64_t (&v)[kKsChunk], uint32_t cc, uint64_t r, uint64_t* lds,
-                                            uint32_t* lds_idx) {
+                                            uint32_t* lds_idx, const uint32_t* rep) {
   constexpr int W = B / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) *lds_idx = kNoSlot;
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
     if (k < cc) {
@@ -180,17 +196,16 @@ __device__ inline uint32_t block_select_row(const int64_t (&v)[kKsChunk], uint32
     }
   }
   if (ks == kNoSlot) return kNoSlot;  // the same for every thread
-  const uint64_t inc = wave_incl_scan_u64(val, lane);
-  if (lane == 63) lds[kKsChunk * W + wave] = inc;
-  __syncthreads();
   uint64_t wb = 0;
-  for (int w = 0; w < wave; ++w) wb += lds[kKsChunk * W + w];
+  for (int w = 0; w < wave; ++w) wb += lds[ks * W + w];
+  const uint64_t inc = wave_incl_scan_u64(val, lane);
   const uint64_t ex = before + wb + inc - val;
-  if (val != 0 && ex < r && r <= ex + val) *lds_idx = ks * B + threadIdx.x;
+  if (val != 0 && ex < r && r <= ex + val) {  // exactly one thread
+    lds_idx[0] = ks * B + threadIdx.x;
+    lds_idx[1] = rep[ks * B + threadIdx.x];
+  }
   __syncthreads();
-  const uint32_t res = *lds_idx;
-  __syncthreads();
-  return res;
+  return *lds_idx;
 }
 
 }  // namespace cac
