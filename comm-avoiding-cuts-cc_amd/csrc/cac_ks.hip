@@ -46,15 +46,6 @@ __device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
   }
 }
 
-// One DPP move of a 64-bit value as two 32-bit halves; lanes whose source row
-// is masked off or out of range read 0, the identity of an unsigned maximum.
-template <int Ctrl, int RowMask>
-__device__ inline uint64_t dpp_u64(uint64_t v) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, Ctrl, RowMask, 0xf, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf, false);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Unsigned maximum over the wave, returned to every lane: inside each row of 16
 // lanes by two quad permutations and two row rotations, then across rows by
 // the row broadcasts of lane 15 (into rows 1 and 3) and lane 31 (into rows 2
@@ -77,11 +68,7 @@ __device__ inline uint64_t wave_max_u64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ inline int64_t wave_sum_i64(int64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
+__device__ inline int64_t wave_sum_i64(int64_t x) { return (int64_t)wave_sum_u64((uint64_t)x); }
 
 // d - (q-1) M, saturating to a large negative value instead of overflowing
 // (a negative term only weakens the bound).

@@ -70,23 +70,55 @@ struct KsLeaf {
 };
 
 // ---- wave / block reductions -------------------------------------------------
-__device__ inline uint64_t wave_sum_u64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+// Called by whole wavefronts.  Cross-lane steps are DPP row moves (VALU data
+// moves, a few cycles each) rather than LDS permutes: the contraction runs
+// about a dozen of these per pick, one after another.
+//
+// One DPP move of a 64-bit value as two 32-bit halves; lanes whose source lane
+// lies outside its row, or whose own row is masked off, get `old`.
+template <int Ctrl, int RowMask>
+__device__ inline uint64_t dpp_u64(uint64_t v, uint64_t old = 0) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)old, (int)(uint32_t)v, Ctrl, RowMask,
+                                                            0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(old >> 32), (int)(uint32_t)(v >> 32),
+                                                            Ctrl, RowMask, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+template <int Ctrl, int RowMask>
+__device__ inline uint32_t dpp_u32(uint32_t v, uint32_t old) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, Ctrl, RowMask, 0xf, false);
+}
+__device__ inline uint64_t readlane_u64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+// Inclusive prefix sum over the wave: Hillis-Steele inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8; a lane with no source adds 0), then the row broadcasts
+// of lane 15 (into rows 1 and 3) and lane 31 (into rows 2 and 3).
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t x, int /*lane*/) {
+  x += dpp_u64<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_u64<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_u64<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_u64<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_u64<0x142, 0xa>(x);  // row_bcast:15
+  x += dpp_u64<0x143, 0xc>(x);  // row_bcast:31
   return x;
 }
+// Wave total, returned to every lane (the scan's last lane).
+__device__ inline uint64_t wave_sum_u64(uint64_t x) { return readlane_u64(wave_incl_scan_u64(x, 0), 63); }
+// Wave minimum, returned to every lane: inside each row by two quad
+// permutations and two row rotations, then across rows by the broadcasts of
+// lanes 15 and 31, which leave it in lane 63; lanes that receive nothing keep
+// the identity 0xFFFFFFFF.
 __device__ inline uint32_t wave_min_u32(uint32_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, 64));
-  return x;
-}
-__device__ inline uint64_t wave_incl_scan_u64(uint64_t x, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint64_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
+  x = min(x, dpp_u32<0xb1, 0xf>(x, ~0u));   // quad_perm [1,0,3,2]
+  x = min(x, dpp_u32<0x4e, 0xf>(x, ~0u));   // quad_perm [2,3,0,1]
+  x = min(x, dpp_u32<0x124, 0xf>(x, ~0u));  // row_ror:4
+  x = min(x, dpp_u32<0x128, 0xf>(x, ~0u));  // row_ror:8
+  x = min(x, dpp_u32<0x142, 0xa>(x, ~0u));  // row_bcast:15
+  x = min(x, dpp_u32<0x143, 0xc>(x, ~0u));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
 // Block sum over B threads; every thread gets the total.  lds: >= B/64 u64.
@@ -140,8 +172,8 @@ __device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint6
   const uint64_t hit = __ballot(incl >= r);
   if (!hit) return kNoSlot;
   const int owner = __ffsll((long long)hit) - 1;
-  const uint64_t before = __shfl(incl - c0 - c1, owner, 64);
-  const uint64_t oc0 = __shfl(c0, owner, 64);
+  const uint64_t before = readlane_u64(incl - c0 - c1, owner);
+  const uint64_t oc0 = readlane_u64(c0, owner);
   uint32_t blk = 2 * owner;
   uint64_t rr = r - before;
   if (rr > oc0) {
