@@ -12,7 +12,12 @@
 //   - min cut: every (query, trial) job of the group shares the sampling,
 //     densify and Karger-Stein launches (sqrt_cut_batch); each job carries its
 //     query's seed, so queries of any seeds share launches.
-// Groups on different workers overlap on the device.  A query is admitted only
+// Groups on different workers overlap on the device, so a min-cut group takes
+// at most an equal share (1 / workers) of the pending min-cut work (trials x
+// edges), the oldest query always: under a backlog every worker gets a share,
+// and one group's latency-bound stretches (its Karger-Stein pick chains) run
+// beside the others' sampling instead of one after another in a single group.
+// A query is admitted only
 // when its footprint estimate fits beside the groups in flight, so requests
 // that arrive while others run are admitted as memory and workers free up.  A
 // min-cut group runs its trials in chunks, so it is charged its uploads plus
@@ -54,6 +59,7 @@ struct Pending {
   size_t bytes;             // upload + scratch
   size_t upload;            // its edges, when it brings them
   size_t scratch;           // device scratch of its work (for a min cut: its trials' chunk)
+  uint64_t work;            // a min cut's trials x edges (its share of a group), 0 for CC
   const cac_edge* d_edges;  // a registered graph's device edges (checked once), or null
 };
 
@@ -121,6 +127,7 @@ struct cac_server {
   std::deque<cac::Pending> pending;
   std::vector<cac_query_result> done;
   uint64_t inflight = 0, next_id = 0;
+  uint64_t pending_work = 0;  // sum of the pending min-cut queries' work
   uint64_t peak_inflight = 0, groups = 0;
   std::unordered_map<uint64_t, cac::Resident> graphs;  // registered graphs by handle
   uint64_t next_graph = 1, resident = 0;               // resident: bytes of registered graphs
@@ -374,6 +381,10 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
       }
       Pending head = S->pending.front();
       S->pending.pop_front();
+      // a min-cut group's share of the pending work (the oldest query always)
+      const uint64_t share = S->pending_work / std::max<size_t>(S->ctxs.size(), 1);
+      uint64_t work = head.work;
+      S->pending_work -= head.work;
       // a min-cut group's scratch is capped at an equal share of the budget
       // per worker: its trials then run in chunks of that size
       const size_t cap = S->budget / std::max<size_t>(S->ctxs.size(), 1);
@@ -392,7 +403,10 @@ void worker_loop(cac_server* S, cac_ctx* ctx) {
           for (auto& g : grp) nsum += g.q.n;
           return nsum + it->q.n < (1ull << 32);
         }();
-        if (same && fits && ids_fit) {
+        const bool in_share = head.q.kind == CAC_QUERY_CC || work + it->work <= share;
+        if (same && fits && ids_fit && in_share) {
+          work += it->work;
+          S->pending_work -= it->work;
           upload += it->upload;
           scratch += it->scratch;
           scratch_max = std::max(scratch_max, it->scratch);
@@ -510,7 +524,13 @@ int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
   }
   if (v == CAC_OK) v = cac::validate(p.q, p.d_edges != nullptr);
   p.upload = p.scratch = 0;
+  p.work = 0;
   if (v == CAC_OK) cac::footprint(p.q, p.d_edges != nullptr, &p.upload, &p.scratch);
+  if (v == CAC_OK && p.q.kind == CAC_QUERY_MINCUT) {
+    uint32_t t, T;
+    cac::mincut_plan(p.q.n, p.q.m, p.q.p, p.q.trials, &t, &T);
+    p.work = (uint64_t)T * std::max<uint64_t>(p.q.m, 1);
+  }
   p.bytes = p.upload + p.scratch;
   p.id = S->next_id++;
   p.submit_s = S->now();
@@ -524,6 +544,7 @@ int cac_server_submit(cac_server* S, const cac_query* q, uint64_t* id_out) {
     return CAC_OK;
   }
   if (p.d_edges) ++S->graphs[q->graph].users;
+  S->pending_work += p.work;
   S->pending.push_back(p);
   S->cv.notify_all();
   return CAC_OK;
