@@ -46,29 +46,7 @@ __device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
   }
 }
 
-// Unsigned maximum over the wave, returned to every lane: inside each row of 16
-// lanes by two quad permutations and two row rotations, then across rows by
-// the row broadcasts of lane 15 (into rows 1 and 3) and lane 31 (into rows 2
-// and 3), which leave the total in lane 63.  VALU data moves only, no LDS.
-__device__ inline uint64_t wave_max_u64(uint64_t v) {
-  uint64_t o = dpp_u64<0xb1, 0xf>(v);  // quad_perm [1,0,3,2]
-  v = o > v ? o : v;
-  o = dpp_u64<0x4e, 0xf>(v);  // quad_perm [2,3,0,1]
-  v = o > v ? o : v;
-  o = dpp_u64<0x124, 0xf>(v);  // row_ror:4
-  v = o > v ? o : v;
-  o = dpp_u64<0x128, 0xf>(v);  // row_ror:8
-  v = o > v ? o : v;
-  o = dpp_u64<0x142, 0xa>(v);  // row_bcast:15
-  v = o > v ? o : v;
-  o = dpp_u64<0x143, 0xc>(v);  // row_bcast:31
-  v = o > v ? o : v;
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ inline int64_t wave_sum_i64(int64_t x) { return (int64_t)wave_sum_u64((uint64_t)x); }
+__device__ inline int64_t wave_sum_i64(int64_t x) { return (int64_t)wave_sum<uint64_t>((uint64_t)x); }
 
 // d - (q-1) M, saturating to a large negative value instead of overflowing
 // (a negative term only weakens the bound).
@@ -274,7 +252,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
       if (packed) {
         const uint64_t p0 = added[0] ? 0 : ((uint64_t)key[0] << 8) | (255u - (uint32_t)lane);
         const uint64_t p1 = added[1] ? 0 : ((uint64_t)key[1] << 8) | (255u - (uint32_t)lane - 64u);
-        const uint64_t b = wave_max_u64(p0 > p1 ? p0 : p1);
+        const uint64_t b = wave_max<uint64_t>(p0 > p1 ? p0 : p1);
         if (b != 0) {
           bk = (int64_t)(b >> 8);
           bi = 255u - (uint32_t)(b & 255u);
@@ -435,7 +413,7 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
   __syncthreads();
   for (uint32_t blk = wave; blk < nb; blk += B / 64) {
     const uint32_t x = blk * 64 + lane;
-    const uint64_t t = wave_sum_u64(x < s ? (uint64_t)L.cap[x] : 0);
+    const uint64_t t = wave_sum<uint64_t>(x < s ? (uint64_t)L.cap[x] : 0);
     if (lane == 0) L.cbs[blk] = t;
   }
   __syncthreads();
@@ -633,12 +611,9 @@ __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ c
       sm += v;
       mx = v > mx ? v : mx;
     }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-      sm += __shfl_xor(sm, d, 64);
-      const int64_t o = __shfl_xor(mx, d, 64);
-      mx = o > mx ? o : mx;
-    }
+    // entries are non-negative, so the unsigned maximum is the maximum
+    sm = (int64_t)wave_sum<uint64_t>((uint64_t)sm);
+    mx = (int64_t)wave_max<uint64_t>((uint64_t)mx);
     if (lane == 0) {
       c.orsum[r] = sm;
       c.ormax[r] = mx;
@@ -655,7 +630,7 @@ __global__ void __launch_bounds__(256) k_row_sums(const int64_t* __restrict__ A,
     const int64_t* row = A + (size_t)r * n;
     uint64_t acc = 0;
     for (uint32_t x = threadIdx.x & 63; x < n; x += 64) acc += (uint64_t)row[x];
-    acc = wave_sum_u64(acc);
+    acc = wave_sum<uint64_t>(acc);
     if ((threadIdx.x & 63) == 0) rsum[r] = (int64_t)acc;
     best = (long long)acc < best ? (long long)acc : best;
   }

@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include "cac_common.h"
+#include "cac_wave.h"
 
 namespace cac {
 
@@ -69,62 +70,11 @@ struct KsLeaf {
   int64_t bound;
 };
 
-// ---- wave / block reductions -------------------------------------------------
-// Called by whole wavefronts.  Cross-lane steps are DPP row moves (VALU data
-// moves, a few cycles each) rather than LDS permutes: the contraction runs
-// about a dozen of these per pick, one after another.
-//
-// One DPP move of a 64-bit value as two 32-bit halves; lanes whose source lane
-// lies outside its row, or whose own row is masked off, get `old`.
-template <int Ctrl, int RowMask>
-__device__ inline uint64_t dpp_u64(uint64_t v, uint64_t old = 0) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)old, (int)(uint32_t)v, Ctrl, RowMask,
-                                                            0xf, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(old >> 32), (int)(uint32_t)(v >> 32),
-                                                            Ctrl, RowMask, 0xf, false);
-  return ((uint64_t)hi << 32) | lo;
-}
-template <int Ctrl, int RowMask>
-__device__ inline uint32_t dpp_u32(uint32_t v, uint32_t old) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, Ctrl, RowMask, 0xf, false);
-}
-__device__ inline uint64_t readlane_u64(uint64_t v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-// Inclusive prefix sum over the wave: Hillis-Steele inside each row of 16 lanes
-// (row_shr 1, 2, 4, 8; a lane with no source adds 0), then the row broadcasts
-// of lane 15 (into rows 1 and 3) and lane 31 (into rows 2 and 3).
-__device__ inline uint64_t wave_incl_scan_u64(uint64_t x, int /*lane*/) {
-  x += dpp_u64<0x111, 0xf>(x);  // row_shr:1
-  x += dpp_u64<0x112, 0xf>(x);  // row_shr:2
-  x += dpp_u64<0x114, 0xf>(x);  // row_shr:4
-  x += dpp_u64<0x118, 0xf>(x);  // row_shr:8
-  x += dpp_u64<0x142, 0xa>(x);  // row_bcast:15
-  x += dpp_u64<0x143, 0xc>(x);  // row_bcast:31
-  return x;
-}
-// Wave total, returned to every lane (the scan's last lane).
-__device__ inline uint64_t wave_sum_u64(uint64_t x) { return readlane_u64(wave_incl_scan_u64(x, 0), 63); }
-// Wave minimum, returned to every lane: inside each row by two quad
-// permutations and two row rotations, then across rows by the broadcasts of
-// lanes 15 and 31, which leave it in lane 63; lanes that receive nothing keep
-// the identity 0xFFFFFFFF.
-__device__ inline uint32_t wave_min_u32(uint32_t x) {
-  x = min(x, dpp_u32<0xb1, 0xf>(x, ~0u));   // quad_perm [1,0,3,2]
-  x = min(x, dpp_u32<0x4e, 0xf>(x, ~0u));   // quad_perm [2,3,0,1]
-  x = min(x, dpp_u32<0x124, 0xf>(x, ~0u));  // row_ror:4
-  x = min(x, dpp_u32<0x128, 0xf>(x, ~0u));  // row_ror:8
-  x = min(x, dpp_u32<0x142, 0xa>(x, ~0u));  // row_bcast:15
-  x = min(x, dpp_u32<0x143, 0xc>(x, ~0u));  // row_bcast:31
-  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
-}
-
+// ---- wave / block reductions (wave primitives in cac_wave.h) --------------------
 // Block sum over B threads; every thread gets the total.  lds: >= B/64 u64.
 template <int B>
 __device__ inline uint64_t block_sum_u64(uint64_t x, uint64_t* lds) {
-  x = wave_sum_u64(x);
+  x = wave_sum<uint64_t>(x);
   if (B == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -138,7 +88,7 @@ __device__ inline uint64_t block_sum_u64(uint64_t x, uint64_t* lds) {
 // lds is written again.
 template <int B>
 __device__ inline uint64_t block_sum_u64_open(uint64_t x, uint64_t* lds) {
-  x = wave_sum_u64(x);
+  x = wave_sum<uint64_t>(x);
   if (B == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -149,7 +99,7 @@ __device__ inline uint64_t block_sum_u64_open(uint64_t x, uint64_t* lds) {
 }
 template <int B>
 __device__ inline uint32_t block_min_u32(uint32_t x, uint32_t* lds) {
-  x = wave_min_u32(x);
+  x = wave_min<uint32_t>(x);
   if (B == 64) return x;
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
   __syncthreads();
@@ -168,7 +118,7 @@ __device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint6
   const int lane = threadIdx.x & 63;
   const uint32_t b0 = 2 * lane, b1 = b0 + 1;
   const uint64_t c0 = b0 < nb ? cbs[b0] : 0, c1 = b1 < nb ? cbs[b1] : 0;
-  const uint64_t incl = wave_incl_scan_u64(c0 + c1, lane);
+  const uint64_t incl = wave_incl_scan<uint64_t>(c0 + c1);
   const uint64_t hit = __ballot(incl >= r);
   if (!hit) return kNoSlot;
   const int owner = __ffsll((long long)hit) - 1;
@@ -182,7 +132,7 @@ __device__ inline uint32_t wave_select_two_level(const int64_t* cap, const uint6
   }
   const uint32_t x = blk * 64 + lane;
   const uint64_t v = x < s ? (uint64_t)cap[x] : 0;
-  const uint64_t inc2 = wave_incl_scan_u64(v, lane);
+  const uint64_t inc2 = wave_incl_scan<uint64_t>(v);
   const uint64_t hit2 = __ballot(x < s && inc2 >= rr);
   if (!hit2) return kNoSlot;
   return blk * 64 + (uint32_t)(__ffsll((long long)hit2) - 1);
@@ -206,7 +156,7 @@ __device__ inline uint32_t block_select_row(const int64_t (&v)[kKsChunk], uint32
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
     if (k < cc) {
-      const uint64_t t = wave_sum_u64((uint64_t)v[k]);
+      const uint64_t t = wave_sum<uint64_t>((uint64_t)v[k]);
       if (lane == 0) lds[k * W + wave] = t;
     }
   }
@@ -230,7 +180,7 @@ __device__ inline uint32_t block_select_row(const int64_t (&v)[kKsChunk], uint32
   if (ks == kNoSlot) return kNoSlot;  // the same for every thread
   uint64_t wb = 0;
   for (int w = 0; w < wave; ++w) wb += lds[ks * W + w];
-  const uint64_t inc = wave_incl_scan_u64(val, lane);
+  const uint64_t inc = wave_incl_scan<uint64_t>(val);
   const uint64_t ex = before + wb + inc - val;
   if (val != 0 && ex < r && r <= ex + val) {  // exactly one thread
     lds_idx[0] = ks * B + threadIdx.x;

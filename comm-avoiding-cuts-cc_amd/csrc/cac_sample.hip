@@ -36,6 +36,7 @@
 #include "cac_ks_kernels.h"
 #include "cac_scan.h"
 #include "cac_sort.h"
+#include "cac_wave.h"
 #include "cac_trial.h"
 
 namespace cac {
@@ -71,11 +72,7 @@ __global__ void __launch_bounds__(256) k_mixed_weights(const EdgeSpan* spans) {
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(sp.mixed, 1u);
 }
 
-__device__ inline uint64_t wsum64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
+__device__ inline uint64_t wsum64(uint64_t x) { return wave_sum<uint64_t>(x); }
 
 __device__ inline uint64_t bsum64(uint64_t x, uint64_t* lds) {
   x = wsum64(x);
@@ -292,12 +289,7 @@ __global__ void __launch_bounds__(kSmpBlock, kMinWaves)
       }
       // block exclusive scan of s
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      uint64_t inc = s;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        uint64_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-      }
+      const uint64_t inc = wave_incl_scan<uint64_t>(s);
       __shared__ uint64_t wtot[kSmpWaves];
       if (lane == 63) wtot[wave] = inc;
       __syncthreads();
@@ -398,12 +390,7 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
     const uint32_t x = base + threadIdx.x;
     const uint32_t live = (x < n && S.lab[x] == x) ? 1u : 0u;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = live;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      uint32_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
+    const uint32_t inc = wave_incl_scan<uint32_t>(live);
     if (lane == 63) wtot[wave] = inc;
     __syncthreads();
     uint32_t off = carry, tot = 0;
@@ -468,10 +455,7 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
       best = acc < best ? acc : best;
     }
   }
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint64_t o = __shfl_xor(best, d, 64);
-    best = o < best ? o : best;
-  }
+  best = wave_min<uint64_t>(best);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = best;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -548,8 +532,7 @@ __global__ void __launch_bounds__(256) k_lt_min(const unsigned long long* deg, u
   __shared__ unsigned long long w[4];
   unsigned long long b = ~0ull;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < k; i += (uint64_t)gridDim.x * 256) b = min(b, deg[i]);
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) b = min(b, (unsigned long long)__shfl_xor(b, d, 64));
+  b = (unsigned long long)wave_min<uint64_t>((uint64_t)b);
   if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = b;
   __syncthreads();
   if (threadIdx.x == 0) atomicMin(out, min(min(w[0], w[1]), min(w[2], w[3])));

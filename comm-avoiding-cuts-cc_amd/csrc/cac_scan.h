@@ -2,12 +2,14 @@
 // rescan), templated on value type, input loader and output writer.
 //
 // Used for the dense ranks of canonical CC labels and for the integer weight
-// prefix that drives edge sampling.  Wave64 shuffles inside a block, LDS for the
+// prefix that drives edge sampling.  DPP wave scans inside a block, LDS for the
 // per-wave totals, one partial per 4096-element tile.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "cac_wave.h"
 
 namespace cac {
 
@@ -17,13 +19,7 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 
 template <class T>
 __device__ inline T wave_inclusive_scan(T x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    T y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
+  return wave_incl_scan<T>(x);
 }
 
 // Exclusive scan of one value per thread across the block; returns the block total.

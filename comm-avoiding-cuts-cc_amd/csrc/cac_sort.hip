@@ -27,6 +27,7 @@
 #include "cac_internal.h"
 #include "cac_scan.h"
 #include "cac_sort.h"
+#include "cac_wave.h"
 
 namespace cac {
 namespace {
@@ -121,12 +122,7 @@ __global__ void __launch_bounds__(kSortBlock) k_radix_scatter(const uint64_t* __
     tot += c;
   }
   {
-    uint32_t inc = tot;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
+    const uint32_t inc = wave_incl_scan<uint32_t>(tot);
     if (lane == 63) wsum[wave] = inc;
     __syncthreads();
     uint32_t add = 0;

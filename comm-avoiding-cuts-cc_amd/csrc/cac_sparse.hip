@@ -137,7 +137,7 @@ __device__ inline uint64_t sp_max(uint64_t x, uint64_t* red) {
 // Exclusive scan of per-thread counts in thread order; *total gets the sum.
 __device__ inline uint64_t sp_exscan(uint64_t x, uint64_t* red, uint64_t* total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t inc = wave_incl_scan_u64(x, lane);
+  const uint64_t inc = wave_incl_scan<uint64_t>(x);
   if (lane == 63) red[wave] = inc;
   __syncthreads();
   uint64_t before = 0, t = 0;
@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(256) k_sp_rowsum(const SpDense* __restrict__ n
     const int64_t* row = d.mat + (uint64_t)r * d.n;
     uint64_t acc = 0;
     for (uint32_t x = lane; x < d.n; x += 64) acc += (uint64_t)row[x];
-    acc = wave_sum_u64(acc);
+    acc = wave_sum<uint64_t>(acc);
     if (lane == 0) d.rsum[r] = (int64_t)acc;
   }
 }

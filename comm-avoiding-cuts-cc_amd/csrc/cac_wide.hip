@@ -42,6 +42,7 @@
 #include "cac_common.h"
 #include "cac_internal.h"
 #include "cac_ks.h"
+#include "cac_wave.h"
 #include "cac_trial.h"
 
 namespace cac {
@@ -124,20 +125,11 @@ __device__ inline bool wuf_union(uint32_t* par, uint32_t a, uint32_t b) {
   return false;
 }
 
-__device__ inline uint64_t wave_sum64(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
+__device__ inline uint64_t wave_sum64(uint64_t x) { return wave_sum<uint64_t>(x); }
 
 template <typename T>
-__device__ inline T wave_incl(T x, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
+__device__ inline T wave_incl(T x, int /*lane*/) {
+  return wave_incl_scan<T>(x);
 }
 
 // Block-wide sum (every thread gets it).
