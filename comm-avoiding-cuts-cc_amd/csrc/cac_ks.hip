@@ -324,7 +324,9 @@ __device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int
 }
 
 // Merge row a into row b (a dies, b keeps its index).  The calling threads hold
-// their chunk of row `held` (a or b) in v; the other row is loaded here.  The
+// their chunk of row `held` (a or b) in v, and of the other row in u when
+// u_has_other (the caller keeps the last merged row there); otherwise the other
+// row is loaded into u here.  On return u holds the merged row (0 past s).  The
 // merged row goes to b's pool row, else a's (a dies, and every thread rewrites
 // exactly the entries it read), else the child's spare pool row (misc[5]),
 // which thread 0 then replaces with a fresh one.  Every thread derives that
@@ -336,12 +338,12 @@ __device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int
 // Called by the whole block.
 template <int B>
 __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, uint32_t a, uint32_t b, uint32_t held,
-                         const int64_t (&v)[kKsChunk], uint32_t cc, int64_t& w) {
+                         const int64_t (&v)[kKsChunk], int64_t (&u)[kKsChunk], bool u_has_other, uint32_t cc,
+                         int64_t& w) {
   const uint32_t s = c.s;
   const uint32_t other = held == a ? b : a;
   const uint32_t slot_a = L.wslot[a], slot_b = L.wslot[b];
-  int64_t u[kKsChunk];
-  load_row<B>(ks_row(c, pool, other == a ? slot_a : slot_b, other), cc, s, u);
+  if (!u_has_other) load_row<B>(ks_row(c, pool, other == a ? slot_a : slot_b, other), cc, s, u);
   const bool spare = slot_b == kNoSlot && slot_a == kNoSlot;
   const uint32_t db = slot_b != kNoSlot ? slot_b : slot_a != kNoSlot ? slot_a : L.misc[5];
   if (db == kNoSlot) return false;  // only a spare can be missing: the pool is exhausted
@@ -356,6 +358,7 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
       if (rx == a || rx == b) t = 0;  // loop of the merged vertex
       if (rx == a) L.rep[x] = b;
       dst[x] = t;  // each thread reads back only its own chunk later
+      u[k] = t;
       sum += (uint64_t)t;
     }
   }
@@ -382,10 +385,11 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
   return true;
 }
 
-// Below 1024 threads, at most 80 VGPRs (6 waves per SIMD): one more workgroup
-// per CU for the mid-sized children.
+// Below 1024 threads, at most 96 VGPRs (5 waves per SIMD): room for the two
+// rows a thread keeps (the picked row and the last merged row) without
+// spilling; at 80 VGPRs (6 waves) the kernel spills and is slower.
 template <int B>
-__global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
+__global__ void __launch_bounds__(B, B >= 1024 ? 1 : 5)
     k_ks_contract(const KsChild* __restrict__ children, KsPool pool, uint32_t* __restrict__ k_out,
                   int64_t* __restrict__ results) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -424,7 +428,12 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
   const uint32_t lo = threadIdx.x * cc;
   uint32_t nv = s, pr1 = 0;
   int status = 0;  // 1 disconnected, 2 bad selection, 3 pool exhausted
-  int64_t v[kKsChunk];
+  // v: the row being picked or checked; u: the last merged row, whose index is
+  // cb (most picks touch it: the heavy super-vertex is picked or is the picked
+  // row's partner, and the PR pass checks it next after merging into it), so
+  // it is not loaded again
+  int64_t v[kKsChunk], u[kKsChunk];
+  uint32_t cb = kNoSlot;
   for (uint32_t p = 0; p + c.tgt < s; ++p) {
     if (W == 0) {
       status = 1;
@@ -441,7 +450,12 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
       status = 2;
       break;
     }
-    load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
+    if (i == cb) {
+#pragma unroll
+      for (uint32_t q = 0; q < kKsChunk; ++q) v[q] = u[q];
+    } else {
+      load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
+    }
     const uint32_t k =
         block_select_row<B>(v, cc, uniform1(hi64(rnd), (uint64_t)L.cap[i]), L.red, L.misc + 1, L.rep);
     if (k >= s) {
@@ -451,10 +465,11 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
     const uint32_t j = L.misc[2];  // rep[k], read by the selecting thread before the merge relabels
     const uint32_t a = i > j ? i : j, b = i > j ? j : i;
     int64_t w = 0;
-    if (!ks_merge<B>(c, pool, L, a, b, i, v, cc, w)) {
+    if (!ks_merge<B>(c, pool, L, a, b, i, v, u, j == cb, cc, w)) {
       status = 3;
       break;
     }
+    cb = b;
     W -= 2 * (uint64_t)w;
     --nv;
     if (w >= c.bound) ++pr1;
@@ -463,7 +478,12 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
   if (status == 0 && pr1 > 0 && c.bound > 0) {
     for (uint32_t i = 0; i < s && nv > 2; ++i) {
       if (L.rep[i] != i || L.cap[i] < c.bound) continue;
-      load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
+      if (i == cb) {
+#pragma unroll
+        for (uint32_t q = 0; q < kKsChunk; ++q) v[q] = u[q];
+      } else {
+        load_row<B>(ks_row(c, pool, L.wslot[i], i), cc, s, v);
+      }
       uint32_t first = kNoSlot;
 #pragma unroll
       for (uint32_t k = 0; k < kKsChunk; ++k)
@@ -473,10 +493,12 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 6)
       // rep[first] is not i: row i's entries of its own class are 0, below the
       // bound, so this merge's relabelling (of class i) leaves it unchanged
       int64_t w = 0;
-      if (!ks_merge<B>(c, pool, L, i, L.rep[first], i, v, cc, w)) {
+      const uint32_t tgt = L.rep[first];
+      if (!ks_merge<B>(c, pool, L, i, tgt, i, v, u, tgt == cb, cc, w)) {
         status = 3;
         break;
       }
+      cb = tgt;
       --nv;
     }
   }
