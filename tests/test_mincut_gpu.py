@@ -99,6 +99,25 @@ def test_sqrt_trials_varying_cuts(ctx, orc):
     assert raw.tolist() == want
 
 
+def test_sqrt_trials_pr_chain_bit_exact(ctx, orc):
+    """G(10^5, 48/n) sampled to t = 3100: one super-vertex takes most of the
+    weight, so each level-0 child runs about 900 picks, then a PR pass of about
+    2200 merges chained through the heavy row (512-thread contraction blocks).
+    Raw trials bit-exact against the oracle."""
+    rng = np.random.default_rng(11)
+    n, m = 100_000, 2_400_000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    E = np.stack([u[keep], v[keep], rng.integers(1, 101, keep.sum())], 1).astype(np.uint64)
+    t = orc.intermediate_size(n, len(E))
+    assert 2048 < t <= 4096
+    raw = ctx.sqrt_cut_trials(E, n, t, 3, 0, 4)
+    want = [orc.sqrt_trial(n, E, t, 3, k) for k in range(4)]
+    _say(f"pr chain: n {n} m {len(E)} t {t}: device {raw.tolist()} oracle {want}")
+    assert raw.tolist() == want
+
+
 def test_mincut_er1000(ctx, orc):
     # the square_root configuration: ER(1000, 0.05), unit weights, p = 0.95, seed 0
     rng = np.random.default_rng(0)
