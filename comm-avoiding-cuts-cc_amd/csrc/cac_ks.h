@@ -23,8 +23,10 @@ struct KsRoot {
 
 // Row sums of a dense device matrix into d_rsum and their minimum into *d_min.
 int ks_row_sums(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_rsum, int64_t* d_min, hipStream_t s);
-// Upper estimate of the device bytes one root needs for one level of children.
-size_t ks_level_bytes_per_root(uint32_t n);
+// Upper estimate of the device bytes one root holds at once in the two level
+// buffers (even depths in one, odd depths in the other, each grown to its
+// largest level: children arrays and row pools).
+size_t ks_live_bytes_per_root(uint32_t n);
 // Runs one trial per root (roots of different seeds share launches);
 // d_results[slot] (initialised to the root's bound) receives min(bound, leaves).
 int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, hipStream_t s);

@@ -827,17 +827,20 @@ int ks_row_sums(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, int64_t* d_rsum,
   return CAC_OK;
 }
 
-size_t ks_level_bytes_per_root(uint32_t n) {
-  // children of every node at one depth with their pool rows (and 1/8 spare)
-  size_t best = 0;
+size_t ks_live_bytes_per_root(uint32_t n) {
+  // children of every node at one depth with their pool rows (and 1/8 spare);
+  // a depth's children go to the buffer its parents are not in, so the two
+  // buffers hold the largest even and the largest odd depth
+  size_t best[2] = {0, 0};
   uint64_t nodes = 1;
-  for (uint32_t sz = n; sz > kKsBase; sz = ks_target(sz), nodes *= 2) {
+  int depth = 0;
+  for (uint32_t sz = n; sz > kKsBase; sz = ks_target(sz), nodes *= 2, ++depth) {
     const uint32_t t = ks_target(sz);
     const size_t b =
         2 * nodes * (child_bytes(sz, t) + sizeof(KsChild) + 16 + child_pool_units(sz, t) * kKsUnit * 8 * 9 / 8);
-    best = std::max(best, b);
+    best[depth & 1] = std::max(best[depth & 1], b);
   }
-  return best + 4096;
+  return best[0] + best[1] + 8192;
 }
 
 // Runs one Karger-Stein trial per root.  roots[i]: device matrix (n_i x n_i)
@@ -1021,11 +1024,14 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
   }
   if (rc == CAC_OK) {
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ntrials, 256, 64)), dim3(256), 0, s, d_res, ntrials, bound);
-    // trials run in batches whose two live levels fit beside the free memory
+    // trials run in batches whose two live levels fit in 3/4 of the free
+    // memory: the top levels are chains of picks with one workgroup per child,
+    // so the more trials a batch holds, the more CUs they keep busy (n = 8192:
+    // 169 trials in two batches, not three)
     size_t avail = 0;
     rc = device_memory_available(ctx, &avail);
-    const size_t per_trial = 2 * ks_level_bytes_per_root(n);
-    size_t batch = std::max<size_t>(1, (size_t)(0.6 * (double)avail) / per_trial);
+    const size_t per_trial = ks_live_bytes_per_root(n);
+    size_t batch = std::max<size_t>(1, (size_t)(0.75 * (double)avail) / per_trial);
     batch = std::min<size_t>(batch, ntrials);
     const size_t nb = (ntrials + batch - 1) / batch;
     batch = (ntrials + nb - 1) / nb;  // even batches

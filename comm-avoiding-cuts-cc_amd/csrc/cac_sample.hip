@@ -727,7 +727,7 @@ size_t job_sample_bytes(const SqrtJob& j, uint32_t dpr) {
 // Upper estimate of the Karger-Stein level arenas of one root of size t: each
 // level holds about t^2 entries in copy-on-write work rows plus the contracted
 // children, and two levels are alive at once.
-size_t job_ks_bytes(const SqrtJob& j) { return 2 * ks_level_bytes_per_root(j.t); }
+size_t job_ks_bytes(const SqrtJob& j) { return ks_live_bytes_per_root(j.t); }
 
 // Edge-count bound from which a job of the batch runs in the wide form.  One
 // workgroup scans a trial's edges in time proportional to m on one CU, while

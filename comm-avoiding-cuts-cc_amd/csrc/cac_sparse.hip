@@ -660,7 +660,7 @@ struct SpRun {
       size_t i1 = i0, bytes = 256, mats = 256;
       uint32_t nmax = 0;
       while (i1 < nodes.size() && i1 - i0 < 4096) {
-        const size_t b = dense_bytes(nodes[i1]) + 2 * ks_level_bytes_per_root(nodes[i1].n);
+        const size_t b = dense_bytes(nodes[i1]) + ks_live_bytes_per_root(nodes[i1].n);
         if (i1 > i0 && bytes + b > budget) break;
         bytes += b;
         mats += dense_bytes(nodes[i1]);
