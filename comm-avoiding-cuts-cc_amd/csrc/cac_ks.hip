@@ -33,6 +33,11 @@ namespace {
 // ---- leaves ------------------------------------------------------------------
 // One wavefront per leaf: lane l owns vertices l and l + 64.
 constexpr int kLeafMax = 128;
+// Row stride of a leaf matrix in LDS: one entry more than a row, so a column
+// (lane x reading or writing entry (x, c), as a merge and the PR scan do) spans
+// the banks instead of hitting one: with 128 int64 per row every row starts on
+// the same bank and a column access is a 64-way bank conflict.
+constexpr int kLeafStride = kLeafMax + 1;
 
 __device__ inline void wave_argmax(int64_t& key, uint32_t& idx) {
 #pragma unroll
@@ -140,15 +145,15 @@ __device__ inline void leaf_merge(int64_t* A, uint32_t V, uint32_t prev, uint32_
   for (int h = 0; h < 2; ++h) {
     const uint32_t x = lane + 64 * h;
     if (x < V && x != prev && x != last) {
-      const int64_t v = A[prev * kLeafMax + x] + A[last * kLeafMax + x];
-      A[prev * kLeafMax + x] = v;
-      A[x * kLeafMax + prev] = v;
+      const int64_t v = A[prev * kLeafStride + x] + A[last * kLeafStride + x];
+      A[prev * kLeafStride + x] = v;
+      A[x * kLeafStride + prev] = v;
     }
   }
   __syncthreads();
   if (lane == 0) {
-    A[prev * kLeafMax + last] = 0;
-    A[last * kLeafMax + prev] = 0;
+    A[prev * kLeafStride + last] = 0;
+    A[last * kLeafStride + prev] = 0;
   }
   if ((uint32_t)lane == last) alive[0] = false;
   if ((uint32_t)lane + 64 == last) alive[1] = false;
@@ -162,7 +167,7 @@ __device__ inline void leaf_merge(int64_t* A, uint32_t V, uint32_t prev, uint32_
 // vertices.  Result min(cut, bound).
 __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves, const uint32_t* __restrict__ sw_list,
                                               int64_t* __restrict__ results) {
-  __shared__ int64_t A[kLeafMax * kLeafMax];
+  __shared__ int64_t A[kLeafMax * kLeafStride];
   const KsLeaf lf = leaves[sw_list[blockIdx.x]];
   const uint32_t V = lf.k;
   const int lane = threadIdx.x;
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
   for (uint32_t r = 0; r < V; ++r)
     for (uint32_t x = lane; x < V; x += 64) {
       const int64_t a = lf.mat[(size_t)r * V + x];
-      A[r * kLeafMax + x] = a;
+      A[r * kLeafStride + x] = a;
       total += a;
     }
   total = wave_sum_i64(total);
@@ -194,7 +199,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     uint32_t c = kNoSlot;
     for (uint32_t x = 0; x < V; ++x) {
       const bool live_x = ((x < 64 ? live_lo : live_hi) >> (x & 63)) & 1ull;
-      if (live_x && x != u && A[u * kLeafMax + x] >= lf.bound) {
+      if (live_x && x != u && A[u * kLeafStride + x] >= lf.bound) {
         c = x;
         break;
       }
@@ -220,8 +225,8 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     // and l + 64, and the first set bit of the ballots is the first column
     uint32_t again;
     {
-      const bool q0 = alive[0] && (uint32_t)lane != prev && A[prev * kLeafMax + lane] >= lf.bound;
-      const bool q1 = alive[1] && (uint32_t)lane + 64 != prev && A[prev * kLeafMax + lane + 64] >= lf.bound;
+      const bool q0 = alive[0] && (uint32_t)lane != prev && A[prev * kLeafStride + lane] >= lf.bound;
+      const bool q1 = alive[1] && (uint32_t)lane + 64 != prev && A[prev * kLeafStride + lane + 64] >= lf.bound;
       const uint64_t b0 = __ballot(q0), b1 = __ballot(q1);
       again = b0 ? (uint32_t)(__ffsll((long long)b0) - 1) : b1 ? 64u + (uint32_t)(__ffsll((long long)b1) - 1) : kNoSlot;
     }
@@ -230,7 +235,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
       if (!alive[h]) col[h] = kNoSlot;
       else if (r == prev) col[h] = again;
       else if (col[h] == last) col[h] = prev;
-      else if (prev < col[h] && A[r * kLeafMax + prev] >= lf.bound) col[h] = prev;
+      else if (prev < col[h] && A[r * kLeafStride + prev] >= lf.bound) col[h] = prev;
     }
   }
   if (lf.bound <= 0) nalive = 1;  // a non-positive bound is already the answer
@@ -244,7 +249,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     uint32_t prev = start, last = start;
     if ((uint32_t)lane == start) added[0] = true;
     if ((uint32_t)lane + 64 == start) added[1] = true;
-    for (int h = 0; h < 2; ++h) key[h] += A[start * kLeafMax + lane + 64 * h];
+    for (int h = 0; h < 2; ++h) key[h] += A[start * kLeafStride + lane + 64 * h];
     int64_t cut = 0;
     for (uint32_t step = 1; step < nalive; ++step) {
       int64_t bk = INT64_MIN;
@@ -276,7 +281,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
         break;
       }
       for (int h = 0; h < 2; ++h)
-        if (!added[h]) key[h] += A[bi * kLeafMax + lane + 64 * h];
+        if (!added[h]) key[h] += A[bi * kLeafStride + lane + 64 * h];
     }
     best = cut < best ? cut : best;
     leaf_merge(A, V, prev, last, alive);
