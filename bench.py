@@ -806,6 +806,11 @@ def main():
                     ("ap", lambda: bench_ap(ctx, cac, [int(x) for x in args.sq_sizes.split(",")],
                                                     min(args.cpu_baseline_seconds, 10.0)))):
         if key in wanted:
+            # every leg starts on a fresh context: a context keeps the buffers
+            # it grew (the dense Karger-Stein levels of c3 hold ~160 GB), and
+            # the served leg's workers need that memory
+            ctx.close()
+            ctx = cac.Context(0)
             t0 = time.perf_counter()
             extras[key] = fn()
             print(f"# {key} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
