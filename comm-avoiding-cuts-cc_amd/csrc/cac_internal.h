@@ -31,6 +31,9 @@ struct cac_ctx {
   // in one, children in the other).
   void* ks_buf[2] = {nullptr, nullptr};
   size_t ks_bytes[2] = {0, 0};
+  // compaction work-item prefix of the last dense Karger-Stein level: its
+  // upload may still be in flight when the level returns, so it lives here
+  std::vector<uint32_t> ks_items;
   void* ks_retry_buf = nullptr;  // children redone with a full-size row pool
   size_t ks_retry_bytes = 0;
   // Grow-only buffer for the per-trial scratch of the sampling contraction.

@@ -766,21 +766,15 @@ void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, K
 }
 
 // One level's launches for the children ch (already in d_children's order, by
-// block-size class): contraction per class, then compaction; waits and returns
-// the children's k_out.
+// block-size class): contraction per class; waits for the children's k_out;
+// then the compaction, sized by those k (a child that the PR pass collapsed
+// takes a few workgroups, not t' / rows of them), left running on the stream
+// (the next level's work follows it there).
 int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_children, uint32_t* d_prefix,
                    uint32_t* d_kout, unsigned long long* d_used, KsPool pool, uint32_t max_tgt, int64_t* d_results,
                    std::vector<uint32_t>& kout, hipStream_t s) {
   const size_t nchild = ch.size();
-  std::vector<uint32_t> item_prefix(nchild + 1);
-  uint32_t items = 0;
-  for (size_t i = 0; i < nchild; ++i) {
-    item_prefix[i] = items;
-    items += (ch[i].tgt + compact_rows(ch[i].tgt) - 1) / compact_rows(ch[i].tgt);
-  }
-  item_prefix[nchild] = items;
   CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
-  CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, d_used, 1u);
   prof_begin(ctx, "ks_contract", s);
   size_t c0 = 0;
@@ -804,6 +798,21 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
   }
   prof_end(ctx, s);
   CAC_LAUNCH_CHECK(ctx, "ks contract");
+  CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipStreamSynchronize(s));
+  // compaction work items: ceil(k / rows) per child of 1 <= k <= t' (0 for a
+  // disconnected child and for the error codes)
+  std::vector<uint32_t>& item_prefix = ctx->ks_items;  // rewritten only after the next synchronisation
+  item_prefix.assign(nchild + 1, 0u);
+  uint32_t items = 0;
+  for (size_t i = 0; i < nchild; ++i) {
+    item_prefix[i] = items;
+    const uint32_t k = kout[i];
+    if (k >= 1 && k <= ch[i].tgt) items += (k + compact_rows(ch[i].tgt) - 1) / compact_rows(ch[i].tgt);
+  }
+  item_prefix[nchild] = items;
+  if (items == 0) return CAC_OK;
+  CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
   prof_begin(ctx, "ks_compact", s);
   const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
   const int cb = compact_block(ctx, max_tgt);
@@ -818,8 +827,6 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
                        (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
   prof_end(ctx, s);
   CAC_LAUNCH_CHECK(ctx, "ks compact");
-  CAC_HIP(ctx, hipMemcpyAsync(kout.data(), d_kout, nchild * 4, hipMemcpyDeviceToHost, s));
-  CAC_HIP(ctx, hipStreamSynchronize(s));
   return CAC_OK;
 }
 
