@@ -132,20 +132,25 @@ __global__ void __launch_bounds__(64) k_ks_cert(const KsLeaf* __restrict__ leave
     }
   }
   const int64_t lb = V >= 2 ? leaf_lower_bound(d, M, alive, V, lf.bound) : 0;
+  // leaves whose total weight is below 2^32 (every entry and every merged
+  // entry fits 32 bits) go to the front of the list, the others to the back
+  const uint64_t total = wave_sum<uint64_t>((uint64_t)(d[0] + d[1]));
   if (lane == 0) {
     if (lb >= lf.bound) atomicMin((long long*)&results[lf.slot], (long long)lf.bound);
-    else sw_list[atomicAdd(sw_count, 1u)] = blockIdx.x;
+    else if (total < (1ull << 32)) sw_list[atomicAdd(&sw_count[0], 1u)] = blockIdx.x;
+    else sw_list[gridDim.x - 1 - atomicAdd(&sw_count[1], 1u)] = blockIdx.x;
   }
 }
 
 // Merge vertex `last` into `prev` in the LDS matrix (row and column); lanes own
 // columns lane and lane + 64.
-__device__ inline void leaf_merge(int64_t* A, uint32_t V, uint32_t prev, uint32_t last, bool (&alive)[2]) {
+template <typename T>
+__device__ inline void leaf_merge(T* A, uint32_t V, uint32_t prev, uint32_t last, bool (&alive)[2]) {
   const int lane = threadIdx.x;
   for (int h = 0; h < 2; ++h) {
     const uint32_t x = lane + 64 * h;
     if (x < V && x != prev && x != last) {
-      const int64_t v = A[prev * kLeafStride + x] + A[last * kLeafStride + x];
+      const T v = A[prev * kLeafStride + x] + A[last * kLeafStride + x];
       A[prev * kLeafStride + x] = v;
       A[x * kLeafStride + prev] = v;
     }
@@ -164,10 +169,13 @@ __device__ inline void leaf_merge(int64_t* A, uint32_t V, uint32_t prev, uint32_
 // at the bound (contracting an edge of weight >= bound keeps every cut below the
 // bound, so min(bound, min cut) is unchanged), then Stoer-Wagner (J. ACM 44(4),
 // 1997): maximum adjacency order, cut of the phase, merge of the last two
-// vertices.  Result min(cut, bound).
+// vertices.  Result min(cut, bound).  T: the LDS entry type, uint32_t for a
+// leaf of total weight below 2^32 (66 KiB, two leaves per CU) and int64_t
+// otherwise (132 KiB, one per CU).
+template <typename T>
 __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves, const uint32_t* __restrict__ sw_list,
                                               int64_t* __restrict__ results) {
-  __shared__ int64_t A[kLeafMax * kLeafStride];
+  __shared__ T A[kLeafMax * kLeafStride];
   const KsLeaf lf = leaves[sw_list[blockIdx.x]];
   const uint32_t V = lf.k;
   const int lane = threadIdx.x;
@@ -175,7 +183,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
   for (uint32_t r = 0; r < V; ++r)
     for (uint32_t x = lane; x < V; x += 64) {
       const int64_t a = lf.mat[(size_t)r * V + x];
-      A[r * kLeafStride + x] = a;
+      A[r * kLeafStride + x] = (T)a;
       total += a;
     }
   total = wave_sum_i64(total);
@@ -199,7 +207,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     uint32_t c = kNoSlot;
     for (uint32_t x = 0; x < V; ++x) {
       const bool live_x = ((x < 64 ? live_lo : live_hi) >> (x & 63)) & 1ull;
-      if (live_x && x != u && A[u * kLeafStride + x] >= lf.bound) {
+      if (live_x && x != u && (int64_t)A[u * kLeafStride + x] >= lf.bound) {
         c = x;
         break;
       }
@@ -225,8 +233,9 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     // and l + 64, and the first set bit of the ballots is the first column
     uint32_t again;
     {
-      const bool q0 = alive[0] && (uint32_t)lane != prev && A[prev * kLeafStride + lane] >= lf.bound;
-      const bool q1 = alive[1] && (uint32_t)lane + 64 != prev && A[prev * kLeafStride + lane + 64] >= lf.bound;
+      const bool q0 = alive[0] && (uint32_t)lane != prev && (int64_t)A[prev * kLeafStride + lane] >= lf.bound;
+      const bool q1 =
+          alive[1] && (uint32_t)lane + 64 != prev && (int64_t)A[prev * kLeafStride + lane + 64] >= lf.bound;
       const uint64_t b0 = __ballot(q0), b1 = __ballot(q1);
       again = b0 ? (uint32_t)(__ffsll((long long)b0) - 1) : b1 ? 64u + (uint32_t)(__ffsll((long long)b1) - 1) : kNoSlot;
     }
@@ -235,7 +244,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
       if (!alive[h]) col[h] = kNoSlot;
       else if (r == prev) col[h] = again;
       else if (col[h] == last) col[h] = prev;
-      else if (prev < col[h] && A[r * kLeafStride + prev] >= lf.bound) col[h] = prev;
+      else if (prev < col[h] && (int64_t)A[r * kLeafStride + prev] >= lf.bound) col[h] = prev;
     }
   }
   if (lf.bound <= 0) nalive = 1;  // a non-positive bound is already the answer
@@ -249,7 +258,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
     uint32_t prev = start, last = start;
     if ((uint32_t)lane == start) added[0] = true;
     if ((uint32_t)lane + 64 == start) added[1] = true;
-    for (int h = 0; h < 2; ++h) key[h] += A[start * kLeafStride + lane + 64 * h];
+    for (int h = 0; h < 2; ++h) key[h] += (int64_t)A[start * kLeafStride + lane + 64 * h];
     int64_t cut = 0;
     for (uint32_t step = 1; step < nalive; ++step) {
       int64_t bk = INT64_MIN;
@@ -281,7 +290,7 @@ __global__ void __launch_bounds__(64) k_ks_sw(const KsLeaf* __restrict__ leaves,
         break;
       }
       for (int h = 0; h < 2; ++h)
-        if (!added[h]) key[h] += A[bi * kLeafStride + lane + 64 * h];
+        if (!added[h]) key[h] += (int64_t)A[bi * kLeafStride + lane + 64 * h];
     }
     best = cut < best ? cut : best;
     leaf_merge(A, V, prev, last, alive);
@@ -738,19 +747,24 @@ int run_leaves(cac_ctx* ctx, const std::vector<KsLeaf>& leaves, int buf, int64_t
   Carve64 cv(base);
   KsLeaf* d_leaves = cv.take<KsLeaf>(n);
   uint32_t* d_list = cv.take<uint32_t>(n);
-  uint32_t* d_count = cv.take<uint32_t>(2);
+  uint32_t* d_count = cv.take<uint32_t>(2);  // [0] 32-bit leaves (list front), [1] the others (list back)
   CAC_HIP(ctx, hipMemcpyAsync(d_leaves, leaves.data(), n * sizeof(KsLeaf), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, (unsigned long long*)d_count, 1u);
   prof_begin(ctx, "ks_cert", s);
   hipLaunchKernelGGL(k_ks_cert, dim3((unsigned)n), dim3(64), 0, s, (const KsLeaf*)d_leaves, d_results, d_list, d_count);
   prof_end(ctx, s);
   CAC_LAUNCH_CHECK(ctx, "ks certificate");
-  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_count, 4, hipMemcpyDeviceToHost, s));
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_count, 8, hipMemcpyDeviceToHost, s));
   CAC_HIP(ctx, hipStreamSynchronize(s));
-  const uint32_t nsw = *(uint32_t*)ctx->pinned;
-  if (nsw) {
+  const uint32_t n32 = ((uint32_t*)ctx->pinned)[0], n64 = ((uint32_t*)ctx->pinned)[1];
+  if (n32 + n64) {
     prof_begin(ctx, "ks_sw", s);
-    hipLaunchKernelGGL(k_ks_sw, dim3(nsw), dim3(64), 0, s, (const KsLeaf*)d_leaves, (const uint32_t*)d_list, d_results);
+    if (n32)
+      hipLaunchKernelGGL(k_ks_sw<uint32_t>, dim3(n32), dim3(64), 0, s, (const KsLeaf*)d_leaves,
+                         (const uint32_t*)d_list, d_results);
+    if (n64)
+      hipLaunchKernelGGL(k_ks_sw<int64_t>, dim3(n64), dim3(64), 0, s, (const KsLeaf*)d_leaves,
+                         (const uint32_t*)(d_list + (n - n64)), d_results);
     prof_end(ctx, s);
     CAC_LAUNCH_CHECK(ctx, "ks stoer-wagner");
   }
