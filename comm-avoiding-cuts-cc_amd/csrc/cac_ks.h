@@ -19,6 +19,7 @@ struct KsRoot {
   int64_t bound;     // the trial's upper bound (minimum weighted degree)
   uint32_t seed;     // Philox key word 0 of the trial
   uint32_t nid = 1;  // recursion node id of the matrix (1: the trial's root)
+  bool w32 = false;  // total weight below 2^32: every node of the recursion fits 32-bit sums
 };
 
 // Row sums of a dense device matrix into d_rsum and their minimum into *d_min.

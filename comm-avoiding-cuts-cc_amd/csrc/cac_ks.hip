@@ -581,13 +581,17 @@ __host__ __device__ inline uint32_t compact_rows(uint32_t tgt) {
 // class into LDS, then writes the rows with their sums and maxima.  The work
 // item's child comes from a binary search of the per-child item prefix.  CB
 // threads per workgroup: large children, whose rows go one per workgroup, take
-// more threads, so more of a row's loads are in flight per CU.
-template <int CB>
+// more threads, so more of a row's loads are in flight per CU.  AccT: the LDS
+// accumulator, uint32_t when every node of the level weighs less than 2^32 in
+// total (no class sum can exceed it; half the LDS and one bank per atomic),
+// unsigned long long otherwise.
+template <int CB, typename AccT>
 __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ children,
                                                    const uint32_t* __restrict__ k_out,
                                                    const uint32_t* __restrict__ item_prefix, uint32_t nchild,
                                                    KsPool pool) {
-  extern __shared__ __align__(16) unsigned long long acc[];
+  extern __shared__ __align__(16) unsigned char acc_lds[];
+  AccT* acc = reinterpret_cast<AccT*>(acc_lds);
   const uint32_t item = blockIdx.x;
   uint32_t lo = 0, hi = nchild;  // last child with item_prefix[child] <= item
   while (hi - lo > 1) {
@@ -603,7 +607,7 @@ __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ c
   if (r0 >= kk) return;
   const uint32_t nr = min(R, kk - r0);
   const uint32_t s = c.s;
-  for (uint32_t x = threadIdx.x; x < nr * kk; x += CB) acc[x] = 0ull;
+  for (uint32_t x = threadIdx.x; x < nr * kk; x += CB) acc[x] = 0;
   __syncthreads();
   // Rows and the column map are global memory (the struct's pointers are
   // generic, which would give flat loads); kCompactUnroll elements per thread
@@ -616,7 +620,7 @@ __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ c
   for (uint32_t rr = 0; rr < nr; ++rr) {
     const uint32_t e = c.live[r0 + rr];
     g_i64* src = (g_i64*)((e & kLiveWork) ? pool.base + (size_t)(e & ~kLiveWork) * kKsUnit : c.parent + (size_t)e * s);
-    unsigned long long* a = acc + (size_t)rr * kk;
+    AccT* a = acc + (size_t)rr * kk;
     uint32_t y = threadIdx.x;
     for (; y + (U - 1) * CB < s; y += U * CB) {
       int64_t v[U];
@@ -628,11 +632,11 @@ __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ c
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u)
-        if (v[u]) atomicAdd(&a[m[u]], (unsigned long long)v[u]);
+        if (v[u]) atomicAdd(&a[m[u]], (AccT)v[u]);
     }
     for (; y < s; y += CB) {
       const int64_t v = src[y];
-      if (v) atomicAdd(&a[cm[y]], (unsigned long long)v);
+      if (v) atomicAdd(&a[cm[y]], (AccT)v);
     }
   }
   __syncthreads();
@@ -642,7 +646,7 @@ __global__ void __launch_bounds__(CB) k_ks_compact(const KsChild* __restrict__ c
     int64_t* dst = c.out + (size_t)r * kk;
     int64_t sm = 0, mx = 0;
     for (uint32_t x = lane; x < kk; x += 64) {
-      const int64_t v = (int64_t)acc[(size_t)rr * kk + x];
+      const int64_t v = (int64_t)(uint64_t)acc[(size_t)rr * kk + x];
       dst[x] = v;
       sm += v;
       mx = v > mx ? v : mx;
@@ -786,7 +790,7 @@ void launch_contract(const KsChild* d_children, uint32_t count, uint32_t smax, K
 // (the next level's work follows it there).
 int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_children, uint32_t* d_prefix,
                    uint32_t* d_kout, unsigned long long* d_used, KsPool pool, uint32_t max_tgt, int64_t* d_results,
-                   std::vector<uint32_t>& kout, hipStream_t s) {
+                   std::vector<uint32_t>& kout, bool w32, hipStream_t s) {
   const size_t nchild = ch.size();
   CAC_HIP(ctx, hipMemcpyAsync(d_children, ch.data(), nchild * sizeof(KsChild), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, d_used, 1u);
@@ -828,17 +832,23 @@ int contract_level(cac_ctx* ctx, const std::vector<KsChild>& ch, KsChild* d_chil
   if (items == 0) return CAC_OK;
   CAC_HIP(ctx, hipMemcpyAsync(d_prefix, item_prefix.data(), (nchild + 1) * 4, hipMemcpyHostToDevice, s));
   prof_begin(ctx, "ks_compact", s);
-  const size_t lds = std::max<size_t>(4096 * 8, (size_t)max_tgt * 8);
+  const size_t acc_bytes = w32 ? 4 : 8;
+  const size_t lds = std::max<size_t>(4096, max_tgt) * acc_bytes;
   const int cb = compact_block(ctx, max_tgt);
-  if (cb == 1024)
-    hipLaunchKernelGGL(k_ks_compact<1024>, dim3(items), dim3(1024), lds, s, (const KsChild*)d_children,
-                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
-  else if (cb == 512)
-    hipLaunchKernelGGL(k_ks_compact<512>, dim3(items), dim3(512), lds, s, (const KsChild*)d_children,
-                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
-  else
-    hipLaunchKernelGGL(k_ks_compact<256>, dim3(items), dim3(256), lds, s, (const KsChild*)d_children,
-                       (const uint32_t*)d_kout, (const uint32_t*)d_prefix, (uint32_t)nchild, pool);
+  const KsChild* dc = d_children;
+  const uint32_t* dk = d_kout;
+  const uint32_t* dp = d_prefix;
+  const uint32_t nc = (uint32_t)nchild;
+  if (w32) {
+    if (cb == 1024) hipLaunchKernelGGL((k_ks_compact<1024, uint32_t>), dim3(items), dim3(1024), lds, s, dc, dk, dp, nc, pool);
+    else if (cb == 512) hipLaunchKernelGGL((k_ks_compact<512, uint32_t>), dim3(items), dim3(512), lds, s, dc, dk, dp, nc, pool);
+    else hipLaunchKernelGGL((k_ks_compact<256, uint32_t>), dim3(items), dim3(256), lds, s, dc, dk, dp, nc, pool);
+  } else {
+    typedef unsigned long long u64;
+    if (cb == 1024) hipLaunchKernelGGL((k_ks_compact<1024, u64>), dim3(items), dim3(1024), lds, s, dc, dk, dp, nc, pool);
+    else if (cb == 512) hipLaunchKernelGGL((k_ks_compact<512, u64>), dim3(items), dim3(512), lds, s, dc, dk, dp, nc, pool);
+    else hipLaunchKernelGGL((k_ks_compact<256, u64>), dim3(items), dim3(256), lds, s, dc, dk, dp, nc, pool);
+  }
   prof_end(ctx, s);
   CAC_LAUNCH_CHECK(ctx, "ks compact");
   return CAC_OK;
@@ -881,6 +891,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, h
     uint32_t s, trial, nid, slot;
     int64_t bound;
     uint32_t seed;
+    bool w32;  // the root's total weight is below 2^32 (a child's total never exceeds its parent's)
   };
   static std::once_flag lds_attr;
   hipError_t attr_err = hipSuccess;
@@ -898,7 +909,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, h
   std::vector<Node> level;
   for (const KsRoot& r : roots) {
     if (r.n > kKsMaxS) return fail(ctx, CAC_EINVAL, "dense Karger-Stein supports at most 8192 vertices");
-    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, r.nid, r.slot, r.bound, r.seed});
+    level.push_back({r.mat, r.rsum, nullptr, r.n, r.trial, r.nid, r.slot, r.bound, r.seed, r.w32});
   }
   int buf = 0;  // level buffer for the children of this level; the parents are in the other one
   while (!level.empty()) {
@@ -968,7 +979,9 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, h
     }
     cv.off = (cv.off + 511) & ~size_t(511);
     KsPool pool{(int64_t*)(cv.base + cv.off), d_used, pool_units};
-    rc = contract_level(ctx, ch, d_children, d_prefix, d_kout, d_used, pool, max_tgt, d_results, kout, s);
+    bool w32 = true;
+    for (const Node& nd : internal) w32 = w32 && nd.w32;
+    rc = contract_level(ctx, ch, d_children, d_prefix, d_kout, d_used, pool, max_tgt, d_results, kout, w32, s);
     if (rc) return rc;
     // children that ran out of pool rows: again, with s pool rows each (a child
     // merges at most s - 2 times, so this cannot overflow)
@@ -995,7 +1008,7 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, h
       rv.off = (rv.off + 511) & ~size_t(511);
       KsPool rpool{(int64_t*)(rv.base + rv.off), d_rused, runits};
       std::vector<uint32_t> rkout(no);
-      rc = contract_level(ctx, rch, d_rch, d_rprefix, d_rkout, d_rused, rpool, max_tgt, d_results, rkout, s);
+      rc = contract_level(ctx, rch, d_rch, d_rprefix, d_rkout, d_rused, rpool, max_tgt, d_results, rkout, w32, s);
       if (rc) return rc;
       for (size_t k = 0; k < no; ++k) {
         if (rkout[k] == kKsOverflow) return fail(ctx, CAC_EDEVICE, "Karger-Stein row pool overflow");
@@ -1006,8 +1019,8 @@ int ks_run(cac_ctx* ctx, const std::vector<KsRoot>& roots, int64_t* d_results, h
     for (size_t i = 0; i < nchild; ++i) {
       if (kout[i] == kKsBad) return fail(ctx, CAC_EDEVICE, "Karger-Stein selection out of range");
       if (kout[i] == 0) continue;  // disconnected child: its 0 is already recorded
-      next.push_back(
-          {ch[i].out, ch[i].orsum, ch[i].ormax, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound, ch[i].seed});
+      next.push_back({ch[i].out, ch[i].orsum, ch[i].ormax, kout[i], ch[i].trial, ch[i].nid, ch[i].slot, ch[i].bound,
+                      ch[i].seed, w32});
     }
     level.swap(next);
     buf ^= 1;
@@ -1043,10 +1056,20 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
   int64_t* d_res = d_small + 1 + n;
   int rc = ks_row_sums(ctx, d_adj, n, d_rsum, d_bound, s);
   int64_t bound = 0;
+  std::vector<int64_t> rs(n);  // the row sums: their total decides 32-bit compaction sums
   if (rc == CAC_OK) {
     hipError_t e = hipMemcpyAsync(&bound, d_bound, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(rs.data(), d_rsum, 8 * (size_t)n, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = hip_fail(ctx, e, "row sums");
+  }
+  bool w32 = rc == CAC_OK;  // total weight below 2^32, checked without wrapping
+  uint64_t total = 0;
+  for (int64_t r : rs) {
+    if (r < 0 || (uint64_t)r >= (1ull << 32) || (total += (uint64_t)r) >= (1ull << 32)) {
+      w32 = false;
+      break;
+    }
   }
   if (rc == CAC_OK) {
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ntrials, 256, 64)), dim3(256), 0, s, d_res, ntrials, bound);
@@ -1064,7 +1087,7 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
     for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
       std::vector<KsRoot> roots;
       for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
-        roots.push_back({d_adj, d_rsum, n, first_trial + t, t, bound, seed});
+        roots.push_back({d_adj, d_rsum, n, first_trial + t, t, bound, seed, 1, w32});
       rc = ks_run(ctx, roots, d_res, s);
     }
   }
