@@ -16,11 +16,14 @@
 // identity between probes makes each probe cost only its own edges.  The child
 // is the node's edges in their order, relabelled by the rank of each
 // component's minimum id, loops dropped and parallel edges kept.  A node with
-// n <= 2048, or with m >= n^2/32 and n <= 8192, is densified and continues in
-// the dense engine under its own node id.  Every node uses the trial's bound,
+// n <= 2048, or with m >= n^2/32 and n <= 8192, is densified (band by band in
+// LDS: every matrix entry written once) and continues in the dense engine under
+// its own node id.  Every node uses the trial's bound,
 // the minimum weighted degree of the input.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "cac_common.h"
@@ -525,45 +528,146 @@ void launch_sparse_contract(const SpChild* d_children, uint32_t nchild, uint64_t
                        d_children + c0);
 }
 
-// Handover to the dense engine: n x n matrix and row sums from an edge list.
+// Handover to the dense engine: n x n matrix and row sums from an edge list,
+// built band by band.  A band is `rows` consecutive rows (rows n <= kSpBandEntries,
+// so a band's entries fit LDS).  The edge endpoints are bucketed by band (count,
+// exclusive scan, place: entry weight and its index inside the band), then a
+// workgroup sums each band's entries in LDS and writes the band's rows, zeros
+// included, and their row sums, once.  Every matrix entry is written once and
+// never read back; the only atomics are on LDS and on per-tile band cursors.
+constexpr uint32_t kSpBandEntries = 8192;                       // int64 entries of a band (64 KiB of LDS)
+constexpr int kSpBucketBlock = 1024;                            // bucketing: edges per thread x block = a tile
+constexpr int kSpBucketEdges = 8;
+constexpr uint64_t kSpBucketTile = (uint64_t)kSpBucketBlock * kSpBucketEdges;
+constexpr int kSpBuildBlock = 512;
+constexpr size_t kSpBucketLdsMax = (size_t)kKsMaxS * 12;        // bands <= n <= kKsMaxS: u64 base + u32 count
+
 struct SpDense {
   const cac_edge* edges;
   uint64_t m;
-  uint32_t n, pad;
-  int64_t* mat;
-  int64_t* rsum;
+  uint32_t n, rows;              // rows per band
+  uint32_t bands, pad;           // ceil(n / rows)
+  int64_t* mat;                  // n x n
+  int64_t* rsum;                 // [n]
+  unsigned long long* boff;      // [bands + 1]: entry counts, then their exclusive prefix (zeroed first)
+  unsigned long long* bcur;      // [bands]: placement cursors
+  unsigned long long* ew;        // [2m] entry weights, grouped by band
+  uint16_t* eli;                 // [2m] entry index inside its band: (row - band * rows) * n + column
 };
 
-// Zeroes every handed-over matrix with 16-byte stores (n x n int64 is a whole
-// number of 16-byte words when n is even; an odd tail word is stored alone).
-__global__ void __launch_bounds__(256) k_sp_zero(const SpDense* __restrict__ nodes) {
-  const SpDense d = nodes[blockIdx.y];
-  const uint64_t nn = (uint64_t)d.n * d.n;
-  ulonglong2* m2 = (ulonglong2*)d.mat;
-  const ulonglong2 z{0ull, 0ull};
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nn / 2; i += (uint64_t)gridDim.x * 256) m2[i] = z;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && (nn & 1)) d.mat[nn - 1] = 0;
-}
+inline uint32_t band_rows(uint32_t n) { return n ? std::max(1u, std::min(n, kSpBandEntries / n)) : 1u; }
 
-__global__ void __launch_bounds__(256) k_sp_scatter(const SpDense* __restrict__ nodes) {
+// Counts (kPlace false) or places (kPlace true) the entries of one tile of a
+// node's edges: edge (u, v, w) gives entry (u, v) in band u / rows and (v, u) in
+// band v / rows.  Placing reserves each band's range of the tile with one
+// atomic on the band's cursor; the order inside a range is arbitrary, which
+// the integer sums do not see.  Dynamic LDS: bands x (8 + 4) bytes.
+template <bool kPlace>
+__global__ void __launch_bounds__(kSpBucketBlock) k_sp_band_bucket(const SpDense* __restrict__ nodes) {
+  extern __shared__ unsigned long long band_lds[];
   const SpDense d = nodes[blockIdx.y];
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < d.m; i += (uint64_t)gridDim.x * 256) {
-    const cac_edge x = d.edges[i];
-    atomicAdd((unsigned long long*)&d.mat[(uint64_t)x.u * d.n + x.v], (unsigned long long)x.w);
-    atomicAdd((unsigned long long*)&d.mat[(uint64_t)x.v * d.n + x.u], (unsigned long long)x.w);
+  const uint64_t e0 = (uint64_t)blockIdx.x * kSpBucketTile;
+  if (e0 >= d.m) return;  // the same for the whole block
+  unsigned long long* base = band_lds;                            // [bands] (place)
+  uint32_t* cnt = (uint32_t*)(band_lds + (kPlace ? d.bands : 0));  // [bands]
+  for (uint32_t b = threadIdx.x; b < d.bands; b += kSpBucketBlock) cnt[b] = 0;
+  __syncthreads();
+  const uint32_t R = d.rows;
+  cac_edge x[kSpBucketEdges];
+  uint32_t bu[kSpBucketEdges], bv[kSpBucketEdges];
+#pragma unroll
+  for (int j = 0; j < kSpBucketEdges; ++j) {
+    const uint64_t e = e0 + (uint64_t)j * kSpBucketBlock + threadIdx.x;
+    x[j] = e < d.m ? d.edges[e] : cac_edge{0u, 0u, 0ull};
+  }
+#pragma unroll
+  for (int j = 0; j < kSpBucketEdges; ++j) {
+    bu[j] = x[j].u / R;
+    bv[j] = x[j].v / R;
+    if (e0 + (uint64_t)j * kSpBucketBlock + threadIdx.x < d.m) {
+      atomicAdd(&cnt[bu[j]], 1u);
+      atomicAdd(&cnt[bv[j]], 1u);
+    }
+  }
+  __syncthreads();
+  if constexpr (!kPlace) {
+    for (uint32_t b = threadIdx.x; b < d.bands; b += kSpBucketBlock)
+      if (cnt[b]) atomicAdd(&d.boff[b], (unsigned long long)cnt[b]);
+  } else {
+    for (uint32_t b = threadIdx.x; b < d.bands; b += kSpBucketBlock) {
+      base[b] = cnt[b] ? atomicAdd(&d.bcur[b], (unsigned long long)cnt[b]) : 0ull;
+      cnt[b] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSpBucketEdges; ++j) {
+      if (e0 + (uint64_t)j * kSpBucketBlock + threadIdx.x >= d.m) continue;
+      const uint64_t pu = base[bu[j]] + atomicAdd(&cnt[bu[j]], 1u);
+      d.ew[pu] = x[j].w;
+      d.eli[pu] = (uint16_t)((x[j].u - bu[j] * R) * d.n + x[j].v);
+      const uint64_t pv = base[bv[j]] + atomicAdd(&cnt[bv[j]], 1u);
+      d.ew[pv] = x[j].w;
+      d.eli[pv] = (uint16_t)((x[j].v - bv[j] * R) * d.n + x[j].u);
+    }
   }
 }
 
-// Row sums (the weighted degrees), one wavefront per row.
-__global__ void __launch_bounds__(256) k_sp_rowsum(const SpDense* __restrict__ nodes) {
+// Exclusive prefix of the band counts, one workgroup per node: boff[b] becomes
+// the band's first entry, boff[bands] the total, and the cursors start there.
+__global__ void __launch_bounds__(1024) k_sp_band_scan(const SpDense* __restrict__ nodes) {
+  __shared__ unsigned long long wtot[16];
+  const SpDense d = nodes[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long carry = 0;
+  for (uint32_t b0 = 0; b0 < d.bands; b0 += 1024) {
+    const uint32_t b = b0 + threadIdx.x;
+    const unsigned long long c = b < d.bands ? d.boff[b] : 0ull;
+    const unsigned long long inc = wave_incl_scan<uint64_t>(c);
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    unsigned long long off = carry, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) off += wtot[w];
+      tot += wtot[w];
+    }
+    if (b < d.bands) {
+      d.boff[b] = off + inc - c;
+      d.bcur[b] = off + inc - c;
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.boff[d.bands] = carry;
+}
+
+// Sums each band's entries in LDS, then writes the band's row sums and its
+// rows (every entry, zeros included) and clears the LDS it read.  Workgroups
+// stride over a node's bands.  Dynamic LDS: kSpBandEntries u64.
+__global__ void __launch_bounds__(kSpBuildBlock) k_sp_band_build(const SpDense* __restrict__ nodes) {
+  extern __shared__ unsigned long long band_acc[];
   const SpDense d = nodes[blockIdx.y];
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6); r < d.n; r += gridDim.x * 4) {
-    const int64_t* row = d.mat + (uint64_t)r * d.n;
-    uint64_t acc = 0;
-    for (uint32_t x = lane; x < d.n; x += 64) acc += (uint64_t)row[x];
-    acc = wave_sum<uint64_t>(acc);
-    if (lane == 0) d.rsum[r] = (int64_t)acc;
+  const uint32_t n = d.n, R = d.rows;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < R * n; i += kSpBuildBlock) band_acc[i] = 0;
+  for (uint32_t b = blockIdx.x; b < d.bands; b += gridDim.x) {
+    __syncthreads();  // LDS clear before the band's sums
+    const uint64_t i1 = d.boff[b + 1];
+    for (uint64_t i = d.boff[b] + threadIdx.x; i < i1; i += kSpBuildBlock) atomicAdd(&band_acc[d.eli[i]], d.ew[i]);
+    __syncthreads();
+    const uint32_t r0 = b * R, nr = min(R, n - r0);
+    for (uint32_t r = wave; r < nr; r += kSpBuildBlock / 64) {
+      uint64_t acc = 0;
+      for (uint32_t c = lane; c < n; c += 64) acc += band_acc[r * n + c];
+      acc = wave_sum<uint64_t>(acc);
+      if (lane == 0) d.rsum[r0 + r] = (int64_t)acc;
+    }
+    __syncthreads();
+    int64_t* out = d.mat + (uint64_t)r0 * n;
+    for (uint32_t j = threadIdx.x; j < nr * n; j += kSpBuildBlock) {
+      out[j] = (int64_t)band_acc[j];
+      band_acc[j] = 0;
+    }
   }
 }
 
@@ -637,7 +741,28 @@ size_t child_tmp_bytes(const SpNode& nd) {
          carve_size((nd.m + kSpTile - 1) / kSpTile * 4);
 }
 size_t dense_bytes(const SpNode& nd) {
-  return carve_size((size_t)nd.n * nd.n * 8) + carve_size((size_t)nd.n * 8) + sizeof(SpDense);
+  const uint32_t bands = (nd.n + band_rows(nd.n) - 1) / band_rows(nd.n);
+  return carve_size((size_t)nd.n * nd.n * 8) + carve_size((size_t)nd.n * 8) + sizeof(SpDense) +
+         ((size_t)bands + 1) * 8 + 256 + carve_size((size_t)bands * 8) + carve_size(nd.m * 16) + carve_size(nd.m * 4);
+}
+
+// The bucketing and band kernels' dynamic LDS may exceed the default limit:
+// raise it once.
+int band_attr(cac_ctx* ctx) {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    err = hipFuncSetAttribute((const void*)k_sp_band_bucket<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kSpBucketLdsMax);
+    if (err == hipSuccess)
+      err = hipFuncSetAttribute((const void*)k_sp_band_bucket<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kSpBucketLdsMax);
+    if (err == hipSuccess)
+      err = hipFuncSetAttribute((const void*)k_sp_band_build, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(kSpBandEntries * 8));
+  });
+  CAC_HIP(ctx, err);
+  return CAC_OK;
 }
 
 struct SpRun {
@@ -670,28 +795,58 @@ struct SpRun {
       const size_t count = i1 - i0;
       rc = grow_buffer(ctx, &ctx->sp_dense_buf, &ctx->sp_dense_bytes, mats);
       if (rc) return rc;
+      rc = band_attr(ctx);
+      if (rc) return rc;
       BufPin pin(ctx, kBufSpDense);
       Carver cv(ctx->sp_dense_buf);
       SpDense* d_nodes = cv.take<SpDense>(count);
+      uint64_t nboff = 0;
+      uint32_t bmax = 0;
+      for (size_t i = 0; i < count; ++i) {
+        const uint32_t n = nodes[i0 + i].n;
+        if (n > kKsMaxS) return fail(ctx, CAC_EINVAL, "sparse densify: node of " + std::to_string(n) + " vertices");
+        const uint32_t bands = (n + band_rows(n) - 1) / band_rows(n);
+        nboff += bands + 1;
+        bmax = std::max(bmax, bands);
+      }
+      unsigned long long* d_boff = cv.take<unsigned long long>(nboff);
       std::vector<SpDense> hd(count);
       std::vector<KsRoot> roots;
-      uint64_t mmax = 0;
+      uint64_t mmax = 0, boff_at = 0;
       for (size_t i = 0; i < count; ++i) {
         const SpNode& nd = nodes[i0 + i];
-        hd[i] = SpDense{nd.edges, nd.m, nd.n, 0, cv.take<int64_t>((size_t)nd.n * nd.n), cv.take<int64_t>(nd.n)};
-        KsRoot r{hd[i].mat, hd[i].rsum, nd.n, nd.trial, nd.slot, nd.bound, nd.seed};
+        const uint32_t R = band_rows(nd.n), bands = (nd.n + R - 1) / R;
+        SpDense& d = hd[i];
+        d.edges = nd.edges;
+        d.m = nd.m;
+        d.n = nd.n;
+        d.rows = R;
+        d.bands = bands;
+        d.pad = 0;
+        d.mat = cv.take<int64_t>((size_t)nd.n * nd.n);
+        d.rsum = cv.take<int64_t>(nd.n);
+        d.boff = d_boff + boff_at;
+        boff_at += bands + 1;
+        d.bcur = cv.take<unsigned long long>(bands);
+        d.ew = cv.take<unsigned long long>(2 * nd.m);
+        d.eli = cv.take<uint16_t>(2 * nd.m);
+        KsRoot r{d.mat, d.rsum, nd.n, nd.trial, nd.slot, nd.bound, nd.seed};
         r.nid = nd.nid;
         roots.push_back(r);
         mmax = std::max(mmax, nd.m);
       }
       CAC_HIP(ctx, hipMemcpyAsync(d_nodes, hd.data(), count * sizeof(SpDense), hipMemcpyHostToDevice, s));
-      const unsigned gz = grid_for((uint64_t)nmax * nmax / 2, 256 * 16, 1024);
-      const unsigned gs = grid_for(mmax, 256 * 4, 1024);
-      const unsigned gr = grid_for(nmax, 4, 1024);
+      const unsigned tiles = (unsigned)std::max<uint64_t>(1, (mmax + kSpBucketTile - 1) / kSpBucketTile);
+      const unsigned gb = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(bmax, (2048 + count - 1) / count));
       prof_begin(ctx, "sparse_densify", s);
-      hipLaunchKernelGGL(k_sp_zero, dim3(gz, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
-      hipLaunchKernelGGL(k_sp_scatter, dim3(gs, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
-      hipLaunchKernelGGL(k_sp_rowsum, dim3(gr, (unsigned)count), dim3(256), 0, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_fill, dim3(grid_for(nboff, 256, 1024)), dim3(256), 0, s, d_boff, nboff, 0ull);
+      hipLaunchKernelGGL(k_sp_band_bucket<false>, dim3(tiles, (unsigned)count), dim3(kSpBucketBlock),
+                         (size_t)bmax * 4, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_band_scan, dim3((unsigned)count), dim3(1024), 0, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_band_bucket<true>, dim3(tiles, (unsigned)count), dim3(kSpBucketBlock),
+                         (size_t)bmax * 12, s, (const SpDense*)d_nodes);
+      hipLaunchKernelGGL(k_sp_band_build, dim3(gb, (unsigned)count), dim3(kSpBuildBlock), (size_t)kSpBandEntries * 8,
+                         s, (const SpDense*)d_nodes);
       prof_end(ctx, s);
       CAC_LAUNCH_CHECK(ctx, "sparse densify");
       rc = ks_run(ctx, roots, d_results, s);
