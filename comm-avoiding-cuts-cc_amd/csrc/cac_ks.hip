@@ -337,31 +337,45 @@ __device__ inline void load_row(const int64_t* row, uint32_t cc, uint32_t s, int
   }
 }
 
+template <int B>
+__device__ inline void store_row(int64_t* row, uint32_t cc, uint32_t s, const int64_t (&v)[kKsChunk]) {
+#pragma unroll
+  for (uint32_t k = 0; k < kKsChunk; ++k) {
+    const uint32_t x = k * B + threadIdx.x;
+    if (k < cc && x < s) row[x] = v[k];
+  }
+}
+
 // Merge row a into row b (a dies, b keeps its index).  The calling threads hold
-// their chunk of row `held` (a or b) in v, and of the other row in u when
-// u_has_other (the caller keeps the last merged row there); otherwise the other
-// row is loaded into u here.  On return u holds the merged row (0 past s).  The
-// merged row goes to b's pool row, else a's (a dies, and every thread rewrites
-// exactly the entries it read), else the child's spare pool row (misc[5]),
-// which thread 0 then replaces with a fresh one.  Every thread derives that
-// destination from the same LDS words; thread 0 changes them only after the
-// block sum's barrier, which every thread reaches after its reads, and the
-// merge ends with one more barrier.  Each thread relabels only the
-// representatives of the columns it owns.  Returns the contracted weight
-// (cap[a] + cap[b] - cap_new) / 2 in w; false when the pool is exhausted.
-// Called by the whole block.
+// their chunk of row `held` (a or b) in v, and in u the last merged row, row
+// `cached` (kNoSlot before the first merge), which is the other row when
+// u_has_other.  That row is written back lazily: its pool row is stale until it
+// leaves the registers.  Otherwise the other row is loaded into u here, after
+// row `cached` is written to its pool row if it is still live and not a or b
+// (merged into b's new row or dead, it needs no write).  On return u holds the
+// merged row (0 past s), not yet written.  Its pool row is b's, else a's (a
+// dies), else the child's spare pool row (misc[5]), which thread 0 then
+// replaces with a fresh one.  Every thread derives that destination from the
+// same LDS words; thread 0 changes them only after the block sum's barrier,
+// which every thread reaches after its reads, and the merge ends with one more
+// barrier.  Each thread relabels only the representatives of the columns it
+// owns.  Returns the contracted weight (cap[a] + cap[b] - cap_new) / 2 in w;
+// false when the pool is exhausted.  Called by the whole block.
 template <int B>
 __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, uint32_t a, uint32_t b, uint32_t held,
-                         const int64_t (&v)[kKsChunk], int64_t (&u)[kKsChunk], bool u_has_other, uint32_t cc,
-                         int64_t& w) {
+                         const int64_t (&v)[kKsChunk], int64_t (&u)[kKsChunk], uint32_t cached, bool u_has_other,
+                         uint32_t cc, int64_t& w) {
   const uint32_t s = c.s;
   const uint32_t other = held == a ? b : a;
   const uint32_t slot_a = L.wslot[a], slot_b = L.wslot[b];
-  if (!u_has_other) load_row<B>(ks_row(c, pool, other == a ? slot_a : slot_b, other), cc, s, u);
   const bool spare = slot_b == kNoSlot && slot_a == kNoSlot;
   const uint32_t db = slot_b != kNoSlot ? slot_b : slot_a != kNoSlot ? slot_a : L.misc[5];
   if (db == kNoSlot) return false;  // only a spare can be missing: the pool is exhausted
-  int64_t* dst = pool.base + (size_t)db * kKsUnit;
+  if (!u_has_other) {
+    if (cached != kNoSlot && cached != a && cached != b)  // a merged row has a pool row
+      store_row<B>(pool.base + (size_t)L.wslot[cached] * kKsUnit, cc, s, u);
+    load_row<B>(ks_row(c, pool, other == a ? slot_a : slot_b, other), cc, s, u);
+  }
   uint64_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kKsChunk; ++k) {
@@ -371,7 +385,6 @@ __device__ bool ks_merge(const KsChild& c, const KsPool& pool, const KsLds& L, u
       const uint32_t rx = L.rep[x];
       if (rx == a || rx == b) t = 0;  // loop of the merged vertex
       if (rx == a) L.rep[x] = b;
-      dst[x] = t;  // each thread reads back only its own chunk later
       u[k] = t;
       sum += (uint64_t)t;
     }
@@ -479,7 +492,7 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 5)
     const uint32_t j = L.misc[2];  // rep[k], read by the selecting thread before the merge relabels
     const uint32_t a = i > j ? i : j, b = i > j ? j : i;
     int64_t w = 0;
-    if (!ks_merge<B>(c, pool, L, a, b, i, v, u, j == cb, cc, w)) {
+    if (!ks_merge<B>(c, pool, L, a, b, i, v, u, cb, j == cb, cc, w)) {
       status = 3;
       break;
     }
@@ -508,7 +521,7 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 5)
       // bound, so this merge's relabelling (of class i) leaves it unchanged
       int64_t w = 0;
       const uint32_t tgt = L.rep[first];
-      if (!ks_merge<B>(c, pool, L, i, tgt, i, v, u, tgt == cb, cc, w)) {
+      if (!ks_merge<B>(c, pool, L, i, tgt, i, v, u, cb, tgt == cb, cc, w)) {
         status = 3;
         break;
       }
@@ -516,6 +529,7 @@ __global__ void __launch_bounds__(B, B >= 1024 ? 1 : 5)
       --nv;
     }
   }
+  if (status == 0 && cb != kNoSlot) store_row<B>(pool.base + (size_t)L.wslot[cb] * kKsUnit, cc, s, u);
   if (status != 0) {
     if (threadIdx.x == 0) {
       k_out[blockIdx.x] = status == 1 ? 0u : status == 2 ? kKsBad : kKsOverflow;

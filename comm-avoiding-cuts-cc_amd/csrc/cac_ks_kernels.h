@@ -11,8 +11,10 @@
 //   - every thread owns the same entries of every row for the whole kernel,
 //     interleaved (thread t of B owns x = k B + t, k < 8), so each load or store
 //     instruction of a wavefront covers 512 contiguous bytes of a row; the picked
-//     row stays in registers for the column pick and the merge, and a thread
-//     only ever reads back what it wrote itself;
+//     row stays in registers for the column pick and the merge, the last merged
+//     row stays there too and is written to its pool row only when another
+//     merged row takes its place while it is live, or when the child ends, and
+//     a thread only ever reads back what it wrote itself;
 //   - a PR pass (co_mincut.h:50-63) may follow; the kernel ends by writing the
 //     column map and the list of live rows, and a separate many-workgroup kernel
 //     compacts the live rows into the child's t x t matrix with its row sums and
