@@ -367,6 +367,10 @@ constexpr int kDensifyEdges = 4;  // edges per thread in flight
 // matrix (k == t unless the sampling stopped early on a disconnected graph),
 // with its row sums in LDS, and compute the trial's bound (minimum weighted
 // degree).  bounds is indexed by job (blockIdx.x), results by the job's slot.
+// kRankLabels: the labels are replaced by their ranks before the accumulation,
+// so an edge endpoint costs one dependent load instead of two (label, then
+// rank); for callers that do not read the labels afterwards.
+template <bool kRankLabels>
 __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch* __restrict__ scratch, int64_t* bounds,
                                                               int64_t* results) {
   __shared__ uint32_t wtot[kSmpWaves];
@@ -402,6 +406,10 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
     carry += tot;
     __syncthreads();
   }
+  if constexpr (kRankLabels) {
+    block_sync_global();  // every rank written
+    for (uint32_t x = threadIdx.x; x < n; x += kSmpBlock) S.lab[x] = rank[S.lab[x]];
+  }
   for (uint64_t i = threadIdx.x; i < (uint64_t)k * k; i += kSmpBlock) S.mat[i] = 0;
   const bool lds_rows = k <= kKsMaxS;  // else the row sums are read back from the matrix
   for (uint32_t r = threadIdx.x; lds_rows && r < k; r += kSmpBlock) row_sum[r] = 0;
@@ -420,10 +428,12 @@ __global__ void __launch_bounds__(kSmpBlock) k_densify_trials(const TrialScratch
       a[j] = S.lab[x[j].u];
       b[j] = S.lab[x[j].v];
     }
+    if constexpr (!kRankLabels) {
 #pragma unroll
-    for (int j = 0; j < kDensifyEdges; ++j) {
-      a[j] = rank[a[j]];
-      b[j] = rank[b[j]];
+      for (int j = 0; j < kDensifyEdges; ++j) {
+        a[j] = rank[a[j]];
+        b[j] = rank[b[j]];
+      }
     }
 #pragma unroll
     for (int j = 0; j < kDensifyEdges; ++j) {
@@ -581,8 +591,11 @@ int densify_attr(cac_ctx* ctx) {
   static std::once_flag once;
   static hipError_t err = hipSuccess;
   std::call_once(once, [] {
-    err = hipFuncSetAttribute((const void*)k_densify_trials, hipFuncAttributeMaxDynamicSharedMemorySize,
+    err = hipFuncSetAttribute((const void*)k_densify_trials<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)kDensifyLds);
+    if (err == hipSuccess)
+      err = hipFuncSetAttribute((const void*)k_densify_trials<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kDensifyLds);
   });
   CAC_HIP(ctx, err);
   return CAC_OK;
@@ -1051,7 +1064,7 @@ static int sqrt_dense_chunks(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int
       prof_begin(ctx, "densify_trials", s);
       if (rc == CAC_OK) rc = densify_attr(ctx);
       if (rc == CAC_OK)
-        hipLaunchKernelGGL(k_densify_trials, dim3((unsigned)nb), dim3(kSmpBlock), kDensifyLds, s,
+        hipLaunchKernelGGL(k_densify_trials<true>, dim3((unsigned)nb), dim3(kSmpBlock), kDensifyLds, s,
                            (const TrialScratch*)d_scr, d_bounds, d_results);
       prof_end(ctx, s);
       e = hipGetLastError();
@@ -1166,7 +1179,7 @@ int sample_contract_one(cac_ctx* ctx, const cac_edge* d_edges, uint64_t m, uint3
   CAC_HIP(ctx, hipMemcpyAsync(d_scr, &h, sizeof h, hipMemcpyHostToDevice, s));
   rc = densify_attr(ctx);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_densify_trials, dim3(1), dim3(kSmpBlock), kDensifyLds, s, (const TrialScratch*)d_scr, d_misc,
+  hipLaunchKernelGGL(k_densify_trials<false>, dim3(1), dim3(kSmpBlock), kDensifyLds, s, (const TrialScratch*)d_scr, d_misc,
                      d_misc + 1);
   CAC_LAUNCH_CHECK(ctx, "densify trial");
   CAC_HIP(ctx, hipStreamSynchronize(s));
