@@ -31,6 +31,31 @@ def planted(rng, n: int, deg: float, wmax: int, c: int):
     raise RuntimeError(f"planted: no valid graph for n={n} deg={deg} c={c}")
 
 
+def necklace(rng, n: int, k: int, heavy: int, deg_in: float):
+    """k clusters of about n/k vertices in a ring.  Inside a cluster a path plus
+    about deg_in * size / 2 random chords, all of weight `heavy`; one edge of
+    weight U[10^6, 1.1 * 10^6) joins each cluster to the next.  With heavy at
+    or above the minimum weighted degree, contracted heavy edges reach the
+    Karger-Stein bound (so the PR pass merges), while the ring's many near-equal
+    2-cuts below it make the trials' raw cuts differ."""
+    g = n // k
+    e = []
+    for c in range(k):
+        base = c * g
+        size = g if c < k - 1 else n - base
+        for x in range(1, size):
+            e.append((base + x - 1, base + x, heavy))
+        for _ in range(int(deg_in * size / 2)):
+            a, b = rng.integers(0, size, 2)
+            if a != b:
+                e.append((base + int(a), base + int(b), heavy))
+    for c in range(k):
+        a = c * g + int(rng.integers(0, g))
+        b = ((c + 1) % k) * g + int(rng.integers(0, g))
+        e.append((a, b, int(rng.integers(10**6, 11 * 10**5))))
+    return np.array(e, dtype=np.int64)
+
+
 def erdos_renyi(rng, n: int, p: float, wmax: int = 1):
     """G(n, p) with weights U[1, wmax], redrawn until connected."""
     for _ in range(1000):

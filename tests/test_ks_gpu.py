@@ -5,7 +5,7 @@ import sys
 
 import numpy as np
 import pytest
-from graphgen import dense, erdos_renyi, planted
+from graphgen import dense, erdos_renyi, necklace, planted
 
 import cac
 
@@ -46,6 +46,23 @@ def test_ks_pr_pass(ctx, orc, n):
     want = [orc.ks_trial(A, 5, t) for t in range(8)]
     assert raw.tolist() == want
     assert mn == orc.stoer_wagner(A)
+
+
+@pytest.mark.parametrize("n,k,deg_in", [(900, 225, 2.0), (1500, 375, 2.0), (3000, 600, 3.0)])
+def test_ks_pr_pass_pins_trials(ctx, orc, n, k, deg_in):
+    """Rings of small heavy clusters: the PR pass merges inside clusters and the
+    trials' raw cuts differ, so a PR pass that stops early or is skipped changes
+    them (checked on the CPU with the oracle's PR pass cut short after its
+    first merge, and with it removed).  Children of 900, 1500 and 3000
+    vertices take the 128-, 256- and 512-thread contractions."""
+    rng = np.random.default_rng(n + k)
+    A = dense(n, necklace(rng, n, k, 3 * 10**6, deg_in))
+    T = 8
+    _say(f"ks pr pass necklace n {n} clusters {k}")
+    want = [orc.ks_trial(A, 5, t) for t in range(T)]
+    assert len(set(want)) > 1
+    raw, mn = ctx.ks_dense(A, 5, 0, T)
+    assert raw.tolist() == want
 
 
 def test_ks_disconnected_is_zero(ctx, orc):

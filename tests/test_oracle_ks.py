@@ -4,7 +4,7 @@ distributions against the textbook (chi-square), raw trial cuts against exact
 Stoer-Wagner, final cuts against the reference's own KS and deterministic cut."""
 import numpy as np
 import pytest
-from graphgen import dense, erdos_renyi, planted
+from graphgen import dense, erdos_renyi, necklace, planted
 from scipy.stats import chisquare
 
 F32_95 = float(np.float32(0.95))  # the reference parses p with std::stof
@@ -105,6 +105,22 @@ def test_ks_trial_raw_cut_bounds(orc, seed):
         assert exact <= raw <= bound
         hits += raw == exact
     assert hits >= 2  # per-trial success probability is far above 1/3 here
+
+
+def test_necklace_trials_vary_and_stay_above_the_cut(orc):
+    """The graph family the device test of the PR pass uses: the oracle's raw
+    cuts differ between trials (so they pin the order of merges, PR merges
+    included), every one lies between the exact minimum cut and the minimum
+    weighted degree, and the heavy edges reach that degree (so the PR pass
+    merges)."""
+    rng = np.random.default_rng(900 + 225)
+    A = dense(900, necklace(rng, 900, 225, 3 * 10**6, 2.0))
+    exact = orc.stoer_wagner(A)
+    bound = orc.min_degree(A)
+    assert bound <= 3 * 10**6
+    raw = [orc.ks_trial(A, 5, t) for t in range(8)]
+    assert len(set(raw)) > 1
+    assert all(exact <= r <= bound for r in raw)
 
 
 @pytest.mark.parametrize("seed", range(3))
