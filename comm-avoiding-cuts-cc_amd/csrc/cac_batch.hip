@@ -297,7 +297,7 @@ int cac_mincut_batch_dev(cac_ctx* ctx, const cac_edge* d_edges, const cac_graph_
   hipStream_t s = cac::pick_stream(ctx, stream);
   int64_t* d_res = nullptr;
   CAC_HIP(ctx, hipMalloc((void**)&d_res, 8 * jobs.size()));
-  rc = cac::sqrt_cut_batch(ctx, jobs, d_res, s);
+  rc = cac::sqrt_cut_batch(ctx, jobs, d_res, s, 0, true);
   if (rc == CAC_OK) {
     std::vector<int64_t> res(jobs.size());
     hipError_t e = hipMemcpyAsync(res.data(), d_res, 8 * res.size(), hipMemcpyDeviceToHost, s);

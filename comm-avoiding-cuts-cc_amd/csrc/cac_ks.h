@@ -48,8 +48,10 @@ struct SqrtJob {
 constexpr size_t kMaxJobsPerChunk = 8192;
 // Jobs of any mix of graphs, in chunks of shared launches whose estimated
 // device bytes stay within budget (0: 40% of the free device memory).
+// by_size: the workgroup-form jobs are chunked largest graph first (see the
+// definition for when), not in the order given.
 int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
-                   size_t budget = 0);
+                   size_t budget = 0, bool by_size = false);
 // Estimated device bytes of one job: sampling scratch (dpr: draws per round,
 // 0 = |V|), Karger-Stein levels.
 size_t job_sample_bytes(const SqrtJob& j, uint32_t dpr = 0);

@@ -1094,7 +1094,7 @@ static int sqrt_dense_chunks(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int
 // Runs sqrt-cut jobs (graph, trial) in chunks sized to the free device memory.
 // Raw cut of job i goes to d_results[jobs[i].slot].
 int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_results, hipStream_t s,
-                   size_t budget) {
+                   size_t budget, bool by_size) {
   if (jobs.empty()) return CAC_OK;
   bool any_large = false;
   for (const SqrtJob& j : jobs) any_large = any_large || j.t > kKsMaxS;
@@ -1103,7 +1103,7 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
     for (const SqrtJob& j : jobs) (j.t <= kKsMaxS ? dense : large).push_back(j);
     const int rc = sqrt_large_batch(ctx, large, d_results, s);
     if (rc) return rc;
-    return sqrt_cut_batch(ctx, dense, d_results, s, budget);
+    return sqrt_cut_batch(ctx, dense, d_results, s, budget, by_size);
   }
   if (budget == 0) {
     size_t avail = 0;
@@ -1115,6 +1115,18 @@ int sqrt_cut_batch(cac_ctx* ctx, const std::vector<SqrtJob>& jobs, int64_t* d_re
   const uint64_t wmin = wide_cut(ctx, jobs);
   std::vector<SqrtJob> narrow, wide;
   for (const SqrtJob& j : jobs) (j.m >= wmin ? wide : narrow).push_back(j);
+  // by_size: chunks of trials of like size when no workgroup-form trial reaches
+  // the wide form's floor.  A chunk of small graphs then has a small label
+  // array, its densify launch holds matrices of like size and its Karger-Stein
+  // levels children of like size; results go by slot.  Above the floor the
+  // wide cut has counted on the large trials sharing chunks with small ones.
+  if (by_size) {
+    const uint64_t floor_m = wide_min_edges(ctx);
+    bool below_floor = true;
+    for (const SqrtJob& j : narrow) below_floor = below_floor && j.m < floor_m;
+    if (below_floor)
+      std::stable_sort(narrow.begin(), narrow.end(), [](const SqrtJob& a, const SqrtJob& b) { return a.n > b.n; });
+  }
   int rc = sqrt_dense_chunks(ctx, wide, d_results, s, budget, true);
   if (rc) return rc;
   return sqrt_dense_chunks(ctx, narrow, d_results, s, budget, false);
