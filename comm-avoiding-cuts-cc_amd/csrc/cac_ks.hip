@@ -1089,15 +1089,15 @@ int ks_dense_run(cac_ctx* ctx, const int64_t* d_adj, uint32_t n, uint32_t seed, 
     hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ntrials, 256, 64)), dim3(256), 0, s, d_res, ntrials, bound);
     // trials run in batches whose two live levels fit in 3/4 of the free
     // memory: the top levels are chains of picks with one workgroup per child,
-    // so the more trials a batch holds, the more CUs they keep busy (n = 8192:
-    // 169 trials in two batches, not three)
+    // so the more trials a batch holds, the more CUs they keep busy.  Every
+    // batch but the last is as large as that allows: at n = 8192 the first of
+    // two then fills the CUs at the levels where children outnumber them (two
+    // even batches of 85 left a partial round of workgroups in each)
     size_t avail = 0;
     rc = device_memory_available(ctx, &avail);
     const size_t per_trial = ks_live_bytes_per_root(n);
     size_t batch = std::max<size_t>(1, (size_t)(0.75 * (double)avail) / per_trial);
     batch = std::min<size_t>(batch, ntrials);
-    const size_t nb = (ntrials + batch - 1) / batch;
-    batch = (ntrials + nb - 1) / nb;  // even batches
     for (uint32_t t0 = 0; t0 < ntrials && rc == CAC_OK; t0 += (uint32_t)batch) {
       std::vector<KsRoot> roots;
       for (uint32_t t = t0; t < std::min<uint32_t>(ntrials, t0 + (uint32_t)batch); ++t)
