@@ -1,4 +1,5 @@
 // cac_ctx.hip -- context lifecycle, errors, device buffers and kernel timing.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -113,9 +114,10 @@ int grow_buffer(cac_ctx* ctx, void** buf, size_t* cap, size_t bytes) {
   if (*buf) CAC_HIP(ctx, hipFree(*buf));
   *buf = nullptr;
   *cap = 0;
-  // 1/8 headroom, so a later call a little larger than this one does not
-  // reallocate (a large hipFree + hipMalloc costs far more than the memory)
-  const size_t want = bytes + bytes / 8;
+  // 1/8 headroom (1/2 with grow_wide), at least 1 MiB, so a later call a
+  // little larger than this one does not reallocate (a large hipFree +
+  // hipMalloc costs far more than the memory: the free waits for the device)
+  const size_t want = std::max<size_t>(bytes + (ctx->grow_wide ? bytes / 2 : bytes / 8), (size_t)1 << 20);
   if (hipMalloc(buf, want) == hipSuccess) {
     *cap = want;
     return CAC_OK;

@@ -60,6 +60,10 @@ struct cac_ctx {
   // released when an allocation fails.
   std::vector<int> pins;
   size_t smp_wave = 0;  // sampling workgroups resident at once (0: not asked yet)
+  // Grow-only buffers regrow by half again rather than an eighth: for contexts
+  // whose chunks vary in size from call to call (serving workers), where each
+  // regrowth's hipFree waits for the whole device.
+  bool grow_wide = false;
   // Tuned launch shapes (cac_tune_set): dense Karger-Stein contraction block
   // size per row-size class, keys 512, 1024, 2048, 4096; 0 = built-in rule.
   int32_t ks_contract_block[4] = {0, 0, 0, 0};

@@ -486,6 +486,7 @@ int cac_server_create(int device, uint32_t workers, uint64_t hbm_budget, uint32_
       delete S;
       return rc;
     }
+    c->grow_wide = true;  // groups differ from call to call: regrow with room to spare
     S->ctxs.push_back(c);
   }
   for (uint32_t w = 0; w < workers; ++w) S->workers.emplace_back(cac::worker_loop, S, S->ctxs[w]);
