@@ -121,6 +121,24 @@ def test_small_graph_equals_dense_engine(ctx, n):
     assert list(raw_s) == list(raw_d)
 
 
+@pytest.mark.parametrize("n", [1537, 4099])
+def test_handover_sums_parallel_edges(ctx, n):
+    """A node handed to the dense engine at once (n <= 2048, or m >= n^2/32 at
+    n <= 8192) is built band by band in LDS: n = 1537 leaves a last band of
+    2 rows (bands of 5), n = 4099 has one row per band.  Repeated edges in
+    both endpoint orders must add up in the matrix exactly as the dense
+    engine's own input: raw trials equal ks_dense on the summed matrix."""
+    rng = np.random.default_rng(n)
+    if n < 2048:
+        E = _perturb(rng, planted(rng, n, 8.0, 9, 2))
+    else:
+        E = _noisy(rng, n, n * n // 20, 7)  # m >= n^2/32: handed over by density
+    A = dense(n, _canon(E))
+    raw_s, _ = ctx.ks_sparse(E, n, 4, 0, 3)
+    raw_d, _ = ctx.ks_dense(A, 4, 0, 3)
+    assert list(raw_s) == list(raw_d)
+
+
 def test_disconnected_is_zero(ctx, orc):
     rng = np.random.default_rng(3)
     a = _noisy(rng, 2500, 5000, 3)
